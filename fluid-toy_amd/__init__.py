@@ -1,0 +1,27 @@
+"""fluid-toy_amd — MI355X (gfx950) HIP implementation of fluid-toy's SPH step.
+
+The compute library is libfluidhip.so (C ABI, include/fluid_abi.h); this
+package is its Python host mirror: `SphSimulation3DGrid` follows the
+reference's struct of the same name (src/simulation/sph_3d_grid.rs), and
+`scenes` holds the seeded workloads.  Every call goes to the HIP library;
+there is no CPU fallback, and a missing library raises FileNotFoundError.
+"""
+from . import abi, scenes
+from .sim import (SphSimulation3DGrid, derive_container_geometry, derive_sph_params,
+                  gravity)
+
+
+def make_sim(scene, device: int = 0, max_particles: int = 0) -> SphSimulation3DGrid:
+    """Create a handle for a scenes.Scene with its parameters pushed the way
+    the app does each frame (src/app.rs:963-1019)."""
+    params, cont, g = scenes.make_params(scene, abi, derive_sph_params,
+                                         derive_container_geometry)
+    sim = SphSimulation3DGrid(scene.particles, params, cont, max_particles=max_particles,
+                              bounds_extent=scene.bounds_extent, device=device)
+    sim.set_pcisph_iterations(scene.pcisph_iterations)
+    sim.update_gravity(g)
+    return sim
+
+
+__all__ = ["abi", "scenes", "SphSimulation3DGrid", "derive_sph_params",
+           "derive_container_geometry", "gravity", "make_sim"]

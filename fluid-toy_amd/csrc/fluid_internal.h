@@ -1,0 +1,153 @@
+// fluid_internal.h — device-side data layout of the SPH step and the launch
+// entry points shared by the kernels and the host orchestration.
+//
+// Layout in HBM (all float4 arrays 16-B aligned, one element per slot):
+//   Particle state lives permanently in the cell-sorted order produced by the
+//   last reorder ("slot order"); the original particle index of each slot is
+//   `sid[slot]`.  The reference's canonical array (original index order,
+//   src/simulation/sph_3d_grid.rs:196-201) is materialised only at the ABI
+//   edge, by scattering through `sid`.
+//     pos, vel, frc   canonical position / velocity / force (.w unused)
+//     spos, svel, sfrc  the reference's sorted copy (sorted_particle_buffer)
+//     sdens           sorted density;  snrm = (normal.xyz, near_density)
+//     pred[2]         sorted_predicted_{a,b} split into
+//                       xq = (x*, q) with q = p / max(rho*,1)^2,
+//                       pr = (p, rho*), v = (v*, 0)
+//   Grid: counts / starts (inclusive scan) over G^3 cells; cell_of / slot_of
+//   per old slot; tmp = (old slot, original index) per new slot.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/fluid_abi.h"
+
+static_assert(sizeof(SphParticle3D) == 64, "SphParticle3D is 64 B");
+static_assert(offsetof(SphParticle3D, velocity) == 16, "velocity @16");
+static_assert(offsetof(SphParticle3D, force) == 32, "force @32");
+static_assert(offsetof(SphParticle3D, density) == 44, "density @44");
+static_assert(offsetof(SphParticle3D, near_density) == 48, "near @48");
+static_assert(offsetof(SphParticle3D, normal) == 52, "normal @52");
+static_assert(sizeof(GpuSphParams3D) == 64, "GpuSphParams3D is 64 B");
+static_assert(offsetof(GpuSphParams3D, num_particles) == 44, "n @44");
+static_assert(sizeof(GpuGridParams) == 48, "GpuGridParams is 48 B");
+static_assert(sizeof(GpuContainerGeometry) == 128, "container is 128 B");
+static_assert(offsetof(GpuContainerGeometry, wall_stiffness) == 112, "@112");
+static_assert(sizeof(GpuGravity) == 16, "GpuGravity is 16 B");
+static_assert(sizeof(GpuMouseForce) == 48, "GpuMouseForce is 48 B");
+static_assert(offsetof(GpuMouseForce, direction) == 32, "direction @32");
+static_assert(sizeof(GpuRigidBody) == 96, "GpuRigidBody is 96 B");
+static_assert(offsetof(GpuRigidBody, rot_row0) == 48, "rot @48");
+static_assert(sizeof(GpuRigidBodyAccum) == 32, "accum is 32 B");
+static_assert(sizeof(PredictedState) == 32, "PredictedState is 32 B");
+
+namespace fluid {
+
+constexpr float kPI = 3.14159265359f;  // const PI of every WGSL file
+
+// Uniform block read by every kernel through one pointer (scalar loads).
+// Kernel-constant factors are evaluated on the host in f32 with the same
+// operation order the WGSL uses per invocation, so they are bit-identical.
+struct DevUniforms {
+  // GpuSphParams3D
+  float h, h2, h6, h9, mass, rho0, near_k, visc, dt, st, delta, eps;
+  // hoisted constants
+  float poly6_scale;   // 315 / (64*PI*h9)          sph_density_3d_grid.wgsl:57
+  float near_scale;    // 15 / (PI*h6)                                   :63
+  float grad_scale;    // 45 / (PI*h6)              sph_force_3d_grid.wgsl:64
+  float akinci_scale;  // 32 / (PI*h9)                                   :79
+  float h6_64;         // h6 / 64                                        :86
+  float half_h;        // h * 0.5                                        :82
+  float normal_scale;  // 945 / (32*PI*h8), h8 = h6*h*h  sph_density:177-178
+  float m_over_rho0;   // mass / rest_density                            :161
+  float min_dist;      // h * 0.1                   sph_force_3d_grid.wgsl:162
+  float visc_eps;      // 0.01 * h2                                      :198
+  float max_accel;     // 0.25*h / (dt*dt)          pcisph_solve.wgsl:259
+  float max_accel_sq;  // max_accel^2                                    :261
+  float boundary_layer;  // h * 0.4                 sph_integrate_3d.wgsl:113
+  float interact_range;  // h * 0.7                                      :303
+  // GpuGridParams
+  uint32_t gx, gy, gz, cells, gxy;
+  float inv_cell, ox, oy, oz;
+  // GpuContainerGeometry
+  float hw, hh, hd, cy;
+  float fw[3][4];
+  float iv[3][4];
+  float wall_k, damping;
+  // GpuGravity
+  float grav[3];
+  uint32_t sdf_res;
+  GpuMouseForce mouse;
+  GpuRigidBody rb;
+};
+
+struct PredBuf {
+  float4 *xq;
+  float2 *pr;
+  float4 *v;
+};
+
+struct Bufs {
+  float4 *pos, *vel, *frc;
+  float4 *spos, *svel, *sfrc, *snrm;
+  float *sdens;
+  uint32_t *sid;
+  uint32_t *counts, *starts;
+  uint32_t *cell_of, *slot_of;
+  uint2 *tmp;
+  PredBuf pred[2];
+  unsigned long long *tile_status;
+  uint32_t *tile_counter;
+  uint32_t *err;
+  int32_t *accum;
+  const float *sdf;
+};
+
+// Scan tiling: 256 threads x 4 cells.
+constexpr uint32_t kScanThreads = 256;
+constexpr uint32_t kScanItems = 4;
+constexpr uint32_t kScanTile = kScanThreads * kScanItems;
+
+// Error bits written by kernels into Bufs::err.
+constexpr uint32_t kErrScanSpin = 1u;
+
+// Launch configuration of the neighbour sweeps.
+struct SweepCfg {
+  int lanes;    // lanes cooperating on one particle (1, 2, 4, 8)
+  int threads;  // workgroup size
+};
+
+// Host-side launchers (sph_kernels.hip).  All enqueue on `st`.
+hipError_t launch_clear(const Bufs &b, uint32_t cells, uint32_t ntiles,
+                        hipStream_t st);
+hipError_t launch_build(const DevUniforms *u, const Bufs &b, uint32_t n,
+                        hipStream_t st);
+hipError_t launch_scan(const DevUniforms *u, const Bufs &b, uint32_t cells,
+                       hipStream_t st);
+hipError_t launch_reorder(const DevUniforms *u, const Bufs &b, uint32_t n,
+                          hipStream_t st);
+hipError_t launch_density(const DevUniforms *u, const Bufs &b, uint32_t n,
+                          SweepCfg c, hipStream_t st);
+hipError_t launch_xsph(const DevUniforms *u, const Bufs &b, uint32_t n,
+                       SweepCfg c, hipStream_t st);
+hipError_t launch_force(const DevUniforms *u, const Bufs &b, uint32_t n,
+                        SweepCfg c, hipStream_t st);
+hipError_t launch_predict(const DevUniforms *u, const Bufs &b, uint32_t n,
+                          hipStream_t st);
+hipError_t launch_solve(const DevUniforms *u, const Bufs &b, uint32_t n,
+                        int in_buf, SweepCfg c, hipStream_t st);
+hipError_t launch_finalize(const DevUniforms *u, const Bufs &b, uint32_t n,
+                           int fin_buf, hipStream_t st);
+hipError_t launch_integrate(const DevUniforms *u, const Bufs &b, uint32_t n,
+                            hipStream_t st);
+hipError_t launch_count_pairs(const DevUniforms *u, const Bufs &b, uint32_t n,
+                              unsigned long long *out2, hipStream_t st);
+
+// Host-side parameter helpers (params.cpp).
+void fill_uniforms(DevUniforms &u, const GpuSphParams3D &p,
+                   const GpuGridParams &g, const GpuContainerGeometry &c,
+                   const GpuGravity &grav, const GpuMouseForce &mouse,
+                   const GpuRigidBody &rb, uint32_t sdf_res);
+GpuGridParams make_grid(float h, float bounds_extent, uint32_t n);
+
+}  // namespace fluid

@@ -1,0 +1,711 @@
+// fluid_sim.hip — the C ABI of include/fluid_abi.h: host orchestration of the
+// SPH step (the role of SphSimulation3DGrid, src/simulation/sph_3d_grid.rs).
+//
+// Owns every device buffer and one HIP stream per handle.  A full step is
+// captured once into a HIP graph (re-captured when the particle count or the
+// PCISPH iteration count changes) and replayed per step; uniform updates are
+// stream-ordered copies into one device block that the graph reads, so they
+// never invalidate it.
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "fluid_internal.h"
+
+using namespace fluid;
+
+namespace {
+
+thread_local std::string g_last_error;
+
+int fail(int code, const std::string &msg) {
+  g_last_error = msg;
+  return code;
+}
+
+#define HIP_TRY(expr)                                                         \
+  do {                                                                        \
+    hipError_t _e = (expr);                                                   \
+    if (_e != hipSuccess)                                                     \
+      return fail(FLUID_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(_e)); \
+  } while (0)
+
+constexpr int kUniformRing = 16;
+
+}  // namespace
+
+struct fluid_sim {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  uint32_t n = 0, max_n = 0, iters = 5;
+  GpuSphParams3D params{};
+  GpuGridParams grid{};
+  GpuContainerGeometry cont{};
+  GpuGravity grav{};
+  GpuMouseForce mouse{};
+  GpuRigidBody rb{};
+  uint32_t sdf_res = 0;
+  DevUniforms *d_u = nullptr;
+  DevUniforms *h_ring = nullptr;  // pinned staging ring for uniform copies
+  hipEvent_t ring_ev[kUniformRing] = {};
+  bool ring_used[kUniformRing] = {};
+  int ring_pos = 0;
+  Bufs b{};
+  std::vector<void *> allocs;
+  unsigned long long *d_pairs = nullptr;
+  // canonical density / near density / normal per original index; the step
+  // never writes these canonical fields (sph_density_3d_grid.wgsl:184-193)
+  std::vector<float> extra;
+  hipGraphExec_t graph = nullptr;
+  uint32_t graph_n = 0, graph_iters = 0;
+  int graph_lanes = 0;
+  int cursor = -1;  // last completed stage of a partial step, -1 = none
+  GpuRigidBodyAccum last_accum{};
+  SweepCfg cfg{1, 256};
+  uint32_t ntiles = 0;
+};
+
+namespace {
+
+void *dev_alloc(fluid_sim *s, size_t bytes, hipError_t *err) {
+  void *p = nullptr;
+  *err = hipMalloc(&p, bytes ? bytes : 16);
+  if (*err == hipSuccess) {
+    s->allocs.push_back(p);
+    (void)hipMemset(p, 0, bytes ? bytes : 16);
+  }
+  return p;
+}
+
+SweepCfg pick_cfg(uint32_t n) {
+  // Enough lanes in flight to cover 256 CUs x 4 SIMDs x ~8 waves.
+  SweepCfg c{1, 256};
+  const uint64_t target = 524288;
+  while (c.lanes < 8 && (uint64_t)n * c.lanes < target) c.lanes *= 2;
+  return c;
+}
+
+int push_uniforms(fluid_sim *s) {
+  const int slot = s->ring_pos;
+  s->ring_pos = (s->ring_pos + 1) % kUniformRing;
+  if (s->ring_used[slot]) HIP_TRY(hipEventSynchronize(s->ring_ev[slot]));
+  fill_uniforms(s->h_ring[slot], s->params, s->grid, s->cont, s->grav, s->mouse, s->rb,
+                s->sdf_res);
+  HIP_TRY(hipMemcpyAsync(s->d_u, &s->h_ring[slot], sizeof(DevUniforms),
+                         hipMemcpyHostToDevice, s->stream));
+  HIP_TRY(hipEventRecord(s->ring_ev[slot], s->stream));
+  s->ring_used[slot] = true;
+  return FLUID_OK;
+}
+
+int enqueue_stage(fluid_sim *s, uint32_t stage) {
+  const uint32_t n = s->n;
+  hipStream_t st = s->stream;
+  const Bufs &b = s->b;
+  switch (stage) {
+    case FLUID_STAGE_CLEAR:
+      HIP_TRY(launch_clear(b, s->grid.total_cells, s->ntiles, st));
+      break;
+    case FLUID_STAGE_BUILD:
+      HIP_TRY(launch_build(s->d_u, b, n, st));
+      break;
+    case FLUID_STAGE_SCAN:
+      HIP_TRY(launch_scan(s->d_u, b, s->grid.total_cells, st));
+      break;
+    case FLUID_STAGE_REORDER:
+      HIP_TRY(launch_reorder(s->d_u, b, n, st));
+      break;
+    case FLUID_STAGE_DENSITY:
+      HIP_TRY(launch_density(s->d_u, b, n, s->cfg, st));
+      break;
+    case FLUID_STAGE_XSPH:
+      HIP_TRY(launch_xsph(s->d_u, b, n, s->cfg, st));
+      break;
+    case FLUID_STAGE_FORCE:
+      HIP_TRY(launch_force(s->d_u, b, n, s->cfg, st));
+      break;
+    case FLUID_STAGE_PREDICT:
+      HIP_TRY(launch_predict(s->d_u, b, n, st));
+      break;
+    case FLUID_STAGE_SOLVE:
+      for (uint32_t it = 0; it < s->iters; ++it)
+        HIP_TRY(launch_solve(s->d_u, b, n, (int)(it % 2), s->cfg, st));
+      break;
+    case FLUID_STAGE_FINALIZE:
+      HIP_TRY(launch_finalize(s->d_u, b, n, (s->iters % 2 == 0) ? 0 : 1, st));
+      break;
+    case FLUID_STAGE_INTEGRATE:
+      HIP_TRY(launch_integrate(s->d_u, b, n, st));
+      break;
+    default:
+      return fail(FLUID_ERR_INVALID_ARG, "unknown stage");
+  }
+  return FLUID_OK;
+}
+
+int enqueue_step(fluid_sim *s) {
+  for (uint32_t st = 0; st < FLUID_STAGE_COUNT; ++st) {
+    const int rc = enqueue_stage(s, st);
+    if (rc) return rc;
+  }
+  return FLUID_OK;
+}
+
+void drop_graph(fluid_sim *s) {
+  if (s->graph) (void)hipGraphExecDestroy(s->graph);
+  s->graph = nullptr;
+}
+
+int ensure_graph(fluid_sim *s) {
+  if (s->graph && s->graph_n == s->n && s->graph_iters == s->iters &&
+      s->graph_lanes == s->cfg.lanes)
+    return FLUID_OK;
+  drop_graph(s);
+  hipGraph_t g = nullptr;
+  HIP_TRY(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
+  const int rc = enqueue_step(s);
+  const hipError_t e = hipStreamEndCapture(s->stream, &g);
+  if (rc) {
+    if (g) (void)hipGraphDestroy(g);
+    return rc;
+  }
+  if (e != hipSuccess) return fail(FLUID_ERR_HIP, std::string("capture: ") + hipGetErrorString(e));
+  const hipError_t ie = hipGraphInstantiate(&s->graph, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  if (ie != hipSuccess) {
+    s->graph = nullptr;
+    return fail(FLUID_ERR_HIP, std::string("instantiate: ") + hipGetErrorString(ie));
+  }
+  s->graph_n = s->n;
+  s->graph_iters = s->iters;
+  s->graph_lanes = s->cfg.lanes;
+  return FLUID_OK;
+}
+
+int check_errors(fluid_sim *s) {
+  uint32_t err = 0;
+  HIP_TRY(hipMemcpyAsync(&err, s->b.err, 4, hipMemcpyDeviceToHost, s->stream));
+  HIP_TRY(hipStreamSynchronize(s->stream));
+  if (err & kErrScanSpin) return fail(FLUID_ERR_HIP, "prefix scan look-back exceeded its spin cap");
+  return FLUID_OK;
+}
+
+// Upload particles[0..n) in original-index order: slot i holds index i.
+int upload_canonical(fluid_sim *s, const SphParticle3D *p, uint32_t n) {
+  std::vector<float4> pos(n), vel(n), frc(n);
+  std::vector<uint32_t> sid(n);
+  s->extra.assign((size_t)n * 5, 0.0f);
+  for (uint32_t i = 0; i < n; ++i) {
+    pos[i] = make_float4(p[i].position[0], p[i].position[1], p[i].position[2], 0.f);
+    vel[i] = make_float4(p[i].velocity[0], p[i].velocity[1], p[i].velocity[2], 0.f);
+    frc[i] = make_float4(p[i].force[0], p[i].force[1], p[i].force[2], 0.f);
+    sid[i] = i;
+    float *e = &s->extra[(size_t)i * 5];
+    e[0] = p[i].density;
+    e[1] = p[i].near_density;
+    e[2] = p[i].normal[0];
+    e[3] = p[i].normal[1];
+    e[4] = p[i].normal[2];
+  }
+  if (n) {
+    HIP_TRY(hipMemcpyAsync(s->b.pos, pos.data(), n * 16, hipMemcpyHostToDevice, s->stream));
+    HIP_TRY(hipMemcpyAsync(s->b.vel, vel.data(), n * 16, hipMemcpyHostToDevice, s->stream));
+    HIP_TRY(hipMemcpyAsync(s->b.frc, frc.data(), n * 16, hipMemcpyHostToDevice, s->stream));
+    HIP_TRY(hipMemcpyAsync(s->b.sid, sid.data(), n * 4, hipMemcpyHostToDevice, s->stream));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+  }
+  return FLUID_OK;
+}
+
+template <class T>
+int d2h(fluid_sim *s, std::vector<T> &out, const void *src, size_t count) {
+  out.resize(count);
+  if (count)
+    HIP_TRY(hipMemcpyAsync(out.data(), src, count * sizeof(T), hipMemcpyDeviceToHost, s->stream));
+  return FLUID_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *fluid_last_error(void) { return g_last_error.c_str(); }
+
+int fluid_abi_version(void) { return FLUID_ABI_VERSION; }
+
+int fluid_create(const fluid_create_info *info, fluid_sim **out) {
+  if (!info || !out) return fail(FLUID_ERR_INVALID_ARG, "null argument");
+  *out = nullptr;
+  if (info->num_particles && !info->particles)
+    return fail(FLUID_ERR_INVALID_ARG, "particles is null");
+  const GpuSphParams3D &p = info->sph_params;
+  if (!(p.kernel_radius > 0.0f) || !(p.dt > 0.0f))
+    return fail(FLUID_ERR_INVALID_ARG, "kernel_radius and dt must be > 0");
+  const float extent = info->bounds_extent > 0.0f ? info->bounds_extent : 2.0f;
+  if (info->sdf_voxels && info->sdf_resolution == 0)
+    return fail(FLUID_ERR_INVALID_ARG, "sdf_resolution is 0");
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
+    return fail(FLUID_ERR_NO_DEVICE, "no HIP device");
+  if (info->device < 0 || info->device >= ndev)
+    return fail(FLUID_ERR_NO_DEVICE, "device ordinal out of range");
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, info->device) != hipSuccess)
+    return fail(FLUID_ERR_NO_DEVICE, "hipGetDeviceProperties failed");
+  if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+    return fail(FLUID_ERR_NO_DEVICE, std::string("library is built for gfx950, device is ") +
+                                         prop.gcnArchName);
+  const GpuGridParams grid = make_grid(p.kernel_radius, extent, info->num_particles);
+  const uint64_t cells64 = (uint64_t)grid.grid_size_x * grid.grid_size_y * grid.grid_size_z;
+  if (cells64 > (1ull << 28)) return fail(FLUID_ERR_CAPACITY, "grid exceeds 2^28 cells");
+
+  fluid_sim *s = new fluid_sim();
+  s->device = info->device;
+  s->n = info->num_particles;
+  s->max_n = std::max(info->max_particles, info->num_particles);
+  s->params = p;
+  s->params.num_particles = s->n;
+  s->grid = grid;
+  s->cont = info->container;
+  s->grav = GpuGravity{{0.0f, -9.8f, 0.0f}, 0.0f};  // sph_3d_grid.rs:277-280
+  s->mouse = GpuMouseForce{{0.f, 0.f, 0.f}, 0.5f, 30.0f, 0u, 0u, 0.f, {0.f, 0.f, -1.0f}, 0.f};
+  s->rb = GpuRigidBody{{0.f, 0.f, 0.f}, 0.15f, {0.f, 0.f, 0.f}, 0u, 200.0f, 0u, 0.f, 0.f,
+                       {1.f, 0.f, 0.f, 0.f}, {0.f, 1.f, 0.f, 0.f}, {0.f, 0.f, 1.f, 0.f}};
+  s->iters = 5;  // sph_3d_grid.rs:1253
+  s->cfg = pick_cfg(s->n);
+  s->ntiles = (grid.total_cells + kScanTile - 1) / kScanTile;
+
+  auto bail = [&](int rc) {
+    fluid_destroy(s);
+    return rc;
+  };
+  if (hipSetDevice(s->device) != hipSuccess) return bail(fail(FLUID_ERR_HIP, "hipSetDevice"));
+  if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess)
+    return bail(fail(FLUID_ERR_HIP, "hipStreamCreate"));
+  hipError_t e = hipSuccess;
+  const size_t N = s->max_n, C = grid.total_cells;
+#define ALLOC(field, type, count)                                                 \
+  s->b.field = (type)dev_alloc(s, (size_t)(count) * sizeof(*s->b.field), &e);     \
+  if (e != hipSuccess) return bail(fail(FLUID_ERR_HIP, std::string("hipMalloc " #field ": ") + hipGetErrorString(e)));
+  ALLOC(pos, float4 *, N);
+  ALLOC(vel, float4 *, N);
+  ALLOC(frc, float4 *, N);
+  ALLOC(spos, float4 *, N);
+  ALLOC(svel, float4 *, N);
+  ALLOC(sfrc, float4 *, N);
+  ALLOC(snrm, float4 *, N);
+  ALLOC(sdens, float *, N);
+  ALLOC(sid, uint32_t *, N);
+  ALLOC(counts, uint32_t *, C);
+  ALLOC(starts, uint32_t *, C);
+  ALLOC(cell_of, uint32_t *, N);
+  ALLOC(slot_of, uint32_t *, N);
+  ALLOC(tmp, uint2 *, N);
+  for (int k = 0; k < 2; ++k) {
+    ALLOC(pred[k].xq, float4 *, N);
+    ALLOC(pred[k].pr, float2 *, N);
+    ALLOC(pred[k].v, float4 *, N);
+  }
+  ALLOC(tile_status, unsigned long long *, s->ntiles + 1);
+  ALLOC(tile_counter, uint32_t *, 1);
+  ALLOC(err, uint32_t *, 1);
+  ALLOC(accum, int32_t *, 8);
+#undef ALLOC
+  s->d_pairs = (unsigned long long *)dev_alloc(s, 16, &e);
+  if (e != hipSuccess) return bail(fail(FLUID_ERR_HIP, "hipMalloc pairs"));
+  s->d_u = (DevUniforms *)dev_alloc(s, sizeof(DevUniforms), &e);
+  if (e != hipSuccess) return bail(fail(FLUID_ERR_HIP, "hipMalloc uniforms"));
+  if (info->sdf_voxels) {
+    const size_t nv = (size_t)info->sdf_resolution * info->sdf_resolution * info->sdf_resolution;
+    float *d = (float *)dev_alloc(s, nv * 4, &e);
+    if (e != hipSuccess) return bail(fail(FLUID_ERR_HIP, "hipMalloc sdf"));
+    if (hipMemcpy(d, info->sdf_voxels, nv * 4, hipMemcpyHostToDevice) != hipSuccess)
+      return bail(fail(FLUID_ERR_HIP, "sdf upload"));
+    s->b.sdf = d;
+    s->sdf_res = info->sdf_resolution;
+  }
+  if (hipHostMalloc((void **)&s->h_ring, sizeof(DevUniforms) * kUniformRing) != hipSuccess)
+    return bail(fail(FLUID_ERR_HIP, "hipHostMalloc"));
+  for (int k = 0; k < kUniformRing; ++k)
+    if (hipEventCreateWithFlags(&s->ring_ev[k], hipEventDisableTiming) != hipSuccess)
+      return bail(fail(FLUID_ERR_HIP, "hipEventCreate"));
+  if (hipDeviceSynchronize() != hipSuccess) return bail(fail(FLUID_ERR_HIP, "memset"));
+  int rc = upload_canonical(s, info->particles, s->n);
+  if (rc) return bail(rc);
+  rc = push_uniforms(s);
+  if (rc) return bail(rc);
+  if (hipStreamSynchronize(s->stream) != hipSuccess) return bail(fail(FLUID_ERR_HIP, "sync"));
+  *out = s;
+  return FLUID_OK;
+}
+
+int fluid_destroy(fluid_sim *s) {
+  if (!s) return FLUID_OK;
+  (void)hipSetDevice(s->device);
+  if (s->stream) (void)hipStreamSynchronize(s->stream);
+  drop_graph(s);
+  for (void *p : s->allocs) (void)hipFree(p);
+  for (int k = 0; k < kUniformRing; ++k)
+    if (s->ring_ev[k]) (void)hipEventDestroy(s->ring_ev[k]);
+  if (s->h_ring) (void)hipHostFree(s->h_ring);
+  if (s->stream) (void)hipStreamDestroy(s->stream);
+  delete s;
+  return FLUID_OK;
+}
+
+int fluid_step(fluid_sim *s) { return fluid_step_n(s, 1); }
+
+int fluid_step_n(fluid_sim *s, uint32_t steps) {
+  if (!s) return fail(FLUID_ERR_INVALID_ARG, "null handle");
+  if (s->cursor != -1) return fail(FLUID_ERR_STATE, "a partial step is in progress");
+  HIP_TRY(hipSetDevice(s->device));
+  int rc = ensure_graph(s);
+  if (rc) return rc;
+  for (uint32_t k = 0; k < steps; ++k) HIP_TRY(hipGraphLaunch(s->graph, s->stream));
+  return FLUID_OK;
+}
+
+int fluid_step_stages(fluid_sim *s, uint32_t first, uint32_t last) {
+  if (!s) return fail(FLUID_ERR_INVALID_ARG, "null handle");
+  if (last >= FLUID_STAGE_COUNT || first > last)
+    return fail(FLUID_ERR_INVALID_ARG, "bad stage range");
+  if ((int)first != s->cursor + 1)
+    return fail(FLUID_ERR_STATE, "stages must continue the current step in order");
+  HIP_TRY(hipSetDevice(s->device));
+  for (uint32_t st = first; st <= last; ++st) {
+    const int rc = enqueue_stage(s, st);
+    if (rc) return rc;
+  }
+  s->cursor = (last == FLUID_STAGE_COUNT - 1) ? -1 : (int)last;
+  return FLUID_OK;
+}
+
+int fluid_update_sph_params(fluid_sim *s, const GpuSphParams3D *p) {
+  if (!s || !p) return fail(FLUID_ERR_INVALID_ARG, "null argument");
+  if (!(p->kernel_radius > 0.0f) || !(p->dt > 0.0f))
+    return fail(FLUID_ERR_INVALID_ARG, "kernel_radius and dt must be > 0");
+  if (p->kernel_radius != s->grid.cell_size)
+    return fail(FLUID_ERR_INVALID_ARG,
+                "kernel_radius differs from the grid cell size fixed at creation");
+  HIP_TRY(hipSetDevice(s->device));
+  s->params = *p;
+  s->params.num_particles = s->n;
+  return push_uniforms(s);
+}
+
+int fluid_update_container_geometry(fluid_sim *s, const GpuContainerGeometry *g) {
+  if (!s || !g) return fail(FLUID_ERR_INVALID_ARG, "null argument");
+  HIP_TRY(hipSetDevice(s->device));
+  s->cont = *g;
+  return push_uniforms(s);
+}
+
+int fluid_update_mouse_force(fluid_sim *s, const GpuMouseForce *m) {
+  if (!s || !m) return fail(FLUID_ERR_INVALID_ARG, "null argument");
+  HIP_TRY(hipSetDevice(s->device));
+  s->mouse = *m;
+  return push_uniforms(s);
+}
+
+int fluid_update_gravity(fluid_sim *s, const GpuGravity *g) {
+  if (!s || !g) return fail(FLUID_ERR_INVALID_ARG, "null argument");
+  HIP_TRY(hipSetDevice(s->device));
+  s->grav = *g;
+  return push_uniforms(s);
+}
+
+int fluid_update_rigid_body(fluid_sim *s, const GpuRigidBody *r) {
+  if (!s || !r) return fail(FLUID_ERR_INVALID_ARG, "null argument");
+  HIP_TRY(hipSetDevice(s->device));
+  s->rb = *r;
+  return push_uniforms(s);
+}
+
+int fluid_set_pcisph_iterations(fluid_sim *s, uint32_t it) {
+  if (!s) return fail(FLUID_ERR_INVALID_ARG, "null handle");
+  s->iters = std::max(it, 1u);
+  return FLUID_OK;
+}
+
+uint32_t fluid_pcisph_iterations(const fluid_sim *s) { return s ? s->iters : 0; }
+
+int fluid_clear_rigid_body_accum(fluid_sim *s) {
+  if (!s) return fail(FLUID_ERR_INVALID_ARG, "null handle");
+  HIP_TRY(hipSetDevice(s->device));
+  HIP_TRY(hipMemsetAsync(s->b.accum, 0, 32, s->stream));
+  return FLUID_OK;
+}
+
+int fluid_read_rigid_body_accum(fluid_sim *s, GpuRigidBodyAccum *out) {
+  if (!s || !out) return fail(FLUID_ERR_INVALID_ARG, "null argument");
+  HIP_TRY(hipSetDevice(s->device));
+  HIP_TRY(hipMemcpyAsync(&s->last_accum, s->b.accum, 32, hipMemcpyDeviceToHost, s->stream));
+  HIP_TRY(hipStreamSynchronize(s->stream));
+  *out = s->last_accum;
+  return check_errors(s);
+}
+
+int fluid_spawn_particles(fluid_sim *s, const float position[3], uint32_t count, float spread,
+                          uint32_t *spawned) {
+  if (!s || !position) return fail(FLUID_ERR_INVALID_ARG, "null argument");
+  if (spawned) *spawned = 0;
+  if (s->cursor != -1) return fail(FLUID_ERR_STATE, "a partial step is in progress");
+  const uint32_t avail = s->max_n - s->n;
+  const uint32_t c = std::min(count, avail);
+  if (c == 0) return FLUID_OK;
+  HIP_TRY(hipSetDevice(s->device));
+  // golden-angle spiral of sph_3d_grid.rs:1543-1556, in f32
+  std::vector<float4> pos(c), zero(c, make_float4(0.f, 0.f, 0.f, 0.f));
+  std::vector<uint32_t> sid(c);
+  for (uint32_t i = 0; i < c; ++i) {
+    const float angle = (float)i * 2.39996f;
+    const float r = spread * std::sqrt((float)i / (float)c);
+    const uint32_t layer = i / 20;
+    pos[i] = make_float4(position[0] + r * std::cos(angle),
+                         position[1] + (float)layer * spread * 0.3f,
+                         position[2] + r * std::sin(angle), 0.f);
+    sid[i] = s->n + i;
+  }
+  HIP_TRY(hipMemcpyAsync(s->b.pos + s->n, pos.data(), c * 16, hipMemcpyHostToDevice, s->stream));
+  HIP_TRY(hipMemcpyAsync(s->b.vel + s->n, zero.data(), c * 16, hipMemcpyHostToDevice, s->stream));
+  HIP_TRY(hipMemcpyAsync(s->b.frc + s->n, zero.data(), c * 16, hipMemcpyHostToDevice, s->stream));
+  HIP_TRY(hipMemcpyAsync(s->b.sid + s->n, sid.data(), c * 4, hipMemcpyHostToDevice, s->stream));
+  HIP_TRY(hipStreamSynchronize(s->stream));
+  s->extra.resize((size_t)(s->n + c) * 5, 0.0f);
+  s->n += c;
+  s->grid.num_particles = s->n;
+  s->params.num_particles = s->n;
+  s->cfg = pick_cfg(s->n);
+  if (spawned) *spawned = c;
+  return push_uniforms(s);
+}
+
+uint32_t fluid_num_particles(const fluid_sim *s) { return s ? s->n : 0; }
+uint32_t fluid_max_particles(const fluid_sim *s) { return s ? s->max_n : 0; }
+
+int fluid_read_particles(fluid_sim *s, SphParticle3D *out, uint32_t cap) {
+  if (!s || (!out && s->n)) return fail(FLUID_ERR_INVALID_ARG, "null argument");
+  if (cap < s->n) return fail(FLUID_ERR_CAPACITY, "output smaller than num_particles");
+  HIP_TRY(hipSetDevice(s->device));
+  const int c = s->cursor;
+  // Which buffer holds each canonical field at the current stage boundary.
+  const bool reordered = c >= FLUID_STAGE_REORDER;
+  const float4 *pos = reordered ? s->b.spos : s->b.pos;
+  const float4 *vel = !reordered ? s->b.vel : (c >= FLUID_STAGE_XSPH ? s->b.vel : s->b.svel);
+  const float4 *frc = !reordered ? s->b.frc : (c >= FLUID_STAGE_FORCE ? s->b.frc : s->b.sfrc);
+  std::vector<float4> hp, hv, hf;
+  std::vector<uint32_t> sid;
+  int rc;
+  if ((rc = d2h(s, hp, pos, s->n)) || (rc = d2h(s, hv, vel, s->n)) ||
+      (rc = d2h(s, hf, frc, s->n)) || (rc = d2h(s, sid, s->b.sid, s->n)))
+    return rc;
+  HIP_TRY(hipStreamSynchronize(s->stream));
+  for (uint32_t k = 0; k < s->n; ++k) {
+    const uint32_t i = sid[k];
+    if (i >= s->n) return fail(FLUID_ERR_STATE, "corrupt slot map");
+    SphParticle3D &o = out[i];
+    std::memset(&o, 0, sizeof(o));
+    o.position[0] = hp[k].x; o.position[1] = hp[k].y; o.position[2] = hp[k].z;
+    o.velocity[0] = hv[k].x; o.velocity[1] = hv[k].y; o.velocity[2] = hv[k].z;
+    o.force[0] = hf[k].x; o.force[1] = hf[k].y; o.force[2] = hf[k].z;
+    const float *e = &s->extra[(size_t)i * 5];
+    o.density = e[0];
+    o.near_density = e[1];
+    o.normal[0] = e[2]; o.normal[1] = e[3]; o.normal[2] = e[4];
+  }
+  return check_errors(s);
+}
+
+int fluid_write_particles(fluid_sim *s, const SphParticle3D *in, uint32_t n) {
+  if (!s || (!in && n)) return fail(FLUID_ERR_INVALID_ARG, "null argument");
+  if (n > s->max_n) return fail(FLUID_ERR_CAPACITY, "more particles than max_particles");
+  if (s->cursor != -1) return fail(FLUID_ERR_STATE, "a partial step is in progress");
+  HIP_TRY(hipSetDevice(s->device));
+  HIP_TRY(hipStreamSynchronize(s->stream));
+  const int rc = upload_canonical(s, in, n);
+  if (rc) return rc;
+  s->n = n;
+  s->grid.num_particles = n;
+  s->params.num_particles = n;
+  s->cfg = pick_cfg(n);
+  return push_uniforms(s);
+}
+
+int fluid_read_sorted_particles(fluid_sim *s, SphParticle3D *out, uint32_t cap) {
+  if (!s || (!out && s->n)) return fail(FLUID_ERR_INVALID_ARG, "null argument");
+  if (cap < s->n) return fail(FLUID_ERR_CAPACITY, "output smaller than num_particles");
+  HIP_TRY(hipSetDevice(s->device));
+  std::vector<float4> hp, hv, hf, hn;
+  std::vector<float> hd;
+  int rc;
+  if ((rc = d2h(s, hp, s->b.spos, s->n)) || (rc = d2h(s, hv, s->b.svel, s->n)) ||
+      (rc = d2h(s, hf, s->b.sfrc, s->n)) || (rc = d2h(s, hn, s->b.snrm, s->n)) ||
+      (rc = d2h(s, hd, s->b.sdens, s->n)))
+    return rc;
+  HIP_TRY(hipStreamSynchronize(s->stream));
+  for (uint32_t k = 0; k < s->n; ++k) {
+    SphParticle3D &o = out[k];
+    std::memset(&o, 0, sizeof(o));
+    o.position[0] = hp[k].x; o.position[1] = hp[k].y; o.position[2] = hp[k].z;
+    o.velocity[0] = hv[k].x; o.velocity[1] = hv[k].y; o.velocity[2] = hv[k].z;
+    o.force[0] = hf[k].x; o.force[1] = hf[k].y; o.force[2] = hf[k].z;
+    o.density = hd[k];
+    o.near_density = hn[k].w;
+    o.normal[0] = hn[k].x; o.normal[1] = hn[k].y; o.normal[2] = hn[k].z;
+  }
+  return check_errors(s);
+}
+
+static int read_u32(fluid_sim *s, const uint32_t *src, size_t count, uint32_t *out,
+                    uint32_t cap) {
+  if (!s || (!out && count)) return fail(FLUID_ERR_INVALID_ARG, "null argument");
+  if (cap < count) return fail(FLUID_ERR_CAPACITY, "output too small");
+  HIP_TRY(hipSetDevice(s->device));
+  if (count)
+    HIP_TRY(hipMemcpyAsync(out, src, count * 4, hipMemcpyDeviceToHost, s->stream));
+  HIP_TRY(hipStreamSynchronize(s->stream));
+  return check_errors(s);
+}
+
+int fluid_read_cell_counts(fluid_sim *s, uint32_t *out, uint32_t cap) {
+  if (!s) return fail(FLUID_ERR_INVALID_ARG, "null handle");
+  return read_u32(s, s->b.counts, s->grid.total_cells, out, cap);
+}
+
+int fluid_read_cell_starts(fluid_sim *s, uint32_t *out, uint32_t cap) {
+  if (!s) return fail(FLUID_ERR_INVALID_ARG, "null handle");
+  return read_u32(s, s->b.starts, s->grid.total_cells, out, cap);
+}
+
+int fluid_read_sorted_to_orig(fluid_sim *s, uint32_t *out, uint32_t cap) {
+  if (!s) return fail(FLUID_ERR_INVALID_ARG, "null handle");
+  return read_u32(s, s->b.sid, s->n, out, cap);
+}
+
+int fluid_read_orig_to_sorted(fluid_sim *s, uint32_t *out, uint32_t cap) {
+  if (!s || (!out && s->n)) return fail(FLUID_ERR_INVALID_ARG, "null argument");
+  if (cap < s->n) return fail(FLUID_ERR_CAPACITY, "output too small");
+  std::vector<uint32_t> sid(s->n);
+  const int rc = read_u32(s, s->b.sid, s->n, sid.data(), s->n);
+  if (rc) return rc;
+  for (uint32_t k = 0; k < s->n; ++k) {
+    if (sid[k] >= s->n) return fail(FLUID_ERR_STATE, "corrupt slot map");
+    out[sid[k]] = k;
+  }
+  return FLUID_OK;
+}
+
+int fluid_read_predicted(fluid_sim *s, uint32_t which, PredictedState *out, uint32_t cap) {
+  if (!s || (!out && s->n) || which > 1) return fail(FLUID_ERR_INVALID_ARG, "bad argument");
+  if (cap < s->n) return fail(FLUID_ERR_CAPACITY, "output too small");
+  HIP_TRY(hipSetDevice(s->device));
+  std::vector<float4> xq, v;
+  std::vector<float2> pr;
+  int rc;
+  if ((rc = d2h(s, xq, s->b.pred[which].xq, s->n)) || (rc = d2h(s, pr, s->b.pred[which].pr, s->n)) ||
+      (rc = d2h(s, v, s->b.pred[which].v, s->n)))
+    return rc;
+  HIP_TRY(hipStreamSynchronize(s->stream));
+  for (uint32_t k = 0; k < s->n; ++k) {
+    PredictedState &o = out[k];
+    o.pred_pos[0] = xq[k].x; o.pred_pos[1] = xq[k].y; o.pred_pos[2] = xq[k].z;
+    o.pressure = pr[k].x;
+    o.pred_vel[0] = v[k].x; o.pred_vel[1] = v[k].y; o.pred_vel[2] = v[k].z;
+    o.pred_density = pr[k].y;
+  }
+  return check_errors(s);
+}
+
+int fluid_read_grid_params(const fluid_sim *s, GpuGridParams *out) {
+  if (!s || !out) return fail(FLUID_ERR_INVALID_ARG, "null argument");
+  *out = s->grid;
+  return FLUID_OK;
+}
+
+int fluid_read_sph_params(const fluid_sim *s, GpuSphParams3D *out) {
+  if (!s || !out) return fail(FLUID_ERR_INVALID_ARG, "null argument");
+  *out = s->params;
+  return FLUID_OK;
+}
+
+int fluid_read_container_geometry(const fluid_sim *s, GpuContainerGeometry *out) {
+  if (!s || !out) return fail(FLUID_ERR_INVALID_ARG, "null argument");
+  *out = s->cont;
+  return FLUID_OK;
+}
+
+int fluid_sync(fluid_sim *s) {
+  if (!s) return fail(FLUID_ERR_INVALID_ARG, "null handle");
+  HIP_TRY(hipSetDevice(s->device));
+  HIP_TRY(hipStreamSynchronize(s->stream));
+  return check_errors(s);
+}
+
+void *fluid_stream(fluid_sim *s) { return s ? (void *)s->stream : nullptr; }
+
+int fluid_time_steps(fluid_sim *s, uint32_t warmup, uint32_t steps, float *ms_total) {
+  if (!s || !ms_total) return fail(FLUID_ERR_INVALID_ARG, "null argument");
+  int rc = fluid_step_n(s, warmup);
+  if (rc) return rc;
+  rc = ensure_graph(s);
+  if (rc) return rc;
+  hipEvent_t e0, e1;
+  HIP_TRY(hipEventCreate(&e0));
+  HIP_TRY(hipEventCreate(&e1));
+  HIP_TRY(hipStreamSynchronize(s->stream));
+  HIP_TRY(hipEventRecord(e0, s->stream));
+  for (uint32_t k = 0; k < steps; ++k) HIP_TRY(hipGraphLaunch(s->graph, s->stream));
+  HIP_TRY(hipEventRecord(e1, s->stream));
+  HIP_TRY(hipEventSynchronize(e1));
+  HIP_TRY(hipEventElapsedTime(ms_total, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  return check_errors(s);
+}
+
+int fluid_time_stages(fluid_sim *s, uint32_t steps, float *ms_per_stage) {
+  if (!s || !ms_per_stage) return fail(FLUID_ERR_INVALID_ARG, "null argument");
+  if (s->cursor != -1) return fail(FLUID_ERR_STATE, "a partial step is in progress");
+  HIP_TRY(hipSetDevice(s->device));
+  std::vector<hipEvent_t> ev(2 * FLUID_STAGE_COUNT * std::max(steps, 1u));
+  for (auto &e : ev) HIP_TRY(hipEventCreate(&e));
+  for (uint32_t k = 0; k < FLUID_STAGE_COUNT; ++k) ms_per_stage[k] = 0.f;
+  size_t q = 0;
+  for (uint32_t step = 0; step < steps; ++step) {
+    for (uint32_t st = 0; st < FLUID_STAGE_COUNT; ++st) {
+      HIP_TRY(hipEventRecord(ev[q++], s->stream));
+      const int rc = enqueue_stage(s, st);
+      if (rc) return rc;
+      HIP_TRY(hipEventRecord(ev[q++], s->stream));
+    }
+  }
+  HIP_TRY(hipStreamSynchronize(s->stream));
+  q = 0;
+  for (uint32_t step = 0; step < steps; ++step)
+    for (uint32_t st = 0; st < FLUID_STAGE_COUNT; ++st) {
+      float ms = 0.f;
+      HIP_TRY(hipEventElapsedTime(&ms, ev[q], ev[q + 1]));
+      q += 2;
+      ms_per_stage[st] += ms / (float)steps;
+    }
+  for (auto &e : ev) (void)hipEventDestroy(e);
+  return check_errors(s);
+}
+
+int fluid_count_pairs(fluid_sim *s, uint64_t *candidates, uint64_t *in_range) {
+  if (!s || !candidates || !in_range) return fail(FLUID_ERR_INVALID_ARG, "null argument");
+  HIP_TRY(hipSetDevice(s->device));
+  HIP_TRY(hipMemsetAsync(s->d_pairs, 0, 16, s->stream));
+  HIP_TRY(launch_count_pairs(s->d_u, s->b, s->n, s->d_pairs, s->stream));
+  unsigned long long h[2] = {0, 0};
+  HIP_TRY(hipMemcpyAsync(h, s->d_pairs, 16, hipMemcpyDeviceToHost, s->stream));
+  HIP_TRY(hipStreamSynchronize(s->stream));
+  *candidates = h[0];
+  *in_range = h[1];
+  return FLUID_OK;
+}
+
+}  // extern "C"

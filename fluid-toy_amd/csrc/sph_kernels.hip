@@ -1,0 +1,890 @@
+// sph_kernels.hip — the per-step SPH kernels for gfx950 (wave64).
+//
+// One kernel (or kernel pair) per WGSL entry point dispatched by
+// SphSimulation3DGrid::step (src/simulation/sph_3d_grid.rs:1261-1444).  The
+// arithmetic of every stage follows the WGSL expression structure; three
+// kinds of value are made to agree bit-for-bit with the f32 CPU oracle
+// because a branch or an integer depends on them:
+//   - cell coordinates: floor((p - origin) * inv_cell), no fusion possible;
+//   - squared distances: dot(a, b) evaluated as fma(z, z, fma(y, y, x*x));
+//   - the force pass's close-range test r < 0.1 h, on a correctly rounded
+//     sqrt of that squared distance.
+// Elsewhere the compiler may contract multiply-adds and the neighbour sums
+// may be split across lanes, so floating-point fields agree to a tolerance.
+//
+// Integer/ordering work (build, scan, reorder) is exact and deterministic:
+// the within-cell order is ascending original particle index, which the
+// reference leaves to a racing atomicAdd (grid_reorder.wgsl:51).
+#include "fluid_internal.h"
+
+namespace fluid {
+namespace {
+
+constexpr uint64_t kFlagAgg = 1ull << 32;
+constexpr uint64_t kFlagPrefix = 2ull << 32;
+constexpr uint32_t kSpinCap = 1u << 24;
+
+__device__ __forceinline__ float dot3(float ax, float ay, float az, float bx,
+                                      float by, float bz) {
+  return __builtin_fmaf(az, bz, __builtin_fmaf(ay, by, ax * bx));
+}
+
+// position_to_cell (grid_build.wgsl:33-36): i32(floor((p - o) * inv)).
+// v_cvt_i32_f32 saturates and maps NaN to 0, the same as the oracle.
+__device__ __forceinline__ int cell_coord(float p, float o, float inv) {
+  return (int)floorf((p - o) * inv);
+}
+
+__device__ __forceinline__ float fast_sqrt(float x) {
+  return __builtin_amdgcn_sqrtf(x);
+}
+
+// Inclusive wave64 scan on DPP: row_shr 1,2,4,8 within each 16-lane row,
+// then row_bcast:15 into rows 1 and 3 and row_bcast:31 into rows 2 and 3.
+__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v) {
+  v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x111, 0xf, 0xf, false);
+  v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x112, 0xf, 0xf, false);
+  v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x114, 0xf, 0xf, false);
+  v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x118, 0xf, 0xf, false);
+  v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x142, 0xa, 0xf, false);
+  v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x143, 0xc, 0xf, false);
+  return v;
+}
+
+__device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
+  for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o);
+  return v;
+}
+
+template <int L>
+__device__ __forceinline__ float group_sum(float v) {
+#pragma unroll
+  for (int o = L / 2; o >= 1; o >>= 1) v += __shfl_xor(v, o, L);
+  return v;
+}
+
+// The 27-cell stencil of sph_density_3d_grid.wgsl:129-167, walked as nine
+// x-rows: cells (x-1..x+1, y, z) are consecutive cell indices, so their
+// particles form one contiguous sorted range.  The visit order (dz, dy, dx,
+// then slot) is the reference's.  Lane `sub` of an L-lane group takes every
+// L-th candidate of each row.
+template <int L, class F>
+__device__ __forceinline__ void sweep27(const DevUniforms *__restrict__ U,
+                                        const uint32_t *__restrict__ counts,
+                                        const uint32_t *__restrict__ starts,
+                                        int cx, int cy, int cz, uint32_t sub,
+                                        F &&f) {
+  const int gx = (int)U->gx, gy = (int)U->gy, gz = (int)U->gz;
+  cx = min(max(cx, -2), gx + 1);
+  cy = min(max(cy, -2), gy + 1);
+  cz = min(max(cz, -2), gz + 1);
+  const int x0 = max(cx - 1, 0), x1 = min(cx + 1, gx - 1);
+  if (x0 > x1) return;
+  const int z0 = max(cz - 1, 0), z1 = min(cz + 1, gz - 1);
+  const int y0 = max(cy - 1, 0), y1 = min(cy + 1, gy - 1);
+  for (int z = z0; z <= z1; ++z) {
+    for (int y = y0; y <= y1; ++y) {
+      const uint32_t row = (uint32_t)y * U->gx + (uint32_t)z * U->gxy;
+      const uint32_t c0 = row + (uint32_t)x0, c1 = row + (uint32_t)x1;
+      const uint32_t begin = starts[c0] - counts[c0];
+      const uint32_t end = starts[c1];
+      for (uint32_t j = begin + sub; j < end; j += L) f(j);
+    }
+  }
+}
+
+// ------------------------------------------------------------------------
+// Stage 0 — grid_clear.wgsl:23-29 and clear_buffer(cell_offsets)
+// (sph_3d_grid.rs:1281).  The offsets buffer does not exist here (build
+// hands out the in-cell slots); the scan's tile states are cleared instead.
+__global__ __launch_bounds__(256) void k_clear(uint32_t *__restrict__ counts,
+                                               uint32_t cells,
+                                               unsigned long long *status,
+                                               uint32_t ntiles,
+                                               uint32_t *tile_counter) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t q = cells / 4;
+  if (i < q) reinterpret_cast<uint4 *>(counts)[i] = make_uint4(0, 0, 0, 0);
+  if (i < cells - 4 * q) counts[4 * q + i] = 0u;
+  if (i < ntiles) status[i] = 0ull;
+  if (i == 0) *tile_counter = 0u;
+}
+
+// ------------------------------------------------------------------------
+// Stage 1 — grid_build.wgsl:46-62.  cell = clamp(floor((p-o)/h)) linearised
+// x + y G + z G^2; the per-cell count is an atomic add.  Lanes holding the
+// same cell in a run (slots are spatially sorted from the previous step)
+// merge into one atomic; the returned base + rank in the run is the
+// particle's provisional in-cell slot, used only to scatter before ranking.
+__global__ __launch_bounds__(256) void k_build(const DevUniforms *__restrict__ U,
+                                               Bufs b, uint32_t n) {
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool valid = t < n;
+  uint32_t ci = 0xffffffffu;
+  if (valid) {
+    const float4 p = b.pos[t];
+    const int cx = min(max(cell_coord(p.x, U->ox, U->inv_cell), 0), (int)U->gx - 1);
+    const int cy = min(max(cell_coord(p.y, U->oy, U->inv_cell), 0), (int)U->gy - 1);
+    const int cz = min(max(cell_coord(p.z, U->oz, U->inv_cell), 0), (int)U->gz - 1);
+    ci = (uint32_t)cx + (uint32_t)cy * U->gx + (uint32_t)cz * U->gxy;
+    b.cell_of[t] = ci;
+  }
+  const int lane = threadIdx.x & 63;
+  const uint32_t prev = __shfl_up(ci, 1);
+  const bool head = valid && (lane == 0 || prev != ci);
+  const unsigned long long heads = __ballot(head);
+  const unsigned long long ends = heads | __ballot(!valid);
+  uint32_t base = 0;
+  if (head) {
+    const unsigned long long above = ends & ~((2ull << lane) - 1ull);
+    const int next = above ? (__ffsll((long long)above) - 1) : 64;
+    base = atomicAdd(&b.counts[ci], (uint32_t)(next - lane));
+  }
+  const unsigned long long below = heads & ((2ull << lane) - 1ull);
+  const int hl = below ? 63 - __clzll((long long)below) : lane;
+  const uint32_t hb = __shfl(base, hl);
+  if (valid) b.slot_of[t] = hb + (uint32_t)(lane - hl);
+}
+
+// ------------------------------------------------------------------------
+// Stage 2 — copy(counts -> starts) + prefix_scan.wgsl's three dispatches,
+// as one single-pass inclusive scan with decoupled look-back.  Tiles take
+// ids in launch order from an atomic counter, so a tile only waits on tiles
+// that are already running; every spin is capped (error bit on overflow).
+__global__ __launch_bounds__(256) void k_scan(Bufs b, uint32_t cells) {
+  __shared__ uint32_t s_tile, s_prefix;
+  __shared__ uint32_t s_wave[4];
+  if (threadIdx.x == 0) s_tile = atomicAdd(b.tile_counter, 1u);
+  __syncthreads();
+  const uint32_t tile = s_tile;
+  const uint32_t base = tile * kScanTile + threadIdx.x * kScanItems;
+  uint32_t v0 = 0, v1 = 0, v2 = 0, v3 = 0;
+  if (base + 3 < cells) {
+    const uint4 x = *reinterpret_cast<const uint4 *>(b.counts + base);
+    v0 = x.x; v1 = x.y; v2 = x.z; v3 = x.w;
+  } else {
+    if (base < cells) v0 = b.counts[base];
+    if (base + 1 < cells) v1 = b.counts[base + 1];
+    if (base + 2 < cells) v2 = b.counts[base + 2];
+  }
+  v1 += v0; v2 += v1; v3 += v2;
+  const uint32_t tsum = v3;
+  const uint32_t incl = wave_incl_scan(tsum);
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  if (lane == 63) s_wave[wid] = incl;
+  __syncthreads();
+  uint32_t wave_off = 0;
+  for (int w = 0; w < wid; ++w) wave_off += s_wave[w];
+  const uint32_t agg = s_wave[0] + s_wave[1] + s_wave[2] + s_wave[3];
+  if (wid == 0) {
+    uint32_t excl = 0;
+    if (tile == 0) {
+      if (lane == 0)
+        __hip_atomic_store(&b.tile_status[0], kFlagPrefix | agg,
+                           __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    } else {
+      if (lane == 0)
+        __hip_atomic_store(&b.tile_status[tile], kFlagAgg | agg,
+                           __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+      long long look = (long long)tile - 1;
+      bool failed = false;
+      while (true) {
+        const long long idx = look - lane;
+        unsigned long long st;
+        uint32_t spins = 0;
+        do {
+          st = idx >= 0 ? __hip_atomic_load(&b.tile_status[idx], __ATOMIC_ACQUIRE,
+                                            __HIP_MEMORY_SCOPE_AGENT)
+                        : kFlagPrefix;
+          if (++spins > kSpinCap) { failed = true; break; }
+        } while (__ballot((st >> 32) == 0ull) != 0ull);
+        if (__ballot(failed) != 0ull) {
+          if (lane == 0) atomicOr(b.err, kErrScanSpin);
+          break;
+        }
+        const unsigned long long pm = __ballot((st >> 32) == 2ull);
+        if (pm) {
+          const int first = __ffsll((long long)pm) - 1;
+          excl += wave_sum(lane <= first ? (uint32_t)st : 0u);
+          break;
+        }
+        excl += wave_sum((uint32_t)st);
+        look -= 64;
+      }
+      if (lane == 0)
+        __hip_atomic_store(&b.tile_status[tile], kFlagPrefix | (uint32_t)(excl + agg),
+                           __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    if (lane == 0) s_prefix = excl;
+  }
+  __syncthreads();
+  const uint32_t off = s_prefix + wave_off + (incl - tsum);
+  if (base + 3 < cells) {
+    *reinterpret_cast<uint4 *>(b.starts + base) =
+        make_uint4(off + v0, off + v1, off + v2, off + v3);
+  } else {
+    if (base < cells) b.starts[base] = off + v0;
+    if (base + 1 < cells) b.starts[base + 1] = off + v1;
+    if (base + 2 < cells) b.starts[base + 2] = off + v2;
+  }
+}
+
+// ------------------------------------------------------------------------
+// Stage 3 — grid_reorder.wgsl:37-63 as a deterministic counting sort.
+// (a) scatter (old slot, original index) to cell start + provisional slot;
+__global__ __launch_bounds__(256) void k_scatter(Bufs b, uint32_t n) {
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const uint32_t ci = b.cell_of[t];
+  const uint32_t start = b.starts[ci] - b.counts[ci];
+  b.tmp[start + b.slot_of[t]] = make_uint2(t, b.sid[t]);
+}
+
+// (b) rank every entry by original index within its cell segment and copy
+// the particle (position, velocity, force) to its final sorted slot.
+__global__ __launch_bounds__(256) void k_rank_gather(Bufs b, uint32_t n) {
+  const uint32_t u = blockIdx.x * blockDim.x + threadIdx.x;
+  if (u >= n) return;
+  const uint2 e = b.tmp[u];
+  const uint32_t ci = b.cell_of[e.x];
+  const uint32_t end = b.starts[ci];
+  const uint32_t start = end - b.counts[ci];
+  uint32_t rank = 0;
+  for (uint32_t v = start; v < end; ++v) rank += (b.tmp[v].y < e.y) ? 1u : 0u;
+  const uint32_t dest = start + rank;
+  b.sid[dest] = e.y;
+  b.spos[dest] = b.pos[e.x];
+  b.svel[dest] = b.vel[e.x];
+  b.sfrc[dest] = b.frc[e.x];
+}
+
+// ------------------------------------------------------------------------
+// Stage 4 — sph_density_3d_grid.wgsl:111-194 (+ boundary_gamma :86-109).
+template <int L>
+__global__ __launch_bounds__(256) void k_density(const DevUniforms *__restrict__ U,
+                                                 Bufs b, uint32_t n) {
+  const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t s = g / L, sub = g % L;
+  if (s >= n) return;
+  const float4 pi = b.spos[s];
+  const int cx = cell_coord(pi.x, U->ox, U->inv_cell);
+  const int cy = cell_coord(pi.y, U->oy, U->inv_cell);
+  const int cz = cell_coord(pi.z, U->oz, U->inv_cell);
+  const float h = U->h, h2 = U->h2, mass = U->mass;
+  const float ps = U->poly6_scale, ns = U->near_scale, mr = U->m_over_rho0;
+  float dens = 0.f, near = 0.f, nx = 0.f, ny = 0.f, nz = 0.f;
+  const float4 *__restrict__ spos = b.spos;
+  sweep27<L>(U, b.counts, b.starts, cx, cy, cz, sub, [&](uint32_t j) {
+    const float4 pj = spos[j];
+    const float rx = pi.x - pj.x, ry = pi.y - pj.y, rz = pi.z - pj.z;
+    const float r2 = dot3(rx, ry, rz, rx, ry, rz);
+    if (r2 < h2) {
+      const float r = fast_sqrt(r2);
+      const float d = h2 - r2;
+      dens += mass * (ps * d * d * d);
+      const float dn = h - r;
+      near += mass * (ns * dn * dn * dn);
+      if (r2 > 1e-12f) {
+        nx += mr * rx * d * d;
+        ny += mr * ry * d * d;
+        nz += mr * rz * d * d;
+      }
+    }
+  });
+  dens = group_sum<L>(dens);
+  near = group_sum<L>(near);
+  nx = group_sum<L>(nx);
+  ny = group_sum<L>(ny);
+  nz = group_sum<L>(nz);
+  if (sub != 0) return;
+  // boundary_gamma in the container frame (container_common.wgsl:27-35)
+  const float px = pi.x, py = pi.y - U->cy, pz = pi.z;
+  const float lx = dot3(U->iv[0][0], U->iv[0][1], U->iv[0][2], px, py, pz);
+  const float ly = dot3(U->iv[1][0], U->iv[1][1], U->iv[1][2], px, py, pz);
+  const float lz = dot3(U->iv[2][0], U->iv[2][1], U->iv[2][2], px, py, pz);
+  const float dist[6] = {lx - (-U->hw), U->hw - lx, ly - (-U->hh),
+                         U->hh - ly,    lz - (-U->hd), U->hd - lz};
+  float gamma = 1.0f;
+#pragma unroll
+  for (int k = 0; k < 6; ++k)
+    if (dist[k] < h) gamma *= 0.5f + 0.5f * fminf(fmaxf(dist[k] / h, 0.f), 1.f);
+  gamma = fmaxf(gamma, 0.1f);
+  b.sdens[s] = dens / gamma;
+  const float sc = U->normal_scale;
+  b.snrm[s] = make_float4(nx * sc, ny * sc, nz * sc, near);
+}
+
+// ------------------------------------------------------------------------
+// Stage 5 — xsph.wgsl:81-138.  Writes the canonical velocity; with
+// epsilon <= 0 the reference leaves it untouched, i.e. equal to the sorted
+// copy, which is what is written then.
+template <int L>
+__global__ __launch_bounds__(256) void k_xsph(const DevUniforms *__restrict__ U,
+                                              Bufs b, uint32_t n) {
+  const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t s = g / L, sub = g % L;
+  if (s >= n) return;
+  const float eps = U->eps;
+  const float4 vi = b.svel[s];
+  if (!(eps > 0.0f)) {
+    if (sub == 0) b.vel[s] = vi;
+    return;
+  }
+  const float4 pi = b.spos[s];
+  const int cx = cell_coord(pi.x, U->ox, U->inv_cell);
+  const int cy = cell_coord(pi.y, U->oy, U->inv_cell);
+  const int cz = cell_coord(pi.z, U->oz, U->inv_cell);
+  const float h2 = U->h2, mass = U->mass, ps = U->poly6_scale;
+  float cxv = 0.f, cyv = 0.f, czv = 0.f;
+  const float4 *__restrict__ spos = b.spos;
+  const float4 *__restrict__ svel = b.svel;
+  const float *__restrict__ sdens = b.sdens;
+  sweep27<L>(U, b.counts, b.starts, cx, cy, cz, sub, [&](uint32_t j) {
+    const float4 pj = spos[j];
+    const float rx = pi.x - pj.x, ry = pi.y - pj.y, rz = pi.z - pj.z;
+    const float r2 = dot3(rx, ry, rz, rx, ry, rz);
+    if (r2 < h2 && r2 > 1e-12f) {
+      const float dj = fmaxf(sdens[j], 1.0f);
+      const float4 vj = svel[j];
+      const float d = h2 - r2;
+      const float w = ps * d * d * d;
+      const float f = mass / dj;
+      cxv += f * (vj.x - vi.x) * w;
+      cyv += f * (vj.y - vi.y) * w;
+      czv += f * (vj.z - vi.z) * w;
+    }
+  });
+  cxv = group_sum<L>(cxv);
+  cyv = group_sum<L>(cyv);
+  czv = group_sum<L>(czv);
+  if (sub == 0)
+    b.vel[s] = make_float4(vi.x + eps * cxv, vi.y + eps * cyv, vi.z + eps * czv, 0.f);
+}
+
+// ------------------------------------------------------------------------
+// Stage 6 — sph_force_3d_grid.wgsl:105-227: near pressure + close-range
+// separation, Monaghan viscosity, Akinci cohesion and curvature, gravity.
+template <int L>
+__global__ __launch_bounds__(256) void k_force(const DevUniforms *__restrict__ U,
+                                               Bufs b, uint32_t n) {
+  const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t s = g / L, sub = g % L;
+  if (s >= n) return;
+  const float4 pi = b.spos[s];
+  const float4 vi = b.vel[s];  // post-XSPH self velocity
+  const int cx = cell_coord(pi.x, U->ox, U->inv_cell);
+  const int cy = cell_coord(pi.y, U->oy, U->inv_cell);
+  const int cz = cell_coord(pi.z, U->oz, U->inv_cell);
+  const float di = fmaxf(b.sdens[s], 1.0f);
+  const float4 ni = b.snrm[s];
+  const float ndi = fmaxf(ni.w, 1.0f);
+  const float h = U->h, h2 = U->h2, mass = U->mass, near_k = U->near_k;
+  const float gs = U->grad_scale, md = U->min_dist, visc = U->visc;
+  const float veps = U->visc_eps, rho0 = U->rho0, aks = U->akinci_scale;
+  const float hh = U->half_h, h6_64 = U->h6_64;
+  const float npi = near_k * ndi;
+  float apx = 0.f, apy = 0.f, apz = 0.f;
+  float avx = 0.f, avy = 0.f, avz = 0.f;
+  float acx = 0.f, acy = 0.f, acz = 0.f;
+  const float4 *__restrict__ spos = b.spos;
+  const float4 *__restrict__ svel = b.svel;
+  const float4 *__restrict__ snrm = b.snrm;
+  const float *__restrict__ sdens = b.sdens;
+  sweep27<L>(U, b.counts, b.starts, cx, cy, cz, sub, [&](uint32_t j) {
+    const float4 pj = spos[j];
+    const float rx = pi.x - pj.x, ry = pi.y - pj.y, rz = pi.z - pj.z;
+    const float r2 = dot3(rx, ry, rz, rx, ry, rz);
+    if (!(r2 < h2 && r2 > 1e-12f)) return;
+    const float r = sqrtf(r2);  // correctly rounded: decides r < min_dist
+    float dx, dy, dz;
+    const bool close = r < md;
+    if (close) {
+      const float idx = (float)s - (float)j;
+      const float a = sinf(idx * 1.0f), c = cosf(idx * 2.0f), e = sinf(idx * 3.0f);
+      const float l = sqrtf(dot3(a, c, e, a, c, e));
+      dx = a / l; dy = c / l; dz = e / l;
+    } else {
+      const float ux = -rx, uy = -ry, uz = -rz;
+      const float il = 1.0f / r;
+      dx = ux * il; dy = uy * il; dz = uz * il;
+    }
+    const float dj = fmaxf(sdens[j], 1.0f);
+    const float4 nj = snrm[j];
+    const float ndj = fmaxf(nj.w, 1.0f);
+    const float npj = near_k * ndj;
+    const float dr = h - r;
+    const float grad = gs * dr * dr;
+    const float sp = -mass * (npi + npj) / (2.0f * di * ndj);
+    apx += sp * dx * grad;
+    apy += sp * dy * grad;
+    apz += sp * dz * grad;
+    if (close) {
+      const float sep = 200.0f * (1.0f - r / md);
+      apx += -sep * dx;
+      apy += -sep * dy;
+      apz += -sep * dz;
+    }
+    const float4 vj = svel[j];
+    const float vdx = dot3(vi.x - vj.x, vi.y - vj.y, vi.z - vj.z, rx, ry, rz);
+    if (vdx < 0.0f) {
+      const float mu = h * vdx / (r2 + veps);
+      const float avg = (di + dj) * 0.5f;
+      const float piv = -visc * mu / avg;
+      const float sv = -mass * piv * grad;
+      avx += sv * dx;
+      avy += sv * dy;
+      avz += sv * dz;
+    }
+    const float K = 2.0f * rho0 / (di + dj);
+    const float r3 = r * r * r;
+    const float coh = (r >= hh) ? aks * dr * dr * dr * r3
+                                : aks * (2.0f * dr * dr * dr * r3 - h6_64);
+    const float mk = mass * K;
+    const float sc = mk * coh;
+    acx += sc * dx - mk * (ni.x - nj.x);
+    acy += sc * dy - mk * (ni.y - nj.y);
+    acz += sc * dz - mk * (ni.z - nj.z);
+  });
+  apx = group_sum<L>(apx); apy = group_sum<L>(apy); apz = group_sum<L>(apz);
+  avx = group_sum<L>(avx); avy = group_sum<L>(avy); avz = group_sum<L>(avz);
+  acx = group_sum<L>(acx); acy = group_sum<L>(acy); acz = group_sum<L>(acz);
+  if (sub != 0) return;
+  const float st = U->st;
+  b.frc[s] = make_float4(apx + avx + acx * st + U->grav[0],
+                         apy + avy + acy * st + U->grav[1],
+                         apz + avz + acz * st + U->grav[2], 0.f);
+}
+
+// ------------------------------------------------------------------------
+// Stage 7 — pcisph_predict.wgsl:50-73, run per sorted slot (the canonical
+// state already lives in slot order).
+__global__ __launch_bounds__(256) void k_predict(const DevUniforms *__restrict__ U,
+                                                 Bufs b, uint32_t n) {
+  const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n) return;
+  const float dt = U->dt;
+  const float4 p = b.spos[s], v = b.vel[s], a = b.frc[s];
+  const float vx = v.x + dt * a.x, vy = v.y + dt * a.y, vz = v.z + dt * a.z;
+  b.pred[0].xq[s] = make_float4(p.x + dt * vx, p.y + dt * vy, p.z + dt * vz, 0.f);
+  b.pred[0].pr[s] = make_float2(0.f, 0.f);
+  b.pred[0].v[s] = make_float4(vx, vy, vz, 0.f);
+}
+
+// ------------------------------------------------------------------------
+// Stage 8 — pcisph_solve.wgsl:172-208, one iteration, in -> out.  The
+// neighbour's p / max(rho*,1)^2 is precomputed by the producing iteration
+// (same operations as the reference evaluates per pair).
+template <int L>
+__global__ __launch_bounds__(256) void k_solve(const DevUniforms *__restrict__ U,
+                                               Bufs b, uint32_t n, PredBuf in,
+                                               PredBuf out) {
+  const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t s = g / L, sub = g % L;
+  if (s >= n) return;
+  const float4 xi = in.xq[s];
+  const float4 p0 = b.spos[s];  // current position (grid lookup)
+  const int cx = cell_coord(p0.x, U->ox, U->inv_cell);
+  const int cy = cell_coord(p0.y, U->oy, U->inv_cell);
+  const int cz = cell_coord(p0.z, U->oz, U->inv_cell);
+  const float h = U->h, h2 = U->h2, mass = U->mass, ps = U->poly6_scale;
+  const float gs = U->grad_scale;
+  float pd = 0.f, ax = 0.f, ay = 0.f, az = 0.f;
+  const float4 *__restrict__ xq = in.xq;
+  sweep27<L>(U, b.counts, b.starts, cx, cy, cz, sub, [&](uint32_t j) {
+    const float4 xj = xq[j];
+    const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
+    const float r2 = dot3(rx, ry, rz, rx, ry, rz);
+    if (r2 < h2) {
+      const float d = h2 - r2;
+      pd += mass * (ps * d * d * d);
+      if (r2 > 1e-12f) {
+        const float r = fast_sqrt(r2);
+        const float dr = h - r;
+        const float sc = -mass * (xi.w + xj.w) * (gs * dr * dr);
+        const float il = 1.0f / r;
+        ax += sc * (-rx * il);
+        ay += sc * (-ry * il);
+        az += sc * (-rz * il);
+      }
+    }
+  });
+  pd = group_sum<L>(pd);
+  ax = group_sum<L>(ax);
+  ay = group_sum<L>(ay);
+  az = group_sum<L>(az);
+  if (sub != 0) return;
+  const float old_p = in.pr[s].x;
+  const float err = fmaxf(0.0f, pd - U->rho0);
+  const float new_p = old_p + U->delta * err;
+  const float amag = dot3(ax, ay, az, ax, ay, az);
+  if (amag > U->max_accel_sq) {
+    const float f = U->max_accel / sqrtf(amag);
+    ax *= f; ay *= f; az *= f;
+  }
+  const float dt = U->dt;
+  const float4 v0 = b.vel[s], a0 = b.frc[s];
+  const float vx = v0.x + dt * (a0.x + ax);
+  const float vy = v0.y + dt * (a0.y + ay);
+  const float vz = v0.z + dt * (a0.z + az);
+  const float nd = fmaxf(pd, 1.0f);
+  out.xq[s] = make_float4(p0.x + dt * vx, p0.y + dt * vy, p0.z + dt * vz,
+                          new_p / (nd * nd));
+  out.pr[s] = make_float2(new_p, pd);
+  out.v[s] = make_float4(vx, vy, vz, 0.f);
+}
+
+// ------------------------------------------------------------------------
+// Stage 9 — pcisph_finalize.wgsl:50-74: force <- v* - v, velocity <- v*.
+__global__ __launch_bounds__(256) void k_finalize(Bufs b, uint32_t n, PredBuf fin) {
+  const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n) return;
+  const float4 cv = fin.v[s], ov = b.vel[s];
+  b.frc[s] = make_float4(cv.x - ov.x, cv.y - ov.y, cv.z - ov.z, 0.f);
+  b.vel[s] = make_float4(cv.x, cv.y, cv.z, 0.f);
+}
+
+// ------------------------------------------------------------------------
+// Stage 10 — sph_integrate_3d.wgsl:91-388.
+__device__ __forceinline__ float wsign(float v) {
+  return v > 0.f ? 1.f : (v < 0.f ? -1.f : 0.f);
+}
+__device__ __forceinline__ int32_t to_i32(float v) {
+  // WGSL f32 -> i32: truncate, saturate, NaN -> 0 (v_cvt_i32_f32)
+  return (int32_t)v;
+}
+__device__ float sdf_fetch(const float *sdf, int r, int x, int y, int z) {
+  x = min(max(x, 0), r - 1);
+  y = min(max(y, 0), r - 1);
+  z = min(max(z, 0), r - 1);
+  return sdf[(size_t)x + (size_t)y * r + (size_t)z * r * r];
+}
+// Linear, clamp-to-edge texture sample (sph_3d_grid.rs:336-344).
+__device__ float sdf_sample(const float *sdf, uint32_t res, float ux, float uy,
+                            float uz) {
+  if (res == 0) return 1.0f;  // the reference's 1x1x1 dummy texture of 1.0
+  const float u = ux * (float)res - 0.5f, v = uy * (float)res - 0.5f,
+              w = uz * (float)res - 0.5f;
+  const float fu = floorf(u), fv = floorf(v), fw = floorf(w);
+  const int x0 = (int)fu, y0 = (int)fv, z0 = (int)fw, r = (int)res;
+  const float a = u - fu, bb = v - fv, c = w - fw;
+  const float c000 = sdf_fetch(sdf, r, x0, y0, z0), c100 = sdf_fetch(sdf, r, x0 + 1, y0, z0);
+  const float c010 = sdf_fetch(sdf, r, x0, y0 + 1, z0), c110 = sdf_fetch(sdf, r, x0 + 1, y0 + 1, z0);
+  const float c001 = sdf_fetch(sdf, r, x0, y0, z0 + 1), c101 = sdf_fetch(sdf, r, x0 + 1, y0, z0 + 1);
+  const float c011 = sdf_fetch(sdf, r, x0, y0 + 1, z0 + 1), c111 = sdf_fetch(sdf, r, x0 + 1, y0 + 1, z0 + 1);
+  const float x00 = c000 + (c100 - c000) * a, x10 = c010 + (c110 - c010) * a;
+  const float x01 = c001 + (c101 - c001) * a, x11 = c011 + (c111 - c011) * a;
+  const float y0v = x00 + (x10 - x00) * bb, y1v = x01 + (x11 - x01) * bb;
+  return y0v + (y1v - y0v) * c;
+}
+
+__global__ __launch_bounds__(256) void k_integrate(const DevUniforms *__restrict__ U,
+                                                   Bufs b, uint32_t n) {
+  const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n) return;
+  const float4 p4 = b.spos[s], v4 = b.vel[s];
+  float px = p4.x, py = p4.y, pz = p4.z, vx = v4.x, vy = v4.y, vz = v4.z;
+  const float(*iv)[4] = U->iv;
+  const float(*fw)[4] = U->fw;
+  // world_to_local / world_dir_to_local (container_common.wgsl:27-56)
+  const float qy = py - U->cy;
+  const float lpx = dot3(iv[0][0], iv[0][1], iv[0][2], px, qy, pz);
+  const float lpy = dot3(iv[1][0], iv[1][1], iv[1][2], px, qy, pz);
+  const float lpz = dot3(iv[2][0], iv[2][1], iv[2][2], px, qy, pz);
+  const float lvx = dot3(iv[0][0], iv[0][1], iv[0][2], vx, vy, vz);
+  const float lvy = dot3(iv[1][0], iv[1][1], iv[1][2], vx, vy, vz);
+  const float lvz = dot3(iv[2][0], iv[2][1], iv[2][2], vx, vy, vz);
+  const float bl = U->boundary_layer, k = U->wall_k, wd = 8.0f;
+  const float hw = U->hw, hh = U->hh, hd = U->hd;
+  float wx = 0.f, wy = 0.f, wz = 0.f;
+  float d = lpx - (-hw);
+  if (d < bl) { const float t = 1.0f - d / bl; wx += k * t * t; if (lvx < 0.f) wx -= lvx * wd * t; }
+  d = hw - lpx;
+  if (d < bl) { const float t = 1.0f - d / bl; wx -= k * t * t; if (lvx > 0.f) wx -= lvx * wd * t; }
+  d = lpy - (-hh);
+  if (d < bl) { const float t = 1.0f - d / bl; wy += k * t * t; if (lvy < 0.f) wy -= lvy * wd * t; }
+  d = hh - lpy;
+  if (d < bl) { const float t = 1.0f - d / bl; wy -= k * t * t; if (lvy > 0.f) wy -= lvy * wd * t; }
+  d = lpz - (-hd);
+  if (d < bl) { const float t = 1.0f - d / bl; wz += k * t * t; if (lvz < 0.f) wz -= lvz * wd * t; }
+  d = hd - lpz;
+  if (d < bl) { const float t = 1.0f - d / bl; wz -= k * t * t; if (lvz > 0.f) wz -= lvz * wd * t; }
+  float ax = 0.f + dot3(fw[0][0], fw[0][1], fw[0][2], wx, wy, wz);
+  float ay = 0.f + dot3(fw[1][0], fw[1][1], fw[1][2], wx, wy, wz);
+  float az = 0.f + dot3(fw[2][0], fw[2][1], fw[2][2], wx, wy, wz);
+
+  const GpuMouseForce &m = U->mouse;
+  if (m.is_active == 1u) {  // :163-200
+    const float tx = m.position[0] - px, ty = m.position[1] - py, tz = m.position[2] - pz;
+    const float dist = sqrtf(dot3(tx, ty, tz, tx, ty, tz));
+    if (dist < m.radius && dist > 0.001f) {
+      const float fall = 1.0f - (dist / m.radius);
+      const float fx = tx / dist, fy = ty / dist, fz = tz / dist;
+      const float sm = m.strength * fall;
+      if (m.mode == 1u) {
+        ax += fx * sm; ay += fy * sm; az += fz * sm;
+      } else if (m.mode == 2u) {
+        const float dl = sqrtf(dot3(m.direction[0], m.direction[1], m.direction[2],
+                                    m.direction[0], m.direction[1], m.direction[2]));
+        const float kx = m.direction[0] / dl, ky = m.direction[1] / dl, kz = m.direction[2] / dl;
+        const float gx = fy * kz - fz * ky, gy = fz * kx - fx * kz, gz = fx * ky - fy * kx;
+        const float tl = sqrtf(dot3(gx, gy, gz, gx, gy, gz));
+        if (tl > 0.001f) { ax += (gx / tl) * sm; ay += (gy / tl) * sm; az += (gz / tl) * sm; }
+      } else if (m.mode == 3u) {
+        ax -= fx * sm * 3.0f; ay -= fy * sm * 3.0f; az -= fz * sm * 3.0f;
+      } else if (m.mode == 4u) {
+        const float qx = fx, qy2 = fy + -1.0f, qz = fz;
+        const float ql = sqrtf(dot3(qx, qy2, qz, qx, qy2, qz));
+        ax += (qx / ql) * sm; ay += (qy2 / ql) * sm; az += (qz / ql) * sm;
+      } else {
+        ax -= fx * sm; ay -= fy * sm; az -= fz * sm;
+      }
+    }
+  }
+
+  const GpuRigidBody &rb = U->rb;
+  if (rb.is_active != 0u) {  // :203-341
+    const float wrx = px - rb.position[0], wry = py - rb.position[1], wrz = pz - rb.position[2];
+    const float rlx = dot3(rb.rot_row0[0], rb.rot_row0[1], rb.rot_row0[2], wrx, wry, wrz);
+    const float rly = dot3(rb.rot_row1[0], rb.rot_row1[1], rb.rot_row1[2], wrx, wry, wrz);
+    const float rlz = dot3(rb.rot_row2[0], rb.rot_row2[1], rb.rot_row2[2], wrx, wry, wrz);
+    const float he = rb.half_extent;
+    float sdf, lnx = 0.f, lny = 0.f, lnz = 0.f;
+    if (rb.shape == 1u) {
+      const float dist = sqrtf(dot3(rlx, rly, rlz, rlx, rly, rlz));
+      sdf = dist - he;
+      if (dist > 0.001f) { lnx = rlx / dist; lny = rly / dist; lnz = rlz / dist; }
+      else { lny = 1.f; }
+    } else if (rb.shape == 2u) {
+      const float rd = sqrtf(__builtin_fmaf(rlz, rlz, rlx * rlx));
+      const float d_rad = rd - he, d_cap = fabsf(rly) - he;
+      sdf = fmaxf(d_rad, d_cap);
+      if (d_rad > d_cap) {
+        if (rd > 0.001f) { lnx = rlx / rd; lnz = rlz / rd; } else { lnx = 1.f; }
+      } else {
+        lny = wsign(rly);
+      }
+    } else if (rb.shape == 3u) {
+      const float minor = he * 0.3f;
+      const float xz = sqrtf(__builtin_fmaf(rlz, rlz, rlx * rlx));
+      const float qx = xz - he, qy3 = rly;
+      const float ql = sqrtf(__builtin_fmaf(qy3, qy3, qx * qx));
+      sdf = ql - minor;
+      if (ql > 0.001f && xz > 0.001f) {
+        const float rdx = qx / ql, rdy = qy3 / ql;
+        lnx = rdx * rlx / xz; lny = rdy; lnz = rdx * rlz / xz;
+      } else {
+        lny = 1.f;
+      }
+    } else if (rb.shape == 4u) {
+      const float ux = (rlx / he) * 0.5f + 0.5f, uy = (rly / he) * 0.5f + 0.5f,
+                  uz = (rlz / he) * 0.5f + 0.5f;
+      const uint32_t res = U->sdf_res;
+      sdf = sdf_sample(b.sdf, res, ux, uy, uz) * he;
+      const float e = 1.0f / 32.0f;
+      const float gxv = sdf_sample(b.sdf, res, ux + e, uy, uz) - sdf_sample(b.sdf, res, ux - e, uy, uz);
+      const float gyv = sdf_sample(b.sdf, res, ux, uy + e, uz) - sdf_sample(b.sdf, res, ux, uy - e, uz);
+      const float gzv = sdf_sample(b.sdf, res, ux, uy, uz + e) - sdf_sample(b.sdf, res, ux, uy, uz - e);
+      const float gl = sqrtf(dot3(gxv, gyv, gzv, gxv, gyv, gzv));
+      if (gl > 1e-6f) { lnx = gxv / gl; lny = gyv / gl; lnz = gzv / gl; }
+      else {
+        const float qy4 = rly + 1e-6f;
+        const float l = sqrtf(dot3(rlx, qy4, rlz, rlx, qy4, rlz));
+        lnx = rlx / l; lny = qy4 / l; lnz = rlz / l;
+      }
+    } else {
+      const float dx = fabsf(rlx) - he, dy = fabsf(rly) - he, dz = fabsf(rlz) - he;
+      sdf = fmaxf(dx, fmaxf(dy, dz));
+      if (dx >= dy && dx >= dz) lnx = wsign(rlx);
+      else if (dy >= dx && dy >= dz) lny = wsign(rly);
+      else lnz = wsign(rlz);
+    }
+    const float ir = U->interact_range;
+    if (sdf < ir) {
+      const float nwx = rb.rot_row0[0] * lnx + rb.rot_row1[0] * lny + rb.rot_row2[0] * lnz;
+      const float nwy = rb.rot_row0[1] * lnx + rb.rot_row1[1] * lny + rb.rot_row2[1] * lnz;
+      const float nwz = rb.rot_row0[2] * lnx + rb.rot_row1[2] * lny + rb.rot_row2[2] * lnz;
+      const float t = (ir - sdf) / ir;
+      const float pnx = nwx * rb.stiffness * t * t, pny = nwy * rb.stiffness * t * t,
+                  pnz = nwz * rb.stiffness * t * t;
+      ax += pnx; ay += pny; az += pnz;
+      float dmx = 0.f, dmy = 0.f, dmz = 0.f;
+      const float vn = dot3(vx - rb.velocity[0], vy - rb.velocity[1], vz - rb.velocity[2],
+                            nwx, nwy, nwz);
+      if (vn < 0.f) {
+        dmx = -nwx * vn * 5.0f; dmy = -nwy * vn * 5.0f; dmz = -nwz * vn * 5.0f;
+        ax += dmx; ay += dmy; az += dmz;
+      }
+      const float mass = U->mass;
+      const float rex = -(pnx + dmx) * mass, rey = -(pny + dmy) * mass, rez = -(pnz + dmz) * mass;
+      atomicAdd(&b.accum[0], to_i32(rex * 1000.0f));
+      atomicAdd(&b.accum[1], to_i32(rey * 1000.0f));
+      atomicAdd(&b.accum[2], to_i32(rez * 1000.0f));
+      atomicAdd(reinterpret_cast<uint32_t *>(&b.accum[3]), 1u);
+      const float tqx = wry * rez - wrz * rey, tqy = wrz * rex - wrx * rez,
+                  tqz = wrx * rey - wry * rex;
+      atomicAdd(&b.accum[4], to_i32(tqx * 1000.0f));
+      atomicAdd(&b.accum[5], to_i32(tqy * 1000.0f));
+      atomicAdd(&b.accum[6], to_i32(tqz * 1000.0f));
+    }
+  }
+
+  const float dt = U->dt;
+  vx = vx + dt * ax; vy = vy + dt * ay; vz = vz + dt * az;
+  px = px + dt * vx; py = py + dt * vy; pz = pz + dt * vz;
+  // hard boundary backstop (:350-384)
+  const float qy5 = py - U->cy;
+  float nlx = dot3(iv[0][0], iv[0][1], iv[0][2], px, qy5, pz);
+  float nly = dot3(iv[1][0], iv[1][1], iv[1][2], px, qy5, pz);
+  float nlz = dot3(iv[2][0], iv[2][1], iv[2][2], px, qy5, pz);
+  float mvx = dot3(iv[0][0], iv[0][1], iv[0][2], vx, vy, vz);
+  float mvy = dot3(iv[1][0], iv[1][1], iv[1][2], vx, vy, vz);
+  float mvz = dot3(iv[2][0], iv[2][1], iv[2][2], vx, vy, vz);
+  const float rest = U->damping;
+  if (nlx < -hw) { nlx = -hw; mvx = fabsf(mvx) * rest; }
+  else if (nlx > hw) { nlx = hw; mvx = -fabsf(mvx) * rest; }
+  if (nly < -hh) { nly = -hh; mvy = fabsf(mvy) * rest; }
+  else if (nly > hh) { nly = hh; mvy = -fabsf(mvy) * rest; }
+  if (nlz < -hd) { nlz = -hd; mvz = fabsf(mvz) * rest; }
+  else if (nlz > hd) { nlz = hd; mvz = -fabsf(mvz) * rest; }
+  const float ox = dot3(fw[0][0], fw[0][1], fw[0][2], nlx, nly, nlz);
+  const float oy = dot3(fw[1][0], fw[1][1], fw[1][2], nlx, nly, nlz) + U->cy;
+  const float oz = dot3(fw[2][0], fw[2][1], fw[2][2], nlx, nly, nlz);
+  b.pos[s] = make_float4(ox, oy, oz, 0.f);
+  b.vel[s] = make_float4(dot3(fw[0][0], fw[0][1], fw[0][2], mvx, mvy, mvz),
+                         dot3(fw[1][0], fw[1][1], fw[1][2], mvx, mvy, mvz),
+                         dot3(fw[2][0], fw[2][1], fw[2][2], mvx, mvy, mvz), 0.f);
+}
+
+// Candidate / in-range pair totals of the density stencil.
+__global__ __launch_bounds__(256) void k_count_pairs(const DevUniforms *__restrict__ U,
+                                                     Bufs b, uint32_t n,
+                                                     unsigned long long *out2) {
+  const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+  unsigned long long cand = 0, inr = 0;
+  if (s < n) {
+    const float4 pi = b.spos[s];
+    const int cx = cell_coord(pi.x, U->ox, U->inv_cell);
+    const int cy = cell_coord(pi.y, U->oy, U->inv_cell);
+    const int cz = cell_coord(pi.z, U->oz, U->inv_cell);
+    const float h2 = U->h2;
+    sweep27<1>(U, b.counts, b.starts, cx, cy, cz, 0, [&](uint32_t j) {
+      const float4 pj = b.spos[j];
+      const float rx = pi.x - pj.x, ry = pi.y - pj.y, rz = pi.z - pj.z;
+      ++cand;
+      if (dot3(rx, ry, rz, rx, ry, rz) < h2) ++inr;
+    });
+  }
+  for (int o = 32; o >= 1; o >>= 1) {
+    cand += __shfl_xor(cand, o);
+    inr += __shfl_xor(inr, o);
+  }
+  if ((threadIdx.x & 63) == 0) {
+    atomicAdd(&out2[0], cand);
+    atomicAdd(&out2[1], inr);
+  }
+}
+
+inline uint32_t blocks_for(uint64_t threads, uint32_t per) {
+  return (uint32_t)((threads + per - 1) / per);
+}
+
+template <template <int> class K>
+struct Dummy {};
+
+}  // namespace
+
+hipError_t launch_clear(const Bufs &b, uint32_t cells, uint32_t ntiles, hipStream_t st) {
+  const uint32_t work = std::max(cells / 4 + 4, ntiles + 1);
+  hipLaunchKernelGGL(k_clear, dim3(blocks_for(work, 256)), dim3(256), 0, st, b.counts,
+                     cells, b.tile_status, ntiles, b.tile_counter);
+  return hipGetLastError();
+}
+
+hipError_t launch_build(const DevUniforms *u, const Bufs &b, uint32_t n, hipStream_t st) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_build, dim3(blocks_for(n, 256)), dim3(256), 0, st, u, b, n);
+  return hipGetLastError();
+}
+
+hipError_t launch_scan(const DevUniforms *, const Bufs &b, uint32_t cells, hipStream_t st) {
+  hipLaunchKernelGGL(k_scan, dim3(blocks_for(cells, kScanTile)), dim3(kScanThreads), 0, st,
+                     b, cells);
+  return hipGetLastError();
+}
+
+hipError_t launch_reorder(const DevUniforms *, const Bufs &b, uint32_t n, hipStream_t st) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_scatter, dim3(blocks_for(n, 256)), dim3(256), 0, st, b, n);
+  hipLaunchKernelGGL(k_rank_gather, dim3(blocks_for(n, 256)), dim3(256), 0, st, b, n);
+  return hipGetLastError();
+}
+
+#define FLUID_DISPATCH_L(KERNEL, CFG, N, ...)                                         \
+  do {                                                                                \
+    const uint32_t _grid = blocks_for((uint64_t)(N) * (CFG).lanes, 256);              \
+    switch ((CFG).lanes) {                                                            \
+      case 1: hipLaunchKernelGGL(KERNEL<1>, dim3(_grid), dim3(256), 0, st, __VA_ARGS__); break; \
+      case 2: hipLaunchKernelGGL(KERNEL<2>, dim3(_grid), dim3(256), 0, st, __VA_ARGS__); break; \
+      case 4: hipLaunchKernelGGL(KERNEL<4>, dim3(_grid), dim3(256), 0, st, __VA_ARGS__); break; \
+      default: hipLaunchKernelGGL(KERNEL<8>, dim3(_grid), dim3(256), 0, st, __VA_ARGS__); break; \
+    }                                                                                 \
+  } while (0)
+
+hipError_t launch_density(const DevUniforms *u, const Bufs &b, uint32_t n, SweepCfg c,
+                          hipStream_t st) {
+  if (n == 0) return hipSuccess;
+  FLUID_DISPATCH_L(k_density, c, n, u, b, n);
+  return hipGetLastError();
+}
+
+hipError_t launch_xsph(const DevUniforms *u, const Bufs &b, uint32_t n, SweepCfg c,
+                       hipStream_t st) {
+  if (n == 0) return hipSuccess;
+  FLUID_DISPATCH_L(k_xsph, c, n, u, b, n);
+  return hipGetLastError();
+}
+
+hipError_t launch_force(const DevUniforms *u, const Bufs &b, uint32_t n, SweepCfg c,
+                        hipStream_t st) {
+  if (n == 0) return hipSuccess;
+  FLUID_DISPATCH_L(k_force, c, n, u, b, n);
+  return hipGetLastError();
+}
+
+hipError_t launch_predict(const DevUniforms *u, const Bufs &b, uint32_t n, hipStream_t st) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_predict, dim3(blocks_for(n, 256)), dim3(256), 0, st, u, b, n);
+  return hipGetLastError();
+}
+
+hipError_t launch_solve(const DevUniforms *u, const Bufs &b, uint32_t n, int in_buf,
+                        SweepCfg c, hipStream_t st) {
+  if (n == 0) return hipSuccess;
+  const PredBuf in = b.pred[in_buf], out = b.pred[1 - in_buf];
+  FLUID_DISPATCH_L(k_solve, c, n, u, b, n, in, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_finalize(const DevUniforms *, const Bufs &b, uint32_t n, int fin_buf,
+                           hipStream_t st) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_finalize, dim3(blocks_for(n, 256)), dim3(256), 0, st, b, n,
+                     b.pred[fin_buf]);
+  return hipGetLastError();
+}
+
+hipError_t launch_integrate(const DevUniforms *u, const Bufs &b, uint32_t n, hipStream_t st) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_integrate, dim3(blocks_for(n, 256)), dim3(256), 0, st, u, b, n);
+  return hipGetLastError();
+}
+
+hipError_t launch_count_pairs(const DevUniforms *u, const Bufs &b, uint32_t n,
+                              unsigned long long *out2, hipStream_t st) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_count_pairs, dim3(blocks_for(n, 256)), dim3(256), 0, st, u, b, n,
+                     out2);
+  return hipGetLastError();
+}
+
+}  // namespace fluid

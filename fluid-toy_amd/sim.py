@@ -1,0 +1,235 @@
+"""Python mirror of SphSimulation3DGrid (src/simulation/sph_3d_grid.rs:33-1570)
+over the C ABI of libfluidhip.so.
+
+Method names follow the reference struct; buffers that the reference exposes
+as wgpu::Buffer getters (:1500-1526) are exposed as readbacks returning
+numpy arrays.  Particle records are (N, 16) float32 arrays in the 64-byte
+SphParticle3D layout (columns: abi.POS, abi.VEL, abi.FORCE, abi.DENSITY,
+abi.NEAR, abi.NORMAL).
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import abi
+
+
+def derive_sph_params(kernel_radius: float, stiffness: float, near_stiffness: float,
+                      viscosity: float, mass: float, surface_tension: float,
+                      xsph_epsilon: float, dt: float, num_particles: int,
+                      rest_density: float | None = None) -> abi.GpuSphParams3D:
+    """SphConfig::to_gpu_params_3d (src/state/simulation.rs:315-336); a given
+    rest_density replaces compute_rest_density and feeds compute_pcisph_delta."""
+    lib = abi.load_library()
+    out = abi.GpuSphParams3D()
+    abi.check(lib, lib.fluid_derive_sph_params(
+        kernel_radius, stiffness, near_stiffness, viscosity, mass, surface_tension,
+        xsph_epsilon, dt, num_particles, float(rest_density or 0.0), C.byref(out)))
+    return out
+
+
+def derive_container_geometry(width: float, height: float, depth: float, floor_y: float,
+                              tilt_x: float = 0.0, tilt_z: float = 0.0,
+                              wall_stiffness: float = 200.0, damping: float = 0.55,
+                              clip_enabled: bool = False, clip_margin: float = 0.0
+                              ) -> abi.GpuContainerGeometry:
+    """ContainerConfig::to_gpu_geometry (src/state/simulation.rs:139-184)."""
+    lib = abi.load_library()
+    out = abi.GpuContainerGeometry()
+    abi.check(lib, lib.fluid_derive_container_geometry(
+        width, height, depth, floor_y, tilt_x, tilt_z, wall_stiffness, damping,
+        1 if clip_enabled else 0, clip_margin, C.byref(out)))
+    return out
+
+
+def gravity(x: float, y: float, z: float) -> abi.GpuGravity:
+    g = abi.GpuGravity()
+    g.direction[0], g.direction[1], g.direction[2] = x, y, z
+    return g
+
+
+class SphSimulation3DGrid:
+    """One simulation handle on one HIP device (SphSimulation3DGrid::new,
+    sph_3d_grid.rs:104-1258)."""
+
+    def __init__(self, particles: np.ndarray, sph_params: abi.GpuSphParams3D,
+                 container_geom: abi.GpuContainerGeometry, max_particles: int = 0,
+                 sdf: np.ndarray | None = None, bounds_extent: float = 0.0,
+                 device: int = 0):
+        self._lib = abi.load_library()
+        parts = np.ascontiguousarray(particles, dtype=np.float32).reshape(-1, 16)
+        self._keep = parts
+        info = abi.FluidCreateInfo()
+        info.particles = parts.ctypes.data if len(parts) else None
+        info.num_particles = len(parts)
+        info.max_particles = max(int(max_particles), len(parts))
+        info.sph_params = sph_params
+        info.container = container_geom
+        info.bounds_extent = float(bounds_extent)
+        info.device = int(device)
+        if sdf is not None:
+            vox = np.ascontiguousarray(sdf, dtype=np.float32)
+            if vox.ndim != 3 or len(set(vox.shape)) != 1:
+                raise ValueError("sdf must be a cubic (res, res, res) array indexed [z, y, x]")
+            self._sdf = vox
+            info.sdf_voxels = vox.ctypes.data
+            info.sdf_resolution = vox.shape[0]
+        handle = C.c_void_p()
+        abi.check(self._lib, self._lib.fluid_create(C.byref(info), C.byref(handle)))
+        self._h = handle
+
+    # -- lifetime ---------------------------------------------------------
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            self._lib.fluid_destroy(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _chk(self, rc: int) -> None:
+        abi.check(self._lib, rc)
+
+    # -- the reference's surface ------------------------------------------
+    def step(self, n: int = 1) -> None:
+        """step() (sph_3d_grid.rs:1261); n > 1 runs the substep loop."""
+        self._chk(self._lib.fluid_step_n(self._h, n))
+
+    def step_stages(self, first: int | str, last: int | str) -> None:
+        f = abi.STAGE[first] if isinstance(first, str) else int(first)
+        l = abi.STAGE[last] if isinstance(last, str) else int(last)
+        self._chk(self._lib.fluid_step_stages(self._h, f, l))
+
+    def update_sph_params(self, params: abi.GpuSphParams3D) -> None:
+        self._chk(self._lib.fluid_update_sph_params(self._h, C.byref(params)))
+
+    def update_container_geometry(self, geom: abi.GpuContainerGeometry) -> None:
+        self._chk(self._lib.fluid_update_container_geometry(self._h, C.byref(geom)))
+
+    def update_mouse_force(self, mouse: abi.GpuMouseForce) -> None:
+        self._chk(self._lib.fluid_update_mouse_force(self._h, C.byref(mouse)))
+
+    def update_gravity(self, g: abi.GpuGravity) -> None:
+        self._chk(self._lib.fluid_update_gravity(self._h, C.byref(g)))
+
+    def update_rigid_body(self, body: abi.GpuRigidBody) -> None:
+        self._chk(self._lib.fluid_update_rigid_body(self._h, C.byref(body)))
+
+    def set_pcisph_iterations(self, iterations: int) -> None:
+        self._chk(self._lib.fluid_set_pcisph_iterations(self._h, iterations))
+
+    @property
+    def pcisph_iterations(self) -> int:
+        return self._lib.fluid_pcisph_iterations(self._h)
+
+    def clear_rigid_body_accum(self) -> None:
+        self._chk(self._lib.fluid_clear_rigid_body_accum(self._h))
+
+    def read_rigid_body_accum(self) -> abi.GpuRigidBodyAccum:
+        out = abi.GpuRigidBodyAccum()
+        self._chk(self._lib.fluid_read_rigid_body_accum(self._h, C.byref(out)))
+        self._last_accum = out
+        return out
+
+    def rigid_body_accum(self) -> abi.GpuRigidBodyAccum:
+        return getattr(self, "_last_accum", abi.GpuRigidBodyAccum())
+
+    def spawn_particles(self, position, count: int, spread: float) -> int:
+        pos = (C.c_float * 3)(*[float(v) for v in position])
+        got = C.c_uint32(0)
+        self._chk(self._lib.fluid_spawn_particles(self._h, pos, count, spread, C.byref(got)))
+        return got.value
+
+    def num_particles(self) -> int:
+        return self._lib.fluid_num_particles(self._h)
+
+    def max_particles(self) -> int:
+        return self._lib.fluid_max_particles(self._h)
+
+    # -- buffer access -----------------------------------------------------
+    def _read_records(self, fn) -> np.ndarray:
+        n = self.num_particles()
+        out = np.zeros((n, 16), dtype=np.float32)
+        self._chk(fn(self._h, out.ctypes.data if n else None, n))
+        return out
+
+    def read_particles(self) -> np.ndarray:
+        return self._read_records(self._lib.fluid_read_particles)
+
+    def write_particles(self, particles: np.ndarray) -> None:
+        parts = np.ascontiguousarray(particles, dtype=np.float32).reshape(-1, 16)
+        self._chk(self._lib.fluid_write_particles(
+            self._h, parts.ctypes.data if len(parts) else None, len(parts)))
+
+    def read_sorted_particles(self) -> np.ndarray:
+        return self._read_records(self._lib.fluid_read_sorted_particles)
+
+    def grid_params(self) -> abi.GpuGridParams:
+        g = abi.GpuGridParams()
+        self._chk(self._lib.fluid_read_grid_params(self._h, C.byref(g)))
+        return g
+
+    def sph_params(self) -> abi.GpuSphParams3D:
+        p = abi.GpuSphParams3D()
+        self._chk(self._lib.fluid_read_sph_params(self._h, C.byref(p)))
+        return p
+
+    def container_geometry(self) -> abi.GpuContainerGeometry:
+        g = abi.GpuContainerGeometry()
+        self._chk(self._lib.fluid_read_container_geometry(self._h, C.byref(g)))
+        return g
+
+    def _read_u32(self, fn, count: int) -> np.ndarray:
+        out = np.zeros(count, dtype=np.uint32)
+        self._chk(fn(self._h, out.ctypes.data if count else None, count))
+        return out
+
+    def read_cell_counts(self) -> np.ndarray:
+        return self._read_u32(self._lib.fluid_read_cell_counts, self.grid_params().total_cells)
+
+    def read_cell_starts(self) -> np.ndarray:
+        return self._read_u32(self._lib.fluid_read_cell_starts, self.grid_params().total_cells)
+
+    def read_sorted_to_orig(self) -> np.ndarray:
+        return self._read_u32(self._lib.fluid_read_sorted_to_orig, self.num_particles())
+
+    def read_orig_to_sorted(self) -> np.ndarray:
+        return self._read_u32(self._lib.fluid_read_orig_to_sorted, self.num_particles())
+
+    def read_predicted(self, which: int) -> np.ndarray:
+        """sorted_predicted_a (0) / _b (1) as (N, 8) float32:
+        pred_pos xyz, pressure, pred_vel xyz, pred_density."""
+        n = self.num_particles()
+        out = np.zeros((n, 8), dtype=np.float32)
+        self._chk(self._lib.fluid_read_predicted(self._h, which, out.ctypes.data if n else None, n))
+        return out
+
+    def sync(self) -> None:
+        self._chk(self._lib.fluid_sync(self._h))
+
+    # -- measurement -------------------------------------------------------
+    def time_steps(self, warmup: int, steps: int) -> float:
+        ms = C.c_float(0.0)
+        self._chk(self._lib.fluid_time_steps(self._h, warmup, steps, C.byref(ms)))
+        return ms.value
+
+    def time_stages(self, steps: int) -> dict:
+        arr = (C.c_float * len(abi.STAGES))()
+        self._chk(self._lib.fluid_time_stages(self._h, steps, arr))
+        return {name: arr[i] for i, name in enumerate(abi.STAGES)}
+
+    def count_pairs(self) -> tuple[int, int]:
+        a, b = C.c_uint64(0), C.c_uint64(0)
+        self._chk(self._lib.fluid_count_pairs(self._h, C.byref(a), C.byref(b)))
+        return a.value, b.value

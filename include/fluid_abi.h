@@ -1,0 +1,334 @@
+/*
+ * fluid_abi.h — C ABI of the MI355X (gfx950) SPH step library `libfluidhip.so`.
+ *
+ * The library is a drop-in for the compute side of fluid-toy's
+ * `SphSimulation3DGrid` (src/simulation/sph_3d_grid.rs:33-1570): it owns every
+ * device buffer of the step and a HIP stream, runs the same 13 compute entry
+ * points per step (grid clear/build, prefix scan, reorder, density, XSPH,
+ * force, PCISPH predict/solve xN/finalize, integrate) as hand-written HIP
+ * kernels, and exposes the struct's public methods as plain C functions.
+ *
+ * Conventions
+ *  - Every function returns an int status: FLUID_OK (0) or a negative
+ *    FLUID_ERR_* code; fluid_last_error() returns a thread-local message.
+ *    The library never aborts the process (the reference panics instead,
+ *    src/gpu/mod.rs:33,45).
+ *  - The caller owns host arrays; they are copied in and out during the call
+ *    and no pointer is retained.  Structs are passed by const pointer.
+ *  - One handle is externally serialised.  GPU work is stream-ordered and
+ *    asynchronous; readbacks synchronise the handle's stream (the reference's
+ *    only sync point is device.poll in read_rigid_body_accum,
+ *    sph_3d_grid.rs:1486).
+ *  - Plain pointers and sizes only; no C++ or torch types cross the boundary.
+ *
+ * Every POD struct below is byte-identical to its Rust/WGSL counterpart
+ * (static_asserts in fluid-toy_amd/csrc/fluid_types.h).
+ */
+#ifndef FLUID_ABI_H
+#define FLUID_ABI_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FLUID_ABI_VERSION 1
+
+/* ---- status codes ---------------------------------------------------- */
+#define FLUID_OK 0
+#define FLUID_ERR_INVALID_ARG (-1) /* null pointer, bad size, bad value   */
+#define FLUID_ERR_CAPACITY (-2)    /* request exceeds max_particles/cap    */
+#define FLUID_ERR_HIP (-3)         /* a HIP runtime call failed            */
+#define FLUID_ERR_NO_DEVICE (-4)   /* no usable gfx950 device             */
+#define FLUID_ERR_STATE (-5)       /* call not valid in the current state  */
+
+/* ---- POD structs (byte layouts of the reference) --------------------- */
+
+/* src/simulation/particle.rs:6-17 (64 B, WGSL vec3 padding) */
+typedef struct SphParticle3D {
+  float position[3];
+  float _pad0;
+  float velocity[3];
+  float _pad1;
+  float force[3];
+  float density;
+  float near_density;
+  float normal[3];
+} SphParticle3D;
+
+/* src/state/simulation.rs:342-361 (64 B) */
+typedef struct GpuSphParams3D {
+  float kernel_radius;
+  float kernel_radius_sq;
+  float kernel_radius_pow5;
+  float kernel_radius_pow6;
+  float kernel_radius_pow9;
+  float mass;
+  float rest_density;
+  float stiffness; /* copied into the uniform, read by no kernel */
+  float near_stiffness;
+  float viscosity;
+  float dt;
+  uint32_t num_particles;
+  float surface_tension;
+  float pcisph_delta;
+  float xsph_epsilon;
+  float _padding_st;
+} GpuSphParams3D;
+
+/* src/simulation/sph_3d_grid.rs:16-30 (48 B) */
+typedef struct GpuGridParams {
+  uint32_t grid_size_x;
+  uint32_t grid_size_y;
+  uint32_t grid_size_z;
+  uint32_t total_cells;
+  float cell_size;
+  float inv_cell_size;
+  float grid_origin_x;
+  float grid_origin_y;
+  float grid_origin_z;
+  uint32_t num_particles;
+  uint32_t _padding[2];
+} GpuGridParams;
+
+/* src/state/simulation.rs:365-386 (128 B) */
+typedef struct GpuContainerGeometry {
+  float half_width;
+  float half_height;
+  float half_depth;
+  float center_y;
+  float forward_row0[4];
+  float forward_row1[4];
+  float forward_row2[4];
+  float inverse_row0[4];
+  float inverse_row1[4];
+  float inverse_row2[4];
+  float wall_stiffness;
+  float damping;
+  uint32_t clip_enabled;
+  float clip_margin;
+} GpuContainerGeometry;
+
+/* src/state/simulation.rs:389-394 (16 B) */
+typedef struct GpuGravity {
+  float direction[3];
+  float _padding;
+} GpuGravity;
+
+/* src/state/interaction.rs:102-111 (48 B) */
+typedef struct GpuMouseForce {
+  float position[3];
+  float radius;
+  float strength;
+  uint32_t is_active;
+  uint32_t mode; /* 0 push, 1 pull, 2 vortex, 3 explode, 4 drain */
+  float _pad;
+  float direction[3];
+  float _pad2;
+} GpuMouseForce;
+
+/* src/state/rigid_body.rs:212-224 (96 B) */
+typedef struct GpuRigidBody {
+  float position[3];
+  float half_extent;
+  float velocity[3];
+  uint32_t is_active;
+  float stiffness;
+  uint32_t shape; /* 0 cube, 1 sphere, 2 cylinder, 3 torus, 4 custom SDF */
+  float _pad1;
+  float _pad2;
+  float rot_row0[4];
+  float rot_row1[4];
+  float rot_row2[4];
+} GpuRigidBody;
+
+/* src/state/rigid_body.rs:247-256 (32 B; fixed point x1000) */
+typedef struct GpuRigidBodyAccum {
+  int32_t force_x;
+  int32_t force_y;
+  int32_t force_z;
+  uint32_t contact_count;
+  int32_t torque_x;
+  int32_t torque_y;
+  int32_t torque_z;
+  uint32_t _pad;
+} GpuRigidBodyAccum;
+
+/* src/shaders/pcisph_predict.wgsl:33-42 (32 B) */
+typedef struct PredictedState {
+  float pred_pos[3];
+  float pressure;
+  float pred_vel[3];
+  float pred_density;
+} PredictedState;
+
+/* Parameters of SphSimulation3DGrid::new (sph_3d_grid.rs:104-112).
+ * `bounds_extent` sizes the dense cubic grid exactly as the reference does
+ * (sph_3d_grid.rs:121-143): cell = h, G = ceil(2*extent/h)+2 per axis,
+ * origin = -(extent+h).  0 selects the reference's fixed 2.0.  Larger scenes
+ * (boxes wider than 4.0) pass a larger extent.  `sdf_voxels` (res^3 floats,
+ * may be null) is the custom-shape rigid-body SDF of sph_3d_grid.rs:346-384. */
+typedef struct fluid_create_info {
+  const SphParticle3D *particles;
+  uint32_t num_particles;
+  uint32_t max_particles; /* raised to num_particles if smaller (:115) */
+  GpuSphParams3D sph_params;
+  GpuContainerGeometry container;
+  float bounds_extent;
+  int device; /* HIP device ordinal */
+  const float *sdf_voxels;
+  uint32_t sdf_resolution;
+} fluid_create_info;
+
+typedef struct fluid_sim fluid_sim;
+
+/* ---- stage ids of one step (sph_3d_grid.rs:1269-1432, in order) ------ */
+#define FLUID_STAGE_CLEAR 0     /* grid_clear.wgsl + clear_buffer(:1281)     */
+#define FLUID_STAGE_BUILD 1     /* grid_build.wgsl                           */
+#define FLUID_STAGE_SCAN 2      /* copy(:1295) + prefix_scan.wgsl x3         */
+#define FLUID_STAGE_REORDER 3   /* grid_reorder.wgsl (canonical order)       */
+#define FLUID_STAGE_DENSITY 4   /* sph_density_3d_grid.wgsl                  */
+#define FLUID_STAGE_XSPH 5      /* xsph.wgsl                                 */
+#define FLUID_STAGE_FORCE 6     /* sph_force_3d_grid.wgsl                    */
+#define FLUID_STAGE_PREDICT 7   /* pcisph_predict.wgsl                       */
+#define FLUID_STAGE_SOLVE 8     /* pcisph_solve.wgsl x pcisph_iterations     */
+#define FLUID_STAGE_FINALIZE 9  /* pcisph_finalize.wgsl                      */
+#define FLUID_STAGE_INTEGRATE 10 /* sph_integrate_3d.wgsl                    */
+#define FLUID_STAGE_COUNT 11
+
+/* ---- library --------------------------------------------------------- */
+
+/* Thread-local message of the last failing call ("" if none). */
+const char *fluid_last_error(void);
+
+/* FLUID_ABI_VERSION of the loaded library. */
+int fluid_abi_version(void);
+
+/* Parameter derivation of SphConfig::to_gpu_params_3d
+ * (src/state/simulation.rs:315-336): powers of h by repeated f32 multiply
+ * (Rust powi), rest density from compute_rest_density (:402-425) unless
+ * rest_density_override > 0, PCISPH delta from compute_pcisph_delta
+ * (:429-480) evaluated with that rest density. */
+int fluid_derive_sph_params(float kernel_radius, float stiffness,
+                            float near_stiffness, float viscosity, float mass,
+                            float surface_tension, float xsph_epsilon,
+                            float dt, uint32_t num_particles,
+                            float rest_density_override, GpuSphParams3D *out);
+
+/* ContainerConfig::to_gpu_geometry (src/state/simulation.rs:139-184). */
+int fluid_derive_container_geometry(float width, float height, float depth,
+                                    float floor_y, float tilt_x, float tilt_z,
+                                    float wall_stiffness, float damping,
+                                    uint32_t clip_enabled, float clip_margin,
+                                    GpuContainerGeometry *out);
+
+/* ---- SphSimulation3DGrid surface -------------------------------------- */
+
+/* SphSimulation3DGrid::new (sph_3d_grid.rs:104-1258).  Gravity starts at
+ * (0,-9.8,0) (:277-280), mouse force and rigid body at their inactive
+ * defaults (:270-292), pcisph_iterations at 5 (:1253). */
+int fluid_create(const fluid_create_info *info, fluid_sim **out);
+
+/* Drop the handle and all device memory. Null is accepted. */
+int fluid_destroy(fluid_sim *sim);
+
+/* SphSimulation3DGrid::step (sph_3d_grid.rs:1261-1444): one full step,
+ * launched as a captured HIP graph on the handle's stream. */
+int fluid_step(fluid_sim *sim);
+
+/* `n` consecutive steps (the substep loop of app.rs:1211-1213). */
+int fluid_step_n(fluid_sim *sim, uint32_t n);
+
+/* Run stages [first, last] of one step with plain launches (no graph), for
+ * per-stage inspection; a full step is first=0, last=FLUID_STAGE_COUNT-1. */
+int fluid_step_stages(fluid_sim *sim, uint32_t first, uint32_t last);
+
+/* update_sph_params (sph_3d_grid.rs:1446-1453).  num_particles in the
+ * struct is ignored: the handle's own count drives every stage. */
+int fluid_update_sph_params(fluid_sim *sim, const GpuSphParams3D *params);
+/* update_container_geometry (:1455) */
+int fluid_update_container_geometry(fluid_sim *sim,
+                                    const GpuContainerGeometry *geom);
+/* update_mouse_force (:1459) */
+int fluid_update_mouse_force(fluid_sim *sim, const GpuMouseForce *mouse);
+/* update_gravity (:1463) */
+int fluid_update_gravity(fluid_sim *sim, const GpuGravity *gravity);
+/* update_rigid_body (:1467) */
+int fluid_update_rigid_body(fluid_sim *sim, const GpuRigidBody *body);
+/* set_pcisph_iterations (:1471-1473), clamped to >= 1 */
+int fluid_set_pcisph_iterations(fluid_sim *sim, uint32_t iterations);
+uint32_t fluid_pcisph_iterations(const fluid_sim *sim);
+/* clear_rigid_body_accum (:1475-1481) */
+int fluid_clear_rigid_body_accum(fluid_sim *sim);
+/* read_rigid_body_accum + rigid_body_accum (:1483-1498), blocking. */
+int fluid_read_rigid_body_accum(fluid_sim *sim, GpuRigidBodyAccum *out);
+
+/* spawn_particles (:1534-1569): golden-angle spiral, clamps to capacity;
+ * *spawned receives the count actually added (may be 0). */
+int fluid_spawn_particles(fluid_sim *sim, const float position[3],
+                          uint32_t count, float spread, uint32_t *spawned);
+
+/* num_particles (:1528) */
+uint32_t fluid_num_particles(const fluid_sim *sim);
+uint32_t fluid_max_particles(const fluid_sim *sim);
+
+/* ---- buffer access (the getters of :1500-1526) ------------------------ */
+/* All readbacks synchronise the stream.  `cap` is the element capacity of
+ * `out`; FLUID_ERR_CAPACITY if it is smaller than the buffer. */
+
+/* particle_buffer: canonical (original index) order, 64-B records. */
+int fluid_read_particles(fluid_sim *sim, SphParticle3D *out, uint32_t cap);
+/* queue.write_buffer(particle_buffer, 0, ..) of `n` records (n <= max). */
+int fluid_write_particles(fluid_sim *sim, const SphParticle3D *in,
+                          uint32_t n);
+/* sorted_particle_buffer: grid-sorted order as left by the last reorder. */
+int fluid_read_sorted_particles(fluid_sim *sim, SphParticle3D *out,
+                                uint32_t cap);
+/* cell_counts_buffer / cell_starts_buffer (inclusive scan: cell range end). */
+int fluid_read_cell_counts(fluid_sim *sim, uint32_t *out, uint32_t cap);
+int fluid_read_cell_starts(fluid_sim *sim, uint32_t *out, uint32_t cap);
+/* sorted slot -> original index, and its inverse (the reorder's two maps,
+ * grid_reorder.wgsl:56-62). */
+int fluid_read_sorted_to_orig(fluid_sim *sim, uint32_t *out, uint32_t cap);
+int fluid_read_orig_to_sorted(fluid_sim *sim, uint32_t *out, uint32_t cap);
+/* sorted_predicted_{a,b} (which = 0 or 1), sorted order. */
+int fluid_read_predicted(fluid_sim *sim, uint32_t which, PredictedState *out,
+                         uint32_t cap);
+/* grid_params_buffer / sph_params_buffer / container_geom_buffer */
+int fluid_read_grid_params(const fluid_sim *sim, GpuGridParams *out);
+int fluid_read_sph_params(const fluid_sim *sim, GpuSphParams3D *out);
+int fluid_read_container_geometry(const fluid_sim *sim,
+                                  GpuContainerGeometry *out);
+
+/* Wait for all queued work of the handle. */
+int fluid_sync(fluid_sim *sim);
+
+/* The handle's hipStream_t, as an opaque pointer (for interop). */
+void *fluid_stream(fluid_sim *sim);
+
+/* ---- measurement ------------------------------------------------------ */
+
+/* Time `steps` graph-launched steps with HIP events on the handle's stream
+ * (after `warmup` untimed steps); *ms_total receives the elapsed time. */
+int fluid_time_steps(fluid_sim *sim, uint32_t warmup, uint32_t steps,
+                     float *ms_total);
+
+/* Run `steps` steps with plain launches and a HIP event pair around every
+ * stage kernel on the handle's stream; ms_per_stage[FLUID_STAGE_COUNT]
+ * receives the mean duration of each stage per step (SOLVE summed over its
+ * iterations). */
+int fluid_time_stages(fluid_sim *sim, uint32_t steps, float *ms_per_stage);
+
+/* Candidate and in-range neighbour pair totals of the current sorted state
+ * (the 27-cell sweep of sph_density_3d_grid.wgsl:129-167 counted, not
+ * summed); used to turn stage times into flop rates. */
+int fluid_count_pairs(fluid_sim *sim, uint64_t *candidates,
+                      uint64_t *in_range);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FLUID_ABI_H */

@@ -1,0 +1,1015 @@
+// fluid_oracle.cpp — CPU restatement of fluid-toy's SPH step, used ONLY as a
+// test oracle and as the reported CPU baseline.  It is test infrastructure:
+// nothing in the product library links or calls it.
+//
+// It restates, stage by stage and in the same floating-point operation order,
+// the WGSL compute entry points that SphSimulation3DGrid::step dispatches
+// (src/simulation/sph_3d_grid.rs:1261-1444):
+//   grid_clear.wgsl, grid_build.wgsl, prefix_scan.wgsl, grid_reorder.wgsl,
+//   sph_density_3d_grid.wgsl (+ container_common.wgsl), xsph.wgsl,
+//   sph_force_3d_grid.wgsl, pcisph_predict.wgsl, pcisph_solve.wgsl,
+//   pcisph_finalize.wgsl, sph_integrate_3d.wgsl.
+// One deliberate difference: the reorder's within-cell order, which the
+// reference takes from a racing atomicAdd (grid_reorder.wgsl:51), is fixed
+// to ascending particle index (a stable counting sort).
+//
+// It is built twice from one template: T=float (compile with
+// -ffp-contract=off so every WGSL operation rounds once, in source order) and
+// T=double (the FP64 transcription used for rollout divergence and
+// gradients).  The parameter structs are the reference's f32 uniforms; the
+// FP64 variant promotes their values.
+//
+// Parity status of each function is recorded next to it ("pinned by ...").
+// What only running the real wgpu pipeline could pin — naga/driver rounding
+// of the sums, sin/cos of the close-range fallback direction, the racing
+// reorder order — is not pinned here (no Rust/Vulkan toolchain exists here).
+
+#include "../include/fluid_abi.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <vector>
+
+namespace {
+
+template <class T> struct V3 {
+  T x, y, z;
+};
+template <class T> inline V3<T> mk(T x, T y, T z) { return V3<T>{x, y, z}; }
+template <class T> inline V3<T> add(V3<T> a, V3<T> b) {
+  return mk(a.x + b.x, a.y + b.y, a.z + b.z);
+}
+template <class T> inline V3<T> sub(V3<T> a, V3<T> b) {
+  return mk(a.x - b.x, a.y - b.y, a.z - b.z);
+}
+// scalar * vector (WGSL `s * v`)
+template <class T> inline V3<T> smul(T s, V3<T> v) {
+  return mk(s * v.x, s * v.y, s * v.z);
+}
+// vector * scalar (WGSL `v * s`)
+template <class T> inline V3<T> muls(V3<T> v, T s) {
+  return mk(v.x * s, v.y * s, v.z * s);
+}
+template <class T> inline V3<T> divs(V3<T> v, T s) {
+  return mk(v.x / s, v.y / s, v.z / s);
+}
+// WGSL dot() is evaluated fused, as fma(z, z', fma(y, y', x*x')): the
+// lowering GPU compilers give it, and the one the HIP kernels use, so that
+// every in-range test (r^2 < h^2) is decided on identical bits.
+template <class T> inline T dot(V3<T> a, V3<T> b) {
+  return std::fma(a.z, b.z, std::fma(a.y, b.y, a.x * b.x));
+}
+template <class T> inline T length(V3<T> a) { return std::sqrt(dot(a, a)); }
+template <class T> inline V3<T> normalize(V3<T> a) {
+  return divs(a, length(a));
+}
+template <class T> inline V3<T> cross(V3<T> a, V3<T> b) {
+  return mk(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z,
+            a.x * b.y - a.y * b.x);
+}
+template <class T> inline T clampv(T v, T lo, T hi) {
+  return std::min(std::max(v, lo), hi);
+}
+template <class T> inline T wsign(T v) {
+  return v > T(0) ? T(1) : (v < T(0) ? T(-1) : T(0));
+}
+
+// WGSL f32 -> i32 conversion: truncate toward zero, saturate, NaN -> 0.
+template <class T> inline int32_t to_i32(T v) {
+  if (!(v == v)) return 0;
+  if (v >= T(2147483520.0)) return 2147483647;
+  if (v <= T(-2147483648.0)) return INT32_MIN;
+  return (int32_t)v;
+}
+
+// Particle record in the precision T, same field order as SphParticle3D
+// (src/simulation/particle.rs:6-17).
+template <class T> struct Part {
+  T pos[3];
+  T pad0;
+  T vel[3];
+  T pad1;
+  T force[3];
+  T density;
+  T near_density;
+  T normal[3];
+};
+static_assert(sizeof(Part<float>) == 64, "particle record");
+
+template <class T> struct Pred {  // pcisph_predict.wgsl:33-42
+  T x[3];
+  T pressure;
+  T v[3];
+  T density;
+};
+
+template <class T> inline V3<T> P3(const T *p) { return mk(p[0], p[1], p[2]); }
+template <class T> inline void S3(T *p, V3<T> v) {
+  p[0] = v.x;
+  p[1] = v.y;
+  p[2] = v.z;
+}
+
+template <class T> struct Cont {  // container_common.wgsl:5-24
+  T hw, hh, hd, cy;
+  V3<T> f0, f1, f2, i0, i1, i2;
+  T wall_stiffness, damping;
+};
+
+template <class T> struct Oracle {
+  uint32_t n = 0;
+  uint32_t iters = 4;
+  // uniforms, promoted to T
+  T h, h2, h5, h6, h9, mass, rho0, near_k, visc, dt, st, delta, eps;
+  uint32_t gx, gy, gz, cells;
+  T inv_cell, ox, oy, oz;
+  Cont<T> c;
+  V3<T> gravity;
+  GpuMouseForce mouse;
+  GpuRigidBody rb;
+  std::vector<T> sdf;
+  uint32_t sdf_res = 0;
+  // buffers (sph_3d_grid.rs:196-305)
+  std::vector<Part<T>> particles, sorted;
+  std::vector<uint32_t> counts, starts, offsets, pci, s2o;
+  std::vector<Pred<T>> pa, pb;
+  int32_t accum[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+
+  // ---- container_common.wgsl:27-54 -------------------------------------
+  V3<T> world_to_local(V3<T> pos) const {
+    V3<T> p = mk(pos.x, pos.y - c.cy, pos.z);
+    return mk(dot(c.i0, p), dot(c.i1, p), dot(c.i2, p));
+  }
+  V3<T> local_to_world(V3<T> l) const {
+    V3<T> r = mk(dot(c.f0, l), dot(c.f1, l), dot(c.f2, l));
+    return mk(r.x, r.y + c.cy, r.z);
+  }
+  V3<T> world_dir_to_local(V3<T> d) const {
+    return mk(dot(c.i0, d), dot(c.i1, d), dot(c.i2, d));
+  }
+  V3<T> local_dir_to_world(V3<T> d) const {
+    return mk(dot(c.f0, d), dot(c.f1, d), dot(c.f2, d));
+  }
+
+  // ---- grid helpers (grid_build.wgsl:33-44; sph_density_3d_grid.wgsl:68-81)
+  void position_to_cell(V3<T> pos, int32_t out[3]) const {
+    V3<T> l = sub(pos, mk(ox, oy, oz));
+    out[0] = to_i32(std::floor(l.x * inv_cell));
+    out[1] = to_i32(std::floor(l.y * inv_cell));
+    out[2] = to_i32(std::floor(l.z * inv_cell));
+  }
+  bool is_valid_cell(int32_t x, int32_t y, int32_t z) const {
+    return x >= 0 && x < (int32_t)gx && y >= 0 && y < (int32_t)gy && z >= 0 &&
+           z < (int32_t)gz;
+  }
+  uint32_t cell_index(int32_t x, int32_t y, int32_t z) const {
+    return (uint32_t)x + (uint32_t)y * gx + (uint32_t)z * gx * gy;
+  }
+
+  // kernels (sph_density_3d_grid.wgsl:56-66, sph_force_3d_grid.wgsl:63-88)
+  T PI() const { return T(3.14159265359f); }
+  T poly6(T r_sq) const {
+    T scale = T(315.0) / (T(64.0) * PI() * h9);
+    T diff = h2 - r_sq;
+    return scale * diff * diff * diff;
+  }
+  T near_kernel(T r) const {
+    T scale = T(15.0) / (PI() * h6);
+    T diff = h - r;
+    return scale * diff * diff * diff;
+  }
+  T spiky_grad(T r) const {
+    T scale = T(45.0) / (PI() * h6);
+    T diff = h - r;
+    return scale * diff * diff;
+  }
+  T akinci(T r) const {
+    T scale = T(32.0) / (PI() * h9);
+    T diff = h - r;
+    T r3 = r * r * r;
+    if (r >= h * T(0.5)) return scale * diff * diff * diff * r3;
+    return scale * (T(2.0) * diff * diff * diff * r3 - h6 / T(64.0));
+  }
+
+  // Iterate the 27-cell stencil of `cell_i` in the reference order
+  // (dz, dy, dx, then slot) and call f(j) for every candidate slot j.
+  template <class F> void sweep(const int32_t cell_i[3], F &&f) const {
+    for (int32_t dz = -1; dz <= 1; ++dz)
+      for (int32_t dy = -1; dy <= 1; ++dy)
+        for (int32_t dx = -1; dx <= 1; ++dx) {
+          int32_t x = cell_i[0] + dx, y = cell_i[1] + dy, z = cell_i[2] + dz;
+          if (!is_valid_cell(x, y, z)) continue;
+          uint32_t ci = cell_index(x, y, z);
+          uint32_t count = counts[ci];
+          uint32_t start = starts[ci] - count;
+          for (uint32_t k = 0; k < count; ++k) f(start + k);
+        }
+  }
+
+  // ---- stage 0: grid_clear.wgsl:23-29 + clear_buffer(sph_3d_grid.rs:1281)
+  void clear() {
+    std::fill(counts.begin(), counts.end(), 0u);
+    std::fill(offsets.begin(), offsets.end(), 0u);
+  }
+
+  // ---- stage 1: grid_build.wgsl:46-62 ----------------------------------
+  void build() {
+    for (uint32_t i = 0; i < n; ++i) {
+      int32_t cl[3];
+      position_to_cell(P3(particles[i].pos), cl);
+      int32_t cx = clampv<int32_t>(cl[0], 0, (int32_t)gx - 1);
+      int32_t cy = clampv<int32_t>(cl[1], 0, (int32_t)gy - 1);
+      int32_t cz = clampv<int32_t>(cl[2], 0, (int32_t)gz - 1);
+      uint32_t ci = cell_index(cx, cy, cz);
+      pci[i] = ci;
+      counts[ci] += 1u;
+    }
+  }
+
+  // ---- stage 2: copy (sph_3d_grid.rs:1295) + prefix_scan.wgsl:50-122 ----
+  // The three dispatches compose to an inclusive scan (u32, wrapping).
+  void scan() {
+    uint32_t acc = 0;
+    for (uint32_t ci = 0; ci < cells; ++ci) {
+      acc += counts[ci];
+      starts[ci] = acc;
+    }
+  }
+
+  // ---- stage 3: grid_reorder.wgsl:37-63, serial in index order ---------
+  void reorder() {
+    for (uint32_t i = 0; i < n; ++i) {
+      uint32_t ci = pci[i];
+      uint32_t start = starts[ci] - counts[ci];
+      uint32_t slot = offsets[ci]++;
+      uint32_t dest = start + slot;
+      sorted[dest] = particles[i];
+      pci[i] = dest;
+      s2o[dest] = i;
+    }
+  }
+
+  // ---- stage 4: sph_density_3d_grid.wgsl:86-194 ------------------------
+  T boundary_gamma(V3<T> pos) const {
+    V3<T> l = world_to_local(pos);
+    T g = T(1.0);
+    T d[6] = {l.x - (-c.hw), c.hw - l.x, l.y - (-c.hh),
+              c.hh - l.y,    l.z - (-c.hd), c.hd - l.z};
+    for (int k = 0; k < 6; ++k)
+      if (d[k] < h) g *= T(0.5) + T(0.5) * clampv(d[k] / h, T(0), T(1));
+    return std::max(g, T(0.1));
+  }
+  void density() {
+#pragma omp parallel for schedule(dynamic, 256)
+    for (int64_t si = 0; si < (int64_t)n; ++si) {
+      uint32_t s = (uint32_t)si;
+      V3<T> pos_i = P3(sorted[s].pos);
+      int32_t cl[3];
+      position_to_cell(pos_i, cl);
+      T dens = T(0), near = T(0);
+      V3<T> nrm = mk(T(0), T(0), T(0));
+      sweep(cl, [&](uint32_t j) {
+        V3<T> r_vec = sub(pos_i, P3(sorted[j].pos));
+        T r_sq = dot(r_vec, r_vec);
+        if (r_sq < h2) {
+          T r = std::sqrt(r_sq);
+          dens += mass * poly6(r_sq);
+          near += mass * near_kernel(r);
+          if (r_sq > T(1e-12f)) {
+            T diff = h2 - r_sq;
+            nrm = add(nrm, muls(muls(smul(mass / rho0, r_vec), diff), diff));
+          }
+        }
+      });
+      T gamma = boundary_gamma(pos_i);
+      dens = dens / gamma;
+      T h8 = h6 * h * h;
+      T nscale = T(945.0) / (T(32.0) * PI() * h8);
+      nrm = muls(nrm, nscale);
+      sorted[s].density = dens;
+      sorted[s].near_density = near;
+      S3(sorted[s].normal, nrm);
+    }
+  }
+
+  // ---- stage 5: xsph.wgsl:81-138 ---------------------------------------
+  void xsph() {
+    if (eps <= T(0)) return;  // xsph.wgsl:94-97 (every invocation returns)
+#pragma omp parallel for schedule(dynamic, 256)
+    for (int64_t si = 0; si < (int64_t)n; ++si) {
+      uint32_t s = (uint32_t)si;
+      V3<T> pos_i = P3(sorted[s].pos);
+      V3<T> vel_i = P3(sorted[s].vel);
+      int32_t cl[3];
+      position_to_cell(pos_i, cl);
+      V3<T> corr = mk(T(0), T(0), T(0));
+      sweep(cl, [&](uint32_t j) {
+        V3<T> r_vec = sub(pos_i, P3(sorted[j].pos));
+        T r_sq = dot(r_vec, r_vec);
+        if (r_sq < h2 && r_sq > T(1e-12f)) {
+          T dj = std::max(sorted[j].density, T(1.0));
+          V3<T> vj = P3(sorted[j].vel);
+          T w = poly6(r_sq);
+          corr = add(corr, muls(smul(mass / dj, sub(vj, vel_i)), w));
+        }
+      });
+      S3(particles[s2o[s]].vel, add(vel_i, smul(eps, corr)));
+    }
+  }
+
+  // ---- stage 6: sph_force_3d_grid.wgsl:105-227 --------------------------
+  void force() {
+#pragma omp parallel for schedule(dynamic, 256)
+    for (int64_t si = 0; si < (int64_t)n; ++si) {
+      uint32_t s = (uint32_t)si;
+      uint32_t orig = s2o[s];
+      V3<T> pos_i = P3(sorted[s].pos);
+      V3<T> vel_i = P3(particles[orig].vel);
+      int32_t cl[3];
+      position_to_cell(pos_i, cl);
+      T di = std::max(sorted[s].density, T(1.0));
+      T ndi = std::max(sorted[s].near_density, T(1.0));
+      V3<T> n_i = P3(sorted[s].normal);
+      T npi = near_k * ndi;
+      V3<T> ap = mk(T(0), T(0), T(0)), av = ap, ac = ap;
+      sweep(cl, [&](uint32_t j) {
+        V3<T> pos_j = P3(sorted[j].pos);
+        V3<T> r_vec = sub(pos_i, pos_j);
+        T r_sq = dot(r_vec, r_vec);
+        if (!(r_sq < h2 && r_sq > T(1e-12f))) return;
+        T r = std::sqrt(r_sq);
+        V3<T> dir;
+        T min_dist = h * T(0.1);
+        if (r < min_dist) {
+          T idx = T((float)s) - T((float)j);
+          dir = normalize(mk(std::sin(idx * T(1.0)), std::cos(idx * T(2.0)),
+                             std::sin(idx * T(3.0))));
+        } else {
+          dir = normalize(sub(pos_j, pos_i));
+        }
+        T dj = std::max(sorted[j].density, T(1.0));
+        T ndj = std::max(sorted[j].near_density, T(1.0));
+        T npj = near_k * ndj;
+        T sp = -mass * (npi + npj) / (T(2.0) * di * ndj);
+        ap = add(ap, muls(smul(sp, dir), spiky_grad(r)));
+        if (r < min_dist) {
+          T sep = T(200.0) * (T(1.0) - r / min_dist);
+          ap = add(ap, smul(-sep, dir));
+        }
+        V3<T> vel_j = P3(sorted[j].vel);
+        V3<T> v_ij = sub(vel_i, vel_j);
+        V3<T> x_ij = sub(pos_i, pos_j);
+        T vdx = dot(v_ij, x_ij);
+        if (vdx < T(0)) {
+          T mu = h * vdx / (r_sq + T(0.01) * h2);
+          T avg = (di + dj) * T(0.5);
+          T pi_visc = -visc * mu / avg;
+          av = add(av, smul(-mass * pi_visc * spiky_grad(r), dir));
+        }
+        T K = T(2.0) * rho0 / (di + dj);
+        T coh = akinci(r);
+        ac = add(ac, smul(mass * K * coh, dir));
+        V3<T> n_j = P3(sorted[j].normal);
+        ac = sub(ac, smul(mass * K, sub(n_i, n_j)));
+      });
+      ac = muls(ac, st);
+      S3(particles[orig].force, add(add(add(ap, av), ac), gravity));
+    }
+  }
+
+  // ---- stage 7: pcisph_predict.wgsl:50-73 ------------------------------
+  void predict() {
+    for (uint32_t i = 0; i < n; ++i) {
+      V3<T> pos = P3(particles[i].pos), vel = P3(particles[i].vel);
+      V3<T> a = P3(particles[i].force);
+      V3<T> vs = add(vel, smul(dt, a));
+      V3<T> xs = add(pos, smul(dt, vs));
+      Pred<T> &o = pa[pci[i]];
+      S3(o.x, xs);
+      o.pressure = T(0);
+      S3(o.v, vs);
+      o.density = T(0);
+    }
+  }
+
+  // ---- stage 8: pcisph_solve.wgsl:99-208, ping-pong (sph_3d_grid.rs:1388)
+  void solve_iter(const std::vector<Pred<T>> &in, std::vector<Pred<T>> &out) {
+#pragma omp parallel for schedule(dynamic, 256)
+    for (int64_t si = 0; si < (int64_t)n; ++si) {
+      uint32_t s = (uint32_t)si;
+      uint32_t orig = s2o[s];
+      const Pred<T> &self = in[s];
+      V3<T> pp = P3(self.x);
+      T old_p = self.pressure;
+      int32_t cl[3];
+      position_to_cell(P3(particles[orig].pos), cl);
+      T pd = T(0);
+      V3<T> ap = mk(T(0), T(0), T(0));
+      sweep(cl, [&](uint32_t j) {
+        const Pred<T> &nb = in[j];
+        V3<T> r_vec = sub(pp, P3(nb.x));
+        T r_sq = dot(r_vec, r_vec);
+        if (r_sq < h2) {
+          pd += mass * poly6(r_sq);
+          if (r_sq > T(1e-12f)) {
+            T r = std::sqrt(r_sq);
+            T nd = std::max(nb.density, T(1.0));
+            T np = nb.pressure;
+            T sd = std::max(self.density, T(1.0));
+            V3<T> dir = divs(mk(-r_vec.x, -r_vec.y, -r_vec.z), r);
+            T sc = -mass * (old_p / (sd * sd) + np / (nd * nd)) * spiky_grad(r);
+            ap = add(ap, smul(sc, dir));
+          }
+        }
+      });
+      T err = std::max(T(0), pd - rho0);
+      T new_p = old_p + delta * err;
+      T max_a = T(0.25) * h / (dt * dt);
+      T amag = dot(ap, ap);
+      T max_a2 = max_a * max_a;
+      if (amag > max_a2) ap = muls(ap, max_a / std::sqrt(amag));
+      V3<T> ov = P3(particles[orig].vel), op = P3(particles[orig].pos);
+      V3<T> anp = P3(particles[orig].force);
+      V3<T> at = add(anp, ap);
+      V3<T> vs = add(ov, smul(dt, at));
+      V3<T> xs = add(op, smul(dt, vs));
+      Pred<T> &o = out[s];
+      S3(o.x, xs);
+      o.pressure = new_p;
+      S3(o.v, vs);
+      o.density = pd;
+    }
+  }
+  void solve() {
+    for (uint32_t it = 0; it < iters; ++it) {
+      if (it % 2 == 0)
+        solve_iter(pa, pb);
+      else
+        solve_iter(pb, pa);
+    }
+  }
+
+  // ---- stage 9: pcisph_finalize.wgsl:50-74 (buffer: sph_3d_grid.rs:1409)
+  void finalize() {
+    const std::vector<Pred<T>> &fin = (iters % 2 == 0) ? pa : pb;
+    for (uint32_t i = 0; i < n; ++i) {
+      const Pred<T> &f = fin[pci[i]];
+      V3<T> cv = P3(f.v);
+      V3<T> ov = P3(particles[i].vel);
+      S3(particles[i].force, sub(cv, ov));
+      S3(particles[i].vel, cv);
+    }
+  }
+
+  // ---- stage 10: sph_integrate_3d.wgsl:91-388 --------------------------
+  T sdf_fetch(int x, int y, int z) const {
+    int r = (int)sdf_res;
+    x = std::min(std::max(x, 0), r - 1);
+    y = std::min(std::max(y, 0), r - 1);
+    z = std::min(std::max(z, 0), r - 1);
+    return sdf[(size_t)x + (size_t)y * r + (size_t)z * r * r];
+  }
+  // textureSampleLevel with a linear, clamp-to-edge sampler
+  // (sph_3d_grid.rs:336-344), exact (unquantised) trilinear weights.
+  T sdf_sample(V3<T> uvw) const {
+    if (sdf_res == 0) return T(1.0);  // dummy 1x1x1 texture of 1.0 (:347)
+    T u = uvw.x * T(sdf_res) - T(0.5), v = uvw.y * T(sdf_res) - T(0.5),
+      w = uvw.z * T(sdf_res) - T(0.5);
+    T fu = std::floor(u), fv = std::floor(v), fw = std::floor(w);
+    int x0 = (int)fu, y0 = (int)fv, z0 = (int)fw;
+    T a = u - fu, b = v - fv, cz = w - fw;
+    T c000 = sdf_fetch(x0, y0, z0), c100 = sdf_fetch(x0 + 1, y0, z0);
+    T c010 = sdf_fetch(x0, y0 + 1, z0), c110 = sdf_fetch(x0 + 1, y0 + 1, z0);
+    T c001 = sdf_fetch(x0, y0, z0 + 1), c101 = sdf_fetch(x0 + 1, y0, z0 + 1);
+    T c011 = sdf_fetch(x0, y0 + 1, z0 + 1);
+    T c111 = sdf_fetch(x0 + 1, y0 + 1, z0 + 1);
+    T x00 = c000 + (c100 - c000) * a, x10 = c010 + (c110 - c010) * a;
+    T x01 = c001 + (c101 - c001) * a, x11 = c011 + (c111 - c011) * a;
+    T y0v = x00 + (x10 - x00) * b, y1v = x01 + (x11 - x01) * b;
+    return y0v + (y1v - y0v) * cz;
+  }
+
+  void integrate() {
+    for (uint32_t i = 0; i < n; ++i) {
+      V3<T> pos = P3(particles[i].pos), vel = P3(particles[i].vel);
+      V3<T> accel = mk(T(0), T(0), T(0));
+      V3<T> lp = world_to_local(pos), lv = world_dir_to_local(vel);
+      T bl = h * T(0.4);
+      T wd = T(8.0);
+      V3<T> wa = mk(T(0), T(0), T(0));
+      T k = c.wall_stiffness;
+      {
+        T d = lp.x - (-c.hw);
+        if (d < bl) {
+          T t = T(1.0) - d / bl;
+          wa.x += k * t * t;
+          if (lv.x < T(0)) wa.x -= lv.x * wd * t;
+        }
+        d = c.hw - lp.x;
+        if (d < bl) {
+          T t = T(1.0) - d / bl;
+          wa.x -= k * t * t;
+          if (lv.x > T(0)) wa.x -= lv.x * wd * t;
+        }
+        d = lp.y - (-c.hh);
+        if (d < bl) {
+          T t = T(1.0) - d / bl;
+          wa.y += k * t * t;
+          if (lv.y < T(0)) wa.y -= lv.y * wd * t;
+        }
+        d = c.hh - lp.y;
+        if (d < bl) {
+          T t = T(1.0) - d / bl;
+          wa.y -= k * t * t;
+          if (lv.y > T(0)) wa.y -= lv.y * wd * t;
+        }
+        d = lp.z - (-c.hd);
+        if (d < bl) {
+          T t = T(1.0) - d / bl;
+          wa.z += k * t * t;
+          if (lv.z < T(0)) wa.z -= lv.z * wd * t;
+        }
+        d = c.hd - lp.z;
+        if (d < bl) {
+          T t = T(1.0) - d / bl;
+          wa.z -= k * t * t;
+          if (lv.z > T(0)) wa.z -= lv.z * wd * t;
+        }
+      }
+      accel = add(accel, local_dir_to_world(wa));
+
+      if (mouse.is_active == 1u) {  // :163-200
+        V3<T> mp = mk(T(mouse.position[0]), T(mouse.position[1]),
+                      T(mouse.position[2]));
+        V3<T> tm = sub(mp, pos);
+        T dist = length(tm);
+        T rad = T(mouse.radius);
+        if (dist < rad && dist > T(0.001f)) {
+          T fall = T(1.0) - (dist / rad);
+          V3<T> fd = normalize(tm);
+          T s = T(mouse.strength) * fall;
+          switch (mouse.mode) {
+            case 1:
+              accel = add(accel, muls(fd, s));
+              break;
+            case 2: {
+              V3<T> axis = normalize(mk(T(mouse.direction[0]),
+                                        T(mouse.direction[1]),
+                                        T(mouse.direction[2])));
+              V3<T> tg = cross(fd, axis);
+              T tl = length(tg);
+              if (tl > T(0.001f)) accel = add(accel, muls(divs(tg, tl), s));
+              break;
+            }
+            case 3:
+              accel = sub(accel, muls(muls(fd, s), T(3.0)));
+              break;
+            case 4: {
+              V3<T> dd = normalize(add(fd, mk(T(0), T(-1.0), T(0))));
+              accel = add(accel, muls(dd, s));
+              break;
+            }
+            default:
+              accel = sub(accel, muls(fd, s));
+              break;
+          }
+        }
+      }
+
+      if (rb.is_active != 0u) {  // :203-341
+        V3<T> rpos = mk(T(rb.position[0]), T(rb.position[1]),
+                        T(rb.position[2]));
+        V3<T> r0 = mk(T(rb.rot_row0[0]), T(rb.rot_row0[1]), T(rb.rot_row0[2]));
+        V3<T> r1 = mk(T(rb.rot_row1[0]), T(rb.rot_row1[1]), T(rb.rot_row1[2]));
+        V3<T> r2 = mk(T(rb.rot_row2[0]), T(rb.rot_row2[1]), T(rb.rot_row2[2]));
+        V3<T> wr = sub(pos, rpos);
+        V3<T> rl = mk(dot(r0, wr), dot(r1, wr), dot(r2, wr));
+        T he = T(rb.half_extent);
+        T sdfv;
+        V3<T> ln = mk(T(0), T(0), T(0));
+        switch (rb.shape) {
+          case 1: {
+            T dist = length(rl);
+            sdfv = dist - he;
+            ln = dist > T(0.001f) ? divs(rl, dist) : mk(T(0), T(1), T(0));
+            break;
+          }
+          case 2: {
+            T rd = std::sqrt(std::fma(rl.z, rl.z, rl.x * rl.x));
+            T d_rad = rd - he;
+            T d_cap = std::fabs(rl.y) - he;
+            sdfv = std::max(d_rad, d_cap);
+            if (d_rad > d_cap)
+              ln = rd > T(0.001f) ? mk(rl.x / rd, T(0), rl.z / rd)
+                                  : mk(T(1), T(0), T(0));
+            else
+              ln = mk(T(0), wsign(rl.y), T(0));
+            break;
+          }
+          case 3: {
+            T minor = he * T(0.3);
+            T xz = std::sqrt(std::fma(rl.z, rl.z, rl.x * rl.x));
+            T qx = xz - he, qy = rl.y;
+            T ql = std::sqrt(std::fma(qy, qy, qx * qx));
+            sdfv = ql - minor;
+            if (ql > T(0.001f) && xz > T(0.001f)) {
+              T rdx = qx / ql, rdy = qy / ql;
+              ln = mk(rdx * rl.x / xz, rdy, rdx * rl.z / xz);
+            } else {
+              ln = mk(T(0), T(1), T(0));
+            }
+            break;
+          }
+          case 4: {
+            V3<T> nz = divs(rl, he);
+            V3<T> uvw = add(muls(nz, T(0.5)), mk(T(0.5), T(0.5), T(0.5)));
+            sdfv = sdf_sample(uvw) * he;
+            T e = T(1.0) / T(32.0);
+            T gxv = sdf_sample(add(uvw, mk(e, T(0), T(0)))) -
+                    sdf_sample(sub(uvw, mk(e, T(0), T(0))));
+            T gyv = sdf_sample(add(uvw, mk(T(0), e, T(0)))) -
+                    sdf_sample(sub(uvw, mk(T(0), e, T(0))));
+            T gzv = sdf_sample(add(uvw, mk(T(0), T(0), e))) -
+                    sdf_sample(sub(uvw, mk(T(0), T(0), e)));
+            V3<T> g = mk(gxv, gyv, gzv);
+            T gl = length(g);
+            ln = gl > T(1e-6f) ? divs(g, gl)
+                               : normalize(add(rl, mk(T(0), T(1e-6f), T(0))));
+            break;
+          }
+          default: {
+            V3<T> d = mk(std::fabs(rl.x) - he, std::fabs(rl.y) - he,
+                         std::fabs(rl.z) - he);
+            sdfv = std::max(d.x, std::max(d.y, d.z));
+            if (d.x >= d.y && d.x >= d.z)
+              ln = mk(wsign(rl.x), T(0), T(0));
+            else if (d.y >= d.x && d.y >= d.z)
+              ln = mk(T(0), wsign(rl.y), T(0));
+            else
+              ln = mk(T(0), T(0), wsign(rl.z));
+            break;
+          }
+        }
+        T ir = h * T(0.7);
+        if (sdfv < ir) {
+          V3<T> nw = mk(r0.x * ln.x + r1.x * ln.y + r2.x * ln.z,
+                        r0.y * ln.x + r1.y * ln.y + r2.y * ln.z,
+                        r0.z * ln.x + r1.z * ln.y + r2.z * ln.z);
+          T pen = ir - sdfv;
+          T t = pen / ir;
+          V3<T> pen_a = muls(muls(muls(nw, T(rb.stiffness)), t), t);
+          accel = add(accel, pen_a);
+          V3<T> damp = mk(T(0), T(0), T(0));
+          V3<T> rv = sub(vel, mk(T(rb.velocity[0]), T(rb.velocity[1]),
+                                 T(rb.velocity[2])));
+          T vn = dot(rv, nw);
+          if (vn < T(0)) {
+            damp = muls(muls(mk(-nw.x, -nw.y, -nw.z), vn), T(5.0));
+            accel = add(accel, damp);
+          }
+          V3<T> sum = add(pen_a, damp);
+          V3<T> re = muls(mk(-sum.x, -sum.y, -sum.z), mass);
+          auto acc_add = [&](int k2, int32_t v) {
+            accum[k2] = (int32_t)((uint32_t)accum[k2] + (uint32_t)v);
+          };
+          acc_add(0, to_i32(re.x * T(1000.0)));
+          acc_add(1, to_i32(re.y * T(1000.0)));
+          acc_add(2, to_i32(re.z * T(1000.0)));
+          accum[3] = (int32_t)((uint32_t)accum[3] + 1u);
+          V3<T> tq = cross(wr, re);
+          acc_add(4, to_i32(tq.x * T(1000.0)));
+          acc_add(5, to_i32(tq.y * T(1000.0)));
+          acc_add(6, to_i32(tq.z * T(1000.0)));
+        }
+      }
+
+      vel = add(vel, smul(dt, accel));
+      pos = add(pos, smul(dt, vel));
+
+      V3<T> lpn = world_to_local(pos), lvn = world_dir_to_local(vel);
+      T rest = c.damping;
+      if (lpn.x < -c.hw) {
+        lpn.x = -c.hw;
+        lvn.x = std::fabs(lvn.x) * rest;
+      } else if (lpn.x > c.hw) {
+        lpn.x = c.hw;
+        lvn.x = -std::fabs(lvn.x) * rest;
+      }
+      if (lpn.y < -c.hh) {
+        lpn.y = -c.hh;
+        lvn.y = std::fabs(lvn.y) * rest;
+      } else if (lpn.y > c.hh) {
+        lpn.y = c.hh;
+        lvn.y = -std::fabs(lvn.y) * rest;
+      }
+      if (lpn.z < -c.hd) {
+        lpn.z = -c.hd;
+        lvn.z = std::fabs(lvn.z) * rest;
+      } else if (lpn.z > c.hd) {
+        lpn.z = c.hd;
+        lvn.z = -std::fabs(lvn.z) * rest;
+      }
+      S3(particles[i].pos, local_to_world(lpn));
+      S3(particles[i].vel, local_dir_to_world(lvn));
+    }
+  }
+
+  void stage(uint32_t s) {
+    switch (s) {
+      case FLUID_STAGE_CLEAR: clear(); break;
+      case FLUID_STAGE_BUILD: build(); break;
+      case FLUID_STAGE_SCAN: scan(); break;
+      case FLUID_STAGE_REORDER: reorder(); break;
+      case FLUID_STAGE_DENSITY: density(); break;
+      case FLUID_STAGE_XSPH: xsph(); break;
+      case FLUID_STAGE_FORCE: force(); break;
+      case FLUID_STAGE_PREDICT: predict(); break;
+      case FLUID_STAGE_SOLVE: solve(); break;
+      case FLUID_STAGE_FINALIZE: finalize(); break;
+      case FLUID_STAGE_INTEGRATE: integrate(); break;
+      default: break;
+    }
+  }
+
+  // Candidate / in-range counts of the density sweep of the current state.
+  void count_pairs(uint64_t *cand, uint64_t *inr) const {
+    uint64_t c0 = 0, c1 = 0;
+#pragma omp parallel for reduction(+ : c0, c1) schedule(dynamic, 256)
+    for (int64_t si = 0; si < (int64_t)n; ++si) {
+      V3<T> pos_i = P3(sorted[si].pos);
+      int32_t cl[3];
+      position_to_cell(pos_i, cl);
+      sweep(cl, [&](uint32_t j) {
+        V3<T> r_vec = sub(pos_i, P3(sorted[j].pos));
+        c0++;
+        if (dot(r_vec, r_vec) < h2) c1++;
+      });
+    }
+    *cand = c0;
+    *inr = c1;
+  }
+};
+
+template <class T>
+void set_uniforms(Oracle<T> &o, const GpuSphParams3D *p,
+                  const GpuGridParams *g, const GpuContainerGeometry *c,
+                  const GpuGravity *grav) {
+  o.h = T(p->kernel_radius);
+  o.h2 = T(p->kernel_radius_sq);
+  o.h5 = T(p->kernel_radius_pow5);
+  o.h6 = T(p->kernel_radius_pow6);
+  o.h9 = T(p->kernel_radius_pow9);
+  o.mass = T(p->mass);
+  o.rho0 = T(p->rest_density);
+  o.near_k = T(p->near_stiffness);
+  o.visc = T(p->viscosity);
+  o.dt = T(p->dt);
+  o.st = T(p->surface_tension);
+  o.delta = T(p->pcisph_delta);
+  o.eps = T(p->xsph_epsilon);
+  o.gx = g->grid_size_x;
+  o.gy = g->grid_size_y;
+  o.gz = g->grid_size_z;
+  o.cells = g->total_cells;
+  o.inv_cell = T(g->inv_cell_size);
+  o.ox = T(g->grid_origin_x);
+  o.oy = T(g->grid_origin_y);
+  o.oz = T(g->grid_origin_z);
+  auto v = [](const float *r) { return mk(T(r[0]), T(r[1]), T(r[2])); };
+  o.c.hw = T(c->half_width);
+  o.c.hh = T(c->half_height);
+  o.c.hd = T(c->half_depth);
+  o.c.cy = T(c->center_y);
+  o.c.f0 = v(c->forward_row0);
+  o.c.f1 = v(c->forward_row1);
+  o.c.f2 = v(c->forward_row2);
+  o.c.i0 = v(c->inverse_row0);
+  o.c.i1 = v(c->inverse_row1);
+  o.c.i2 = v(c->inverse_row2);
+  o.c.wall_stiffness = T(c->wall_stiffness);
+  o.c.damping = T(c->damping);
+  o.gravity = v(grav->direction);
+}
+
+struct Handle {
+  int fp64;
+  Oracle<float> f;
+  Oracle<double> d;
+};
+
+// ---- restatement of the reference's host-side parameter derivation ------
+// Rust powi(x, n) with a constant n: square-and-multiply, low bit first.
+float powi_f32(float x, int n) {
+  float res = 1.0f, cur = x;
+  bool first = true;
+  while (n) {
+    if (n & 1) {
+      res = first ? cur : res * cur;
+      first = false;
+    }
+    n >>= 1;
+    if (n) cur = cur * cur;
+  }
+  return res;
+}
+
+}  // namespace
+
+extern "C" {
+
+// compute_rest_density (src/state/simulation.rs:402-425), f32.
+float oracle_rest_density_f32(float h, float mass) {
+  float spacing = h * 0.6f;
+  int cells = (int)std::ceil(h / spacing) + 1;
+  float pi = 3.14159265359f;
+  float h2 = h * h;
+  float h9 = powi_f32(h, 9);
+  float scale = 315.0f / (64.0f * pi * h9);
+  float density = 0.0f;
+  for (int dz = -cells; dz <= cells; ++dz)
+    for (int dy = -cells; dy <= cells; ++dy)
+      for (int dx = -cells; dx <= cells; ++dx) {
+        float a = (float)dx * spacing, b = (float)dy * spacing,
+              c = (float)dz * spacing;
+        float r_sq = a * a + b * b + c * c;
+        if (r_sq < h2) {
+          float diff = h2 - r_sq;
+          density += mass * scale * diff * diff * diff;
+        }
+      }
+  return density;
+}
+
+// compute_pcisph_delta (src/state/simulation.rs:429-480), f32.
+float oracle_pcisph_delta_f32(float h, float mass, float rest_density,
+                              float dt) {
+  float spacing = h * 0.6f;
+  int cells = (int)std::ceil(h / spacing) + 1;
+  float sg[3] = {0.0f, 0.0f, 0.0f};
+  float sgsq = 0.0f;
+  float pi = 3.14159265359f;
+  float h6 = powi_f32(h, 6);
+  for (int dz = -cells; dz <= cells; ++dz)
+    for (int dy = -cells; dy <= cells; ++dy)
+      for (int dx = -cells; dx <= cells; ++dx) {
+        if (dx == 0 && dy == 0 && dz == 0) continue;
+        float rv[3] = {(float)dx * spacing, (float)dy * spacing,
+                       (float)dz * spacing};
+        float r_sq = rv[0] * rv[0] + rv[1] * rv[1] + rv[2] * rv[2];
+        if (r_sq < h * h && r_sq > 1e-12f) {
+          float r = std::sqrt(r_sq);
+          float gm = 45.0f / (pi * h6) * (h - r) * (h - r);
+          float g[3] = {gm * rv[0] / r, gm * rv[1] / r, gm * rv[2] / r};
+          sg[0] += g[0];
+          sg[1] += g[1];
+          sg[2] += g[2];
+          sgsq += g[0] * g[0] + g[1] * g[1] + g[2] * g[2];
+        }
+      }
+  float sgd = sg[0] * sg[0] + sg[1] * sg[1] + sg[2] * sg[2];
+  float beta = dt * dt * mass * mass / (rest_density * rest_density);
+  float denom = beta * (sgd + sgsq);
+  float omega = 0.2f;
+  return std::fabs(denom) > 1e-10f ? omega / denom : 0.0f;
+}
+
+float oracle_powi_f32(float x, int n) { return powi_f32(x, n); }
+
+// particles: n records of 16 values in the handle's precision.
+void *oracle_create(int fp64, const void *particles, uint32_t n,
+                    const GpuSphParams3D *p, const GpuGridParams *g,
+                    const GpuContainerGeometry *c, const GpuGravity *grav,
+                    const GpuMouseForce *mouse, const GpuRigidBody *rb,
+                    uint32_t iters, const float *sdf, uint32_t sdf_res) {
+  Handle *hd = new Handle();
+  hd->fp64 = fp64;
+  auto init = [&](auto &o, auto tag) {
+    using T = decltype(tag);
+    set_uniforms(o, p, g, c, grav);
+    o.n = n;
+    o.iters = iters < 1 ? 1 : iters;
+    o.mouse = *mouse;
+    o.rb = *rb;
+    o.sdf_res = sdf ? sdf_res : 0;
+    if (sdf) o.sdf.assign(sdf, sdf + (size_t)sdf_res * sdf_res * sdf_res);
+    o.particles.resize(n);
+    std::memcpy(o.particles.data(), particles, sizeof(Part<T>) * n);
+    o.sorted.assign(n, Part<T>{});
+    o.counts.assign(o.cells, 0u);
+    o.starts.assign(o.cells, 0u);
+    o.offsets.assign(o.cells, 0u);
+    o.pci.assign(n, 0u);
+    o.s2o.assign(n, 0u);
+    o.pa.assign(n, Pred<T>{});
+    o.pb.assign(n, Pred<T>{});
+  };
+  if (fp64)
+    init(hd->d, double());
+  else
+    init(hd->f, float());
+  return hd;
+}
+
+void oracle_destroy(void *h) { delete (Handle *)h; }
+
+void oracle_stage(void *h, uint32_t stage) {
+  Handle *hd = (Handle *)h;
+  if (hd->fp64)
+    hd->d.stage(stage);
+  else
+    hd->f.stage(stage);
+}
+
+void oracle_step(void *h, uint32_t steps) {
+  for (uint32_t k = 0; k < steps; ++k)
+    for (uint32_t s = 0; s < FLUID_STAGE_COUNT; ++s) oracle_stage(h, s);
+}
+
+void oracle_set_params(void *h, const GpuSphParams3D *p,
+                       const GpuContainerGeometry *c, const GpuGravity *grav,
+                       uint32_t iters) {
+  Handle *hd = (Handle *)h;
+  auto upd = [&](auto &o) {
+    GpuGridParams g{};
+    g.grid_size_x = o.gx;
+    g.grid_size_y = o.gy;
+    g.grid_size_z = o.gz;
+    g.total_cells = o.cells;
+    g.inv_cell_size = (float)o.inv_cell;
+    g.grid_origin_x = (float)o.ox;
+    g.grid_origin_y = (float)o.oy;
+    g.grid_origin_z = (float)o.oz;
+    set_uniforms(o, p, &g, c, grav);
+    o.iters = iters < 1 ? 1 : iters;
+  };
+  if (hd->fp64)
+    upd(hd->d);
+  else
+    upd(hd->f);
+}
+
+// which: 0 canonical particles, 1 sorted particles (n x 16 values)
+void oracle_get_particles(void *h, int which, void *out) {
+  Handle *hd = (Handle *)h;
+  if (hd->fp64) {
+    auto &v = which ? hd->d.sorted : hd->d.particles;
+    std::memcpy(out, v.data(), v.size() * sizeof(v[0]));
+  } else {
+    auto &v = which ? hd->f.sorted : hd->f.particles;
+    std::memcpy(out, v.data(), v.size() * sizeof(v[0]));
+  }
+}
+
+void oracle_set_particles(void *h, const void *in) {
+  Handle *hd = (Handle *)h;
+  if (hd->fp64)
+    std::memcpy(hd->d.particles.data(), in,
+                hd->d.particles.size() * sizeof(hd->d.particles[0]));
+  else
+    std::memcpy(hd->f.particles.data(), in,
+                hd->f.particles.size() * sizeof(hd->f.particles[0]));
+}
+
+// which: 0 counts, 1 starts, 2 particle_cell_indices, 3 sorted_to_orig,
+//        4 offsets
+void oracle_get_u32(void *h, int which, uint32_t *out) {
+  Handle *hd = (Handle *)h;
+  auto get = [&](auto &o) {
+    const std::vector<uint32_t> *v[5] = {&o.counts, &o.starts, &o.pci, &o.s2o,
+                                         &o.offsets};
+    std::memcpy(out, v[which]->data(), v[which]->size() * 4);
+  };
+  if (hd->fp64)
+    get(hd->d);
+  else
+    get(hd->f);
+}
+
+// which: 0 = sorted_predicted_a, 1 = b (n x 8 values)
+void oracle_get_pred(void *h, int which, void *out) {
+  Handle *hd = (Handle *)h;
+  if (hd->fp64) {
+    auto &v = which ? hd->d.pb : hd->d.pa;
+    std::memcpy(out, v.data(), v.size() * sizeof(v[0]));
+  } else {
+    auto &v = which ? hd->f.pb : hd->f.pa;
+    std::memcpy(out, v.data(), v.size() * sizeof(v[0]));
+  }
+}
+
+void oracle_get_accum(void *h, int32_t *out8) {
+  Handle *hd = (Handle *)h;
+  std::memcpy(out8, hd->fp64 ? hd->d.accum : hd->f.accum, 32);
+}
+
+void oracle_count_pairs(void *h, uint64_t *cand, uint64_t *inr) {
+  Handle *hd = (Handle *)h;
+  if (hd->fp64)
+    hd->d.count_pairs(cand, inr);
+  else
+    hd->f.count_pairs(cand, inr);
+}
+
+}  // extern "C"

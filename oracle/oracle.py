@@ -1,0 +1,186 @@
+"""ctypes wrapper of the CPU oracle (oracle/liboracle.so).
+
+Test infrastructure only: the parity tests, __graft_entry__.smoke() and the
+bench's cpu_baseline leg use it as the checker.  The product library never
+loads it.  See fluid_oracle.cpp for what it restates and how it is pinned.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+import numpy as np
+
+ORACLE_DIR = Path(__file__).resolve().parent
+LIB_PATH = ORACLE_DIR / "liboracle.so"
+
+_LIB = None
+
+
+def load():
+    global _LIB
+    if _LIB is None:
+        if not LIB_PATH.exists():
+            raise FileNotFoundError(f"{LIB_PATH} missing: run make -C oracle")
+        lib = C.CDLL(str(LIB_PATH))
+        P = C.c_void_p
+        lib.oracle_create.restype = P
+        lib.oracle_create.argtypes = [C.c_int, P, C.c_uint32, P, P, P, P, P, P, C.c_uint32,
+                                      P, C.c_uint32]
+        lib.oracle_destroy.argtypes = [P]
+        lib.oracle_stage.argtypes = [P, C.c_uint32]
+        lib.oracle_step.argtypes = [P, C.c_uint32]
+        lib.oracle_set_params.argtypes = [P, P, P, P, C.c_uint32]
+        lib.oracle_get_particles.argtypes = [P, C.c_int, P]
+        lib.oracle_set_particles.argtypes = [P, P]
+        lib.oracle_get_u32.argtypes = [P, C.c_int, P]
+        lib.oracle_get_pred.argtypes = [P, C.c_int, P]
+        lib.oracle_get_accum.argtypes = [P, P]
+        lib.oracle_count_pairs.argtypes = [P, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]
+        lib.oracle_rest_density_f32.restype = C.c_float
+        lib.oracle_rest_density_f32.argtypes = [C.c_float, C.c_float]
+        lib.oracle_pcisph_delta_f32.restype = C.c_float
+        lib.oracle_pcisph_delta_f32.argtypes = [C.c_float] * 4
+        lib.oracle_powi_f32.restype = C.c_float
+        lib.oracle_powi_f32.argtypes = [C.c_float, C.c_int]
+        _LIB = lib
+    return _LIB
+
+
+def grid_for(h: float, bounds_extent: float, n: int, abi_mod):
+    """GpuGridParams of SphSimulation3DGrid::new (sph_3d_grid.rs:121-143),
+    restated in float32 arithmetic."""
+    f = np.float32
+    cell = f(h)
+    G = int(np.ceil(f(f(2.0) * f(bounds_extent)) / cell)) + 2
+    g = abi_mod.GpuGridParams()
+    g.grid_size_x = g.grid_size_y = g.grid_size_z = G
+    g.total_cells = G * G * G
+    g.cell_size = float(cell)
+    g.inv_cell_size = float(f(1.0) / cell)
+    o = float(f(-f(bounds_extent)) - cell)
+    g.grid_origin_x = g.grid_origin_y = g.grid_origin_z = o
+    g.num_particles = n
+    return g
+
+
+class Oracle:
+    """One oracle state: canonical + sorted particles, grid, predicted
+    buffers, in float32 (WGSL order) or float64."""
+
+    def __init__(self, particles: np.ndarray, params, grid, container, gravity, mouse,
+                 rigid_body, iterations: int, fp64: bool = False, sdf: np.ndarray | None = None):
+        self._lib = load()
+        self.fp64 = bool(fp64)
+        self.dtype = np.float64 if fp64 else np.float32
+        parts = np.ascontiguousarray(particles, dtype=self.dtype).reshape(-1, 16)
+        self.n = len(parts)
+        self.cells = grid.total_cells
+        sdf_arr = None if sdf is None else np.ascontiguousarray(sdf, dtype=np.float32)
+        self._sdf = sdf_arr
+        self._h = self._lib.oracle_create(
+            int(self.fp64), parts.ctypes.data, self.n, C.addressof(params), C.addressof(grid),
+            C.addressof(container), C.addressof(gravity), C.addressof(mouse),
+            C.addressof(rigid_body), iterations,
+            None if sdf_arr is None else sdf_arr.ctypes.data,
+            0 if sdf_arr is None else sdf_arr.shape[0])
+
+    def close(self):
+        if self._h:
+            self._lib.oracle_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def stage(self, s: int) -> None:
+        self._lib.oracle_stage(self._h, s)
+
+    def stages(self, first: int, last: int) -> None:
+        for s in range(first, last + 1):
+            self.stage(s)
+
+    def step(self, steps: int = 1) -> None:
+        self._lib.oracle_step(self._h, steps)
+
+    def set_params(self, params, container, gravity, iterations: int) -> None:
+        self._lib.oracle_set_params(self._h, C.addressof(params), C.addressof(container),
+                                    C.addressof(gravity), iterations)
+
+    def particles(self) -> np.ndarray:
+        out = np.zeros((self.n, 16), dtype=self.dtype)
+        self._lib.oracle_get_particles(self._h, 0, out.ctypes.data)
+        return out
+
+    def set_particles(self, parts: np.ndarray) -> None:
+        p = np.ascontiguousarray(parts, dtype=self.dtype).reshape(self.n, 16)
+        self._lib.oracle_set_particles(self._h, p.ctypes.data)
+
+    def sorted_particles(self) -> np.ndarray:
+        out = np.zeros((self.n, 16), dtype=self.dtype)
+        self._lib.oracle_get_particles(self._h, 1, out.ctypes.data)
+        return out
+
+    def _u32(self, which: int, count: int) -> np.ndarray:
+        out = np.zeros(count, dtype=np.uint32)
+        self._lib.oracle_get_u32(self._h, which, out.ctypes.data)
+        return out
+
+    def cell_counts(self) -> np.ndarray:
+        return self._u32(0, self.cells)
+
+    def cell_starts(self) -> np.ndarray:
+        return self._u32(1, self.cells)
+
+    def particle_cell_indices(self) -> np.ndarray:
+        return self._u32(2, self.n)
+
+    def sorted_to_orig(self) -> np.ndarray:
+        return self._u32(3, self.n)
+
+    def predicted(self, which: int) -> np.ndarray:
+        out = np.zeros((self.n, 8), dtype=self.dtype)
+        self._lib.oracle_get_pred(self._h, which, out.ctypes.data)
+        return out
+
+    def accum(self) -> np.ndarray:
+        out = np.zeros(8, dtype=np.int32)
+        self._lib.oracle_get_accum(self._h, out.ctypes.data)
+        return out
+
+    def count_pairs(self) -> tuple[int, int]:
+        a, b = C.c_uint64(0), C.c_uint64(0)
+        self._lib.oracle_count_pairs(self._h, C.byref(a), C.byref(b))
+        return a.value, b.value
+
+
+def rest_density_f32(h: float, mass: float) -> float:
+    return load().oracle_rest_density_f32(h, mass)
+
+
+def pcisph_delta_f32(h: float, mass: float, rho0: float, dt: float) -> float:
+    return load().oracle_pcisph_delta_f32(h, mass, rho0, dt)
+
+
+def powi_f32(x: float, n: int) -> float:
+    return load().oracle_powi_f32(x, n)
+
+
+def from_scene(scene, abi_mod, params, container, gravity, fp64: bool = False,
+               particles: np.ndarray | None = None, mouse=None, rigid_body=None, sdf=None):
+    """An Oracle for a scenes.Scene with the given uniforms."""
+    from importlib import import_module  # noqa: F401
+    g = grid_for(scene.h, scene.bounds_extent, scene.n, abi_mod)
+    if mouse is None:
+        mouse = abi_mod.GpuMouseForce()
+        mouse.radius, mouse.strength, mouse.direction[2] = 0.5, 30.0, -1.0
+    if rigid_body is None:
+        rigid_body = abi_mod.GpuRigidBody()
+        rigid_body.half_extent, rigid_body.stiffness = 0.15, 200.0
+        rigid_body.rot_row0[0] = rigid_body.rot_row1[1] = rigid_body.rot_row2[2] = 1.0
+    parts = scene.particles if particles is None else particles
+    return Oracle(parts, params, g, container, gravity, mouse, rigid_body,
+                  scene.pcisph_iterations, fp64=fp64, sdf=sdf)
