@@ -154,11 +154,14 @@ template <class T> struct Oracle {
   }
 
   // ---- grid helpers (grid_build.wgsl:33-44; sph_density_3d_grid.wgsl:68-81)
+  // Cell coordinates are a discrete decision taken on the f32 value of the
+  // position in f32 arithmetic at both precisions, so the float64 oracle
+  // sorts exactly like the float32 one and differs only in the sums.
   void position_to_cell(V3<T> pos, int32_t out[3]) const {
-    V3<T> l = sub(pos, mk(ox, oy, oz));
-    out[0] = to_i32(std::floor(l.x * inv_cell));
-    out[1] = to_i32(std::floor(l.y * inv_cell));
-    out[2] = to_i32(std::floor(l.z * inv_cell));
+    const float fi = (float)inv_cell;
+    out[0] = to_i32(std::floor(((float)pos.x - (float)ox) * fi));
+    out[1] = to_i32(std::floor(((float)pos.y - (float)oy) * fi));
+    out[2] = to_i32(std::floor(((float)pos.z - (float)oz) * fi));
   }
   bool is_valid_cell(int32_t x, int32_t y, int32_t z) const {
     return x >= 0 && x < (int32_t)gx && y >= 0 && y < (int32_t)gy && z >= 0 &&
@@ -195,7 +198,12 @@ template <class T> struct Oracle {
 
   // Iterate the 27-cell stencil of `cell_i` in the reference order
   // (dz, dy, dx, then slot) and call f(j) for every candidate slot j.
-  template <class F> void sweep(const int32_t cell_i[3], F &&f) const {
+  // Coordinates beyond one cell outside the grid see no valid cell either
+  // way; clamping them first keeps cell_i + d from overflowing.
+  template <class F> void sweep(const int32_t cell_in[3], F &&f) const {
+    const int32_t cell_i[3] = {clampv<int32_t>(cell_in[0], -2, (int32_t)gx + 1),
+                               clampv<int32_t>(cell_in[1], -2, (int32_t)gy + 1),
+                               clampv<int32_t>(cell_in[2], -2, (int32_t)gz + 1)};
     for (int32_t dz = -1; dz <= 1; ++dz)
       for (int32_t dy = -1; dy <= 1; ++dy)
         for (int32_t dx = -1; dx <= 1; ++dx) {

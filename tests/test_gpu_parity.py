@@ -66,6 +66,9 @@ def report(name, err):
 
 
 def check_fp(name, got, want, want64=None, tol=TOL):
+    """99.9% of particles within `tol` of the float32 oracle; the maximum
+    within max(tol, 2 x the float32 oracle's own maximum distance from
+    float64), and the same bound against float64."""
     err = rowwise_rel_err(got, want)
     report(name, err)
     bound = tol
@@ -76,6 +79,7 @@ def check_fp(name, got, want, want64=None, tol=TOL):
         print(f"    f32-oracle noise {noise:.3e}  gpu-vs-f64 {e64:.3e}  bound {bound:.3e}",
               flush=True)
         assert e64 <= bound, f"{name}: gpu vs f64 {e64:.3e} > {bound:.3e}"
+    assert np.quantile(err, 0.999) <= tol, f"{name}: p99.9 rel err > {tol}"
     assert err.max() <= bound, f"{name}: max rel err {err.max():.3e} > {bound:.3e}"
 
 
