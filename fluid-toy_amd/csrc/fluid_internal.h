@@ -9,7 +9,13 @@
 //   edge, by scattering through `sid`.
 //     pos, vel, frc   canonical position / velocity / force (.w unused)
 //     spos, svel, sfrc  the reference's sorted copy (sorted_particle_buffer)
-//     sdens           sorted density;  snrm = (normal.xyz, near_density)
+//     srec            per sorted slot, three float4 written by the density
+//                     pass and gathered by XSPH and force:
+//                       (pos, density), (vel, near_density), (normal, 0)
+//     nbr, ncnt       in-range neighbour lists of the density pass: entry k
+//                     of lane g (= slot * lanes + lane) at nbr[k * n * lanes
+//                     + g], so a wave's k-th entries are one contiguous run;
+//                     ncnt[g] > capacity marks an overflowed list
 //     pred[2]         sorted_predicted_{a,b} split into
 //                       xq = (x*, q) with q = p / max(rho*,1)^2,
 //                       pr = (p, rho*), v = (v*, 0)
@@ -66,6 +72,8 @@ struct DevUniforms {
   float max_accel_sq;  // max_accel^2                                    :261
   float boundary_layer;  // h * 0.4                 sph_integrate_3d.wgsl:113
   float interact_range;  // h * 0.7                                      :303
+  float md2_lo, md2_hi;  // (0.1 h)^2 (1 -/+ 1e-6): r < 0.1h is decided on
+                         // r^2 outside this band, on a rounded sqrt inside
   // GpuGridParams
   uint32_t gx, gy, gz, cells, gxy;
   float inv_cell, ox, oy, oz;
@@ -89,8 +97,9 @@ struct PredBuf {
 
 struct Bufs {
   float4 *pos, *vel, *frc;
-  float4 *spos, *svel, *sfrc, *snrm;
-  float *sdens;
+  float4 *spos, *svel, *sfrc;
+  float4 *srec;
+  uint32_t *nbr, *ncnt;
   uint32_t *sid;
   uint32_t *counts, *starts;
   uint32_t *cell_of, *slot_of;
@@ -107,6 +116,9 @@ struct Bufs {
 constexpr uint32_t kScanThreads = 256;
 constexpr uint32_t kScanItems = 4;
 constexpr uint32_t kScanTile = kScanThreads * kScanItems;
+
+// In-range neighbour entries stored per particle (split over its lanes).
+constexpr uint32_t kListCapacity = 384;
 
 // Error bits written by kernels into Bufs::err.
 constexpr uint32_t kErrScanSpin = 1u;

@@ -296,8 +296,9 @@ int fluid_create(const fluid_create_info *info, fluid_sim **out) {
   ALLOC(spos, float4 *, N);
   ALLOC(svel, float4 *, N);
   ALLOC(sfrc, float4 *, N);
-  ALLOC(snrm, float4 *, N);
-  ALLOC(sdens, float *, N);
+  ALLOC(srec, float4 *, 3 * N);
+  ALLOC(nbr, uint32_t *, (size_t)kListCapacity * N);
+  ALLOC(ncnt, uint32_t *, 8 * N);
   ALLOC(sid, uint32_t *, N);
   ALLOC(counts, uint32_t *, C);
   ALLOC(starts, uint32_t *, C);
@@ -538,12 +539,10 @@ int fluid_read_sorted_particles(fluid_sim *s, SphParticle3D *out, uint32_t cap) 
   if (!s || (!out && s->n)) return fail(FLUID_ERR_INVALID_ARG, "null argument");
   if (cap < s->n) return fail(FLUID_ERR_CAPACITY, "output smaller than num_particles");
   HIP_TRY(hipSetDevice(s->device));
-  std::vector<float4> hp, hv, hf, hn;
-  std::vector<float> hd;
+  std::vector<float4> hp, hv, hf, hr;
   int rc;
   if ((rc = d2h(s, hp, s->b.spos, s->n)) || (rc = d2h(s, hv, s->b.svel, s->n)) ||
-      (rc = d2h(s, hf, s->b.sfrc, s->n)) || (rc = d2h(s, hn, s->b.snrm, s->n)) ||
-      (rc = d2h(s, hd, s->b.sdens, s->n)))
+      (rc = d2h(s, hf, s->b.sfrc, s->n)) || (rc = d2h(s, hr, s->b.srec, 3 * (size_t)s->n)))
     return rc;
   HIP_TRY(hipStreamSynchronize(s->stream));
   for (uint32_t k = 0; k < s->n; ++k) {
@@ -552,9 +551,11 @@ int fluid_read_sorted_particles(fluid_sim *s, SphParticle3D *out, uint32_t cap) 
     o.position[0] = hp[k].x; o.position[1] = hp[k].y; o.position[2] = hp[k].z;
     o.velocity[0] = hv[k].x; o.velocity[1] = hv[k].y; o.velocity[2] = hv[k].z;
     o.force[0] = hf[k].x; o.force[1] = hf[k].y; o.force[2] = hf[k].z;
-    o.density = hd[k];
-    o.near_density = hn[k].w;
-    o.normal[0] = hn[k].x; o.normal[1] = hn[k].y; o.normal[2] = hn[k].z;
+    o.density = hr[3 * (size_t)k].w;
+    o.near_density = hr[3 * (size_t)k + 1].w;
+    o.normal[0] = hr[3 * (size_t)k + 2].x;
+    o.normal[1] = hr[3 * (size_t)k + 2].y;
+    o.normal[2] = hr[3 * (size_t)k + 2].z;
   }
   return check_errors(s);
 }

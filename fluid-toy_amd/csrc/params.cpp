@@ -127,6 +127,9 @@ void fill_uniforms(DevUniforms &u, const GpuSphParams3D &p,
   u.max_accel_sq = u.max_accel * u.max_accel;
   u.boundary_layer = p.kernel_radius * 0.4f;
   u.interact_range = p.kernel_radius * 0.7f;
+  const double md2 = (double)u.min_dist * (double)u.min_dist;
+  u.md2_lo = (float)(md2 * (1.0 - 1e-6));
+  u.md2_hi = (float)(md2 * (1.0 + 1e-6));
   u.gx = g.grid_size_x;
   u.gy = g.grid_size_y;
   u.gz = g.grid_size_z;

@@ -63,15 +63,26 @@ __device__ __forceinline__ float group_sum(float v) {
   return v;
 }
 
+// Workgroup index remapped so that each XCD (workgroups are dealt to the 8
+// XCDs round-robin) receives one contiguous run of tiles: neighbouring
+// particles then share that XCD's L2.
+__device__ __forceinline__ uint32_t xcd_block() {
+  const uint32_t nb = gridDim.x, i = blockIdx.x;
+  const uint32_t q = nb >> 3, r = nb & 7u, x = i & 7u, k = i >> 3;
+  return x * q + min(x, r) + k;
+}
+
 // The 27-cell stencil of sph_density_3d_grid.wgsl:129-167, walked as nine
 // x-rows: cells (x-1..x+1, y, z) are consecutive cell indices, so their
 // particles form one contiguous sorted range.  The visit order (dz, dy, dx,
 // then slot) is the reference's.  Lane `sub` of an L-lane group takes every
-// L-th candidate of each row.
+// L-th candidate of each row; f(j, p_j) receives the candidate's float4 from
+// `pts`, loaded four at a time to keep several gathers in flight.
 template <int L, class F>
 __device__ __forceinline__ void sweep27(const DevUniforms *__restrict__ U,
                                         const uint32_t *__restrict__ counts,
                                         const uint32_t *__restrict__ starts,
+                                        const float4 *__restrict__ pts,
                                         int cx, int cy, int cz, uint32_t sub,
                                         F &&f) {
   const int gx = (int)U->gx, gy = (int)U->gy, gz = (int)U->gz;
@@ -88,7 +99,15 @@ __device__ __forceinline__ void sweep27(const DevUniforms *__restrict__ U,
       const uint32_t c0 = row + (uint32_t)x0, c1 = row + (uint32_t)x1;
       const uint32_t begin = starts[c0] - counts[c0];
       const uint32_t end = starts[c1];
-      for (uint32_t j = begin + sub; j < end; j += L) f(j);
+      uint32_t j = begin + sub;
+      for (; j + 3 * L < end; j += 4 * L) {
+        const float4 p0 = pts[j], p1 = pts[j + L], p2 = pts[j + 2 * L], p3 = pts[j + 3 * L];
+        f(j, p0);
+        f(j + L, p1);
+        f(j + 2 * L, p2);
+        f(j + 3 * L, p3);
+      }
+      for (; j < end; j += L) f(j, pts[j]);
     }
   }
 }
@@ -260,10 +279,15 @@ __global__ __launch_bounds__(256) void k_rank_gather(Bufs b, uint32_t n) {
 
 // ------------------------------------------------------------------------
 // Stage 4 — sph_density_3d_grid.wgsl:111-194 (+ boundary_gamma :86-109).
+// Besides density, near density and the Akinci normal, each lane records the
+// in-range neighbours it met (0 < r^2 < h^2, the pairs XSPH and force sum
+// over) in its list, and the particle's neighbour record is written for the
+// later gathers.
 template <int L>
 __global__ __launch_bounds__(256) void k_density(const DevUniforms *__restrict__ U,
                                                  Bufs b, uint32_t n) {
-  const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+  constexpr uint32_t kCap = kListCapacity / L;
+  const uint32_t g = xcd_block() * blockDim.x + threadIdx.x;
   const uint32_t s = g / L, sub = g % L;
   if (s >= n) return;
   const float4 pi = b.spos[s];
@@ -272,10 +296,11 @@ __global__ __launch_bounds__(256) void k_density(const DevUniforms *__restrict__
   const int cz = cell_coord(pi.z, U->oz, U->inv_cell);
   const float h = U->h, h2 = U->h2, mass = U->mass;
   const float ps = U->poly6_scale, ns = U->near_scale, mr = U->m_over_rho0;
+  const uint64_t stride = (uint64_t)n * L;
+  uint32_t *__restrict__ nbr = b.nbr + g;
+  uint32_t cnt = 0;
   float dens = 0.f, near = 0.f, nx = 0.f, ny = 0.f, nz = 0.f;
-  const float4 *__restrict__ spos = b.spos;
-  sweep27<L>(U, b.counts, b.starts, cx, cy, cz, sub, [&](uint32_t j) {
-    const float4 pj = spos[j];
+  sweep27<L>(U, b.counts, b.starts, b.spos, cx, cy, cz, sub, [&](uint32_t j, float4 pj) {
     const float rx = pi.x - pj.x, ry = pi.y - pj.y, rz = pi.z - pj.z;
     const float r2 = dot3(rx, ry, rz, rx, ry, rz);
     if (r2 < h2) {
@@ -288,9 +313,12 @@ __global__ __launch_bounds__(256) void k_density(const DevUniforms *__restrict__
         nx += mr * rx * d * d;
         ny += mr * ry * d * d;
         nz += mr * rz * d * d;
+        if (cnt < kCap) nbr[(uint64_t)cnt * stride] = j;
+        ++cnt;
       }
     }
   });
+  b.ncnt[g] = cnt;
   dens = group_sum<L>(dens);
   near = group_sum<L>(near);
   nx = group_sum<L>(nx);
@@ -309,9 +337,48 @@ __global__ __launch_bounds__(256) void k_density(const DevUniforms *__restrict__
   for (int k = 0; k < 6; ++k)
     if (dist[k] < h) gamma *= 0.5f + 0.5f * fminf(fmaxf(dist[k] / h, 0.f), 1.f);
   gamma = fmaxf(gamma, 0.1f);
-  b.sdens[s] = dens / gamma;
   const float sc = U->normal_scale;
-  b.snrm[s] = make_float4(nx * sc, ny * sc, nz * sc, near);
+  const float4 vi = b.svel[s];
+  float4 *rec = b.srec + 3 * (size_t)s;
+  rec[0] = make_float4(pi.x, pi.y, pi.z, dens / gamma);
+  rec[1] = make_float4(vi.x, vi.y, vi.z, near);
+  rec[2] = make_float4(nx * sc, ny * sc, nz * sc, 0.f);
+}
+
+// Walk lane g's in-range list, or, if it overflowed, the same candidate
+// subset again with the same test; f(j, p_j) sees every pair once.
+template <int L, class F>
+__device__ __forceinline__ void for_each_neighbour(const DevUniforms *__restrict__ U,
+                                                   const Bufs &b, uint32_t n, uint32_t g,
+                                                   uint32_t sub, float4 pi, F &&f) {
+  constexpr uint32_t kCap = kListCapacity / L;
+  const uint32_t cnt = b.ncnt[g];
+  const float4 *__restrict__ srec = b.srec;
+  if (cnt <= kCap) {
+    const uint64_t stride = (uint64_t)n * L;
+    const uint32_t *__restrict__ nbr = b.nbr + g;
+    uint32_t k = 0;
+    for (; k + 1 < cnt; k += 2) {
+      const uint32_t j0 = nbr[(uint64_t)k * stride], j1 = nbr[(uint64_t)(k + 1) * stride];
+      const float4 p0 = srec[3 * (size_t)j0], p1 = srec[3 * (size_t)j1];
+      f(j0, p0);
+      f(j1, p1);
+    }
+    if (k < cnt) {
+      const uint32_t j0 = nbr[(uint64_t)k * stride];
+      f(j0, srec[3 * (size_t)j0]);
+    }
+    return;
+  }
+  const float h2 = U->h2;
+  const int cx = cell_coord(pi.x, U->ox, U->inv_cell);
+  const int cy = cell_coord(pi.y, U->oy, U->inv_cell);
+  const int cz = cell_coord(pi.z, U->oz, U->inv_cell);
+  sweep27<L>(U, b.counts, b.starts, b.spos, cx, cy, cz, sub, [&](uint32_t j, float4 pj) {
+    const float rx = pi.x - pj.x, ry = pi.y - pj.y, rz = pi.z - pj.z;
+    const float r2 = dot3(rx, ry, rz, rx, ry, rz);
+    if (r2 < h2 && r2 > 1e-12f) f(j, srec[3 * (size_t)j]);
+  });
 }
 
 // ------------------------------------------------------------------------
@@ -321,7 +388,7 @@ __global__ __launch_bounds__(256) void k_density(const DevUniforms *__restrict__
 template <int L>
 __global__ __launch_bounds__(256) void k_xsph(const DevUniforms *__restrict__ U,
                                               Bufs b, uint32_t n) {
-  const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t g = xcd_block() * blockDim.x + threadIdx.x;
   const uint32_t s = g / L, sub = g % L;
   if (s >= n) return;
   const float eps = U->eps;
@@ -331,28 +398,19 @@ __global__ __launch_bounds__(256) void k_xsph(const DevUniforms *__restrict__ U,
     return;
   }
   const float4 pi = b.spos[s];
-  const int cx = cell_coord(pi.x, U->ox, U->inv_cell);
-  const int cy = cell_coord(pi.y, U->oy, U->inv_cell);
-  const int cz = cell_coord(pi.z, U->oz, U->inv_cell);
   const float h2 = U->h2, mass = U->mass, ps = U->poly6_scale;
   float cxv = 0.f, cyv = 0.f, czv = 0.f;
-  const float4 *__restrict__ spos = b.spos;
-  const float4 *__restrict__ svel = b.svel;
-  const float *__restrict__ sdens = b.sdens;
-  sweep27<L>(U, b.counts, b.starts, cx, cy, cz, sub, [&](uint32_t j) {
-    const float4 pj = spos[j];
+  const float4 *__restrict__ srec = b.srec;
+  for_each_neighbour<L>(U, b, n, g, sub, pi, [&](uint32_t j, float4 pj) {
     const float rx = pi.x - pj.x, ry = pi.y - pj.y, rz = pi.z - pj.z;
     const float r2 = dot3(rx, ry, rz, rx, ry, rz);
-    if (r2 < h2 && r2 > 1e-12f) {
-      const float dj = fmaxf(sdens[j], 1.0f);
-      const float4 vj = svel[j];
-      const float d = h2 - r2;
-      const float w = ps * d * d * d;
-      const float f = mass / dj;
-      cxv += f * (vj.x - vi.x) * w;
-      cyv += f * (vj.y - vi.y) * w;
-      czv += f * (vj.z - vi.z) * w;
-    }
+    const float4 vj = srec[3 * (size_t)j + 1];
+    const float d = h2 - r2;
+    const float w = ps * d * d * d;
+    const float f = mass * __builtin_amdgcn_rcpf(fmaxf(pj.w, 1.0f)) * w;
+    cxv += f * (vj.x - vi.x);
+    cyv += f * (vj.y - vi.y);
+    czv += f * (vj.z - vi.z);
   });
   cxv = group_sum<L>(cxv);
   cyv = group_sum<L>(cyv);
@@ -364,95 +422,84 @@ __global__ __launch_bounds__(256) void k_xsph(const DevUniforms *__restrict__ U,
 // ------------------------------------------------------------------------
 // Stage 6 — sph_force_3d_grid.wgsl:105-227: near pressure + close-range
 // separation, Monaghan viscosity, Akinci cohesion and curvature, gravity.
+// The per-pair quotients are regrouped around one reciprocal each
+// (sp = -m/(2 rho_i) (P_i/rho^n_j + k_near), the viscosity and K_ij share
+// 1/(rho_i + rho_j)); r < 0.1 h, which switches the direction, is decided on
+// the exact bits the oracle uses.
 template <int L>
 __global__ __launch_bounds__(256) void k_force(const DevUniforms *__restrict__ U,
                                                Bufs b, uint32_t n) {
-  const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t g = xcd_block() * blockDim.x + threadIdx.x;
   const uint32_t s = g / L, sub = g % L;
   if (s >= n) return;
-  const float4 pi = b.spos[s];
+  const float4 *rec_i = b.srec + 3 * (size_t)s;
+  const float4 pi = rec_i[0];
+  const float4 ni = rec_i[2];
   const float4 vi = b.vel[s];  // post-XSPH self velocity
-  const int cx = cell_coord(pi.x, U->ox, U->inv_cell);
-  const int cy = cell_coord(pi.y, U->oy, U->inv_cell);
-  const int cz = cell_coord(pi.z, U->oz, U->inv_cell);
-  const float di = fmaxf(b.sdens[s], 1.0f);
-  const float4 ni = b.snrm[s];
-  const float ndi = fmaxf(ni.w, 1.0f);
-  const float h = U->h, h2 = U->h2, mass = U->mass, near_k = U->near_k;
-  const float gs = U->grad_scale, md = U->min_dist, visc = U->visc;
-  const float veps = U->visc_eps, rho0 = U->rho0, aks = U->akinci_scale;
+  const float di = fmaxf(pi.w, 1.0f);
+  const float ndi = fmaxf(rec_i[1].w, 1.0f);
+  const float h = U->h, mass = U->mass, near_k = U->near_k;
+  const float gs = U->grad_scale, md = U->min_dist;
+  const float veps = U->visc_eps, aks = U->akinci_scale;
   const float hh = U->half_h, h6_64 = U->h6_64;
+  const float md2_lo = U->md2_lo, md2_hi = U->md2_hi;
   const float npi = near_k * ndi;
+  const float a_sp = -mass / (2.0f * di);
+  const float c_visc = 2.0f * mass * U->visc * h;  // -m * (-visc * h / (avg))
+  const float c_k = 2.0f * U->rho0 * mass;
+  const float inv_md = 1.0f / md;
   float apx = 0.f, apy = 0.f, apz = 0.f;
-  float avx = 0.f, avy = 0.f, avz = 0.f;
   float acx = 0.f, acy = 0.f, acz = 0.f;
-  const float4 *__restrict__ spos = b.spos;
-  const float4 *__restrict__ svel = b.svel;
-  const float4 *__restrict__ snrm = b.snrm;
-  const float *__restrict__ sdens = b.sdens;
-  sweep27<L>(U, b.counts, b.starts, cx, cy, cz, sub, [&](uint32_t j) {
-    const float4 pj = spos[j];
+  const float4 *__restrict__ srec = b.srec;
+  for_each_neighbour<L>(U, b, n, g, sub, pi, [&](uint32_t j, float4 pj) {
     const float rx = pi.x - pj.x, ry = pi.y - pj.y, rz = pi.z - pj.z;
     const float r2 = dot3(rx, ry, rz, rx, ry, rz);
-    if (!(r2 < h2 && r2 > 1e-12f)) return;
-    const float r = sqrtf(r2);  // correctly rounded: decides r < min_dist
+    const float4 vj = srec[3 * (size_t)j + 1];
+    const float4 nj = srec[3 * (size_t)j + 2];
+    bool close = r2 < md2_lo;
+    if (!close && r2 < md2_hi) close = sqrtf(r2) < md;
+    const float r = fast_sqrt(r2);
     float dx, dy, dz;
-    const bool close = r < md;
     if (close) {
       const float idx = (float)s - (float)j;
       const float a = sinf(idx * 1.0f), c = cosf(idx * 2.0f), e = sinf(idx * 3.0f);
-      const float l = sqrtf(dot3(a, c, e, a, c, e));
-      dx = a / l; dy = c / l; dz = e / l;
+      const float il = __builtin_amdgcn_rsqf(dot3(a, c, e, a, c, e));
+      dx = a * il; dy = c * il; dz = e * il;
     } else {
-      const float ux = -rx, uy = -ry, uz = -rz;
-      const float il = 1.0f / r;
-      dx = ux * il; dy = uy * il; dz = uz * il;
+      const float il = __builtin_amdgcn_rcpf(r);
+      dx = -rx * il; dy = -ry * il; dz = -rz * il;
     }
-    const float dj = fmaxf(sdens[j], 1.0f);
-    const float4 nj = snrm[j];
-    const float ndj = fmaxf(nj.w, 1.0f);
-    const float npj = near_k * ndj;
+    const float dj = fmaxf(pj.w, 1.0f);
+    const float ndj = fmaxf(vj.w, 1.0f);
     const float dr = h - r;
     const float grad = gs * dr * dr;
-    const float sp = -mass * (npi + npj) / (2.0f * di * ndj);
-    apx += sp * dx * grad;
-    apy += sp * dy * grad;
-    apz += sp * dz * grad;
-    if (close) {
-      const float sep = 200.0f * (1.0f - r / md);
-      apx += -sep * dx;
-      apy += -sep * dy;
-      apz += -sep * dz;
-    }
-    const float4 vj = svel[j];
+    // near pressure (:183) and separation (:186-189)
+    float sp = a_sp * (npi * __builtin_amdgcn_rcpf(ndj) + near_k) * grad;
+    if (close) sp -= 200.0f * (1.0f - r * inv_md);
+    // Monaghan viscosity (:193-202): -m * pi_visc * grad, only if v.x < 0
     const float vdx = dot3(vi.x - vj.x, vi.y - vj.y, vi.z - vj.z, rx, ry, rz);
-    if (vdx < 0.0f) {
-      const float mu = h * vdx / (r2 + veps);
-      const float avg = (di + dj) * 0.5f;
-      const float piv = -visc * mu / avg;
-      const float sv = -mass * piv * grad;
-      avx += sv * dx;
-      avy += sv * dy;
-      avz += sv * dz;
-    }
-    const float K = 2.0f * rho0 / (di + dj);
+    const float inv_dd = __builtin_amdgcn_rcpf(di + dj);
+    if (vdx < 0.0f)
+      sp += c_visc * vdx * inv_dd * __builtin_amdgcn_rcpf(r2 + veps) * grad;
+    // Akinci cohesion + curvature (:206-214)
+    const float mk = c_k * inv_dd;
     const float r3 = r * r * r;
-    const float coh = (r >= hh) ? aks * dr * dr * dr * r3
-                                : aks * (2.0f * dr * dr * dr * r3 - h6_64);
-    const float mk = mass * K;
+    const float dr3 = dr * dr * dr;
+    const float coh = (r >= hh) ? aks * dr3 * r3 : aks * (2.0f * dr3 * r3 - h6_64);
     const float sc = mk * coh;
+    apx += sp * dx;
+    apy += sp * dy;
+    apz += sp * dz;
     acx += sc * dx - mk * (ni.x - nj.x);
     acy += sc * dy - mk * (ni.y - nj.y);
     acz += sc * dz - mk * (ni.z - nj.z);
   });
   apx = group_sum<L>(apx); apy = group_sum<L>(apy); apz = group_sum<L>(apz);
-  avx = group_sum<L>(avx); avy = group_sum<L>(avy); avz = group_sum<L>(avz);
   acx = group_sum<L>(acx); acy = group_sum<L>(acy); acz = group_sum<L>(acz);
   if (sub != 0) return;
   const float st = U->st;
-  b.frc[s] = make_float4(apx + avx + acx * st + U->grav[0],
-                         apy + avy + acy * st + U->grav[1],
-                         apz + avz + acz * st + U->grav[2], 0.f);
+  b.frc[s] = make_float4(apx + acx * st + U->grav[0], apy + acy * st + U->grav[1],
+                         apz + acz * st + U->grav[2], 0.f);
 }
 
 // ------------------------------------------------------------------------
@@ -478,7 +525,7 @@ template <int L>
 __global__ __launch_bounds__(256) void k_solve(const DevUniforms *__restrict__ U,
                                                Bufs b, uint32_t n, PredBuf in,
                                                PredBuf out) {
-  const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t g = xcd_block() * blockDim.x + threadIdx.x;
   const uint32_t s = g / L, sub = g % L;
   if (s >= n) return;
   const float4 xi = in.xq[s];
@@ -489,22 +536,21 @@ __global__ __launch_bounds__(256) void k_solve(const DevUniforms *__restrict__ U
   const float h = U->h, h2 = U->h2, mass = U->mass, ps = U->poly6_scale;
   const float gs = U->grad_scale;
   float pd = 0.f, ax = 0.f, ay = 0.f, az = 0.f;
-  const float4 *__restrict__ xq = in.xq;
-  sweep27<L>(U, b.counts, b.starts, cx, cy, cz, sub, [&](uint32_t j) {
-    const float4 xj = xq[j];
+  sweep27<L>(U, b.counts, b.starts, in.xq, cx, cy, cz, sub, [&](uint32_t, float4 xj) {
     const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
     const float r2 = dot3(rx, ry, rz, rx, ry, rz);
     if (r2 < h2) {
       const float d = h2 - r2;
       pd += mass * (ps * d * d * d);
       if (r2 > 1e-12f) {
-        const float r = fast_sqrt(r2);
+        const float il = __builtin_amdgcn_rsqf(r2);
+        const float r = r2 * il;
         const float dr = h - r;
-        const float sc = -mass * (xi.w + xj.w) * (gs * dr * dr);
-        const float il = 1.0f / r;
-        ax += sc * (-rx * il);
-        ay += sc * (-ry * il);
-        az += sc * (-rz * il);
+        // -m (q_i + q_j) grad * dir, dir = -r_vec / r
+        const float sc = mass * (xi.w + xj.w) * (gs * dr * dr) * il;
+        ax += sc * rx;
+        ay += sc * ry;
+        az += sc * rz;
       }
     }
   });
@@ -767,8 +813,7 @@ __global__ __launch_bounds__(256) void k_count_pairs(const DevUniforms *__restri
     const int cy = cell_coord(pi.y, U->oy, U->inv_cell);
     const int cz = cell_coord(pi.z, U->oz, U->inv_cell);
     const float h2 = U->h2;
-    sweep27<1>(U, b.counts, b.starts, cx, cy, cz, 0, [&](uint32_t j) {
-      const float4 pj = b.spos[j];
+    sweep27<1>(U, b.counts, b.starts, b.spos, cx, cy, cz, 0, [&](uint32_t, float4 pj) {
       const float rx = pi.x - pj.x, ry = pi.y - pj.y, rz = pi.z - pj.z;
       ++cand;
       if (dot3(rx, ry, rz, rx, ry, rz) < h2) ++inr;
@@ -787,9 +832,6 @@ __global__ __launch_bounds__(256) void k_count_pairs(const DevUniforms *__restri
 inline uint32_t blocks_for(uint64_t threads, uint32_t per) {
   return (uint32_t)((threads + per - 1) / per);
 }
-
-template <template <int> class K>
-struct Dummy {};
 
 }  // namespace
 
