@@ -45,12 +45,15 @@ def stage_launches(stage: str, iters: int) -> int:
 
 def run_gpu(pkg, scene, warmup: int, steps: int, stage_steps: int):
     sim = pkg.make_sim(scene)
-    ms = sim.time_steps(warmup, steps)
+    sim.step(warmup)
+    lanes = sim.autotune(2)  # does not advance the state
+    ms = sim.time_steps(0, steps)
     stages = sim.time_stages(stage_steps)
     cand, inr = sim.count_pairs()
     cells = sim.grid_params().total_cells
     iters = sim.pcisph_iterations
     sim.close()
+    run_gpu.lanes = lanes
     return ms, stages, cand, inr, cells, iters
 
 
@@ -136,7 +139,8 @@ def main() -> int:
         "data": "synthetic",
         "config": {"workload": "dam_break_50k", "particles": scene.n, "h": scene.h,
                    "spacing": 0.025, "box": list(scene.box), "pcisph_iterations": iters,
-                   "grid_cells": cells, "launch": "hipGraph per step"},
+                   "grid_cells": cells, "launch": "hipGraph per step",
+                   "lanes_per_particle": list(run_gpu.lanes)},
         "roofline": roof,
         "stages": per_stage,
         "pairs_per_particle": {"candidates": round(cand / scene.n, 1),

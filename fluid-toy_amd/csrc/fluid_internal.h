@@ -125,8 +125,8 @@ constexpr uint32_t kErrScanSpin = 1u;
 
 // Launch configuration of the neighbour sweeps.
 struct SweepCfg {
-  int lanes;    // lanes cooperating on one particle (1, 2, 4, 8)
-  int threads;  // workgroup size
+  int lanes;        // lanes per particle in density / XSPH / force
+  int solve_lanes;  // lanes per particle in the PCISPH solve
 };
 
 // Host-side launchers (sph_kernels.hip).  All enqueue on `st`.
@@ -153,7 +153,7 @@ hipError_t launch_finalize(const DevUniforms *u, const Bufs &b, uint32_t n,
 hipError_t launch_integrate(const DevUniforms *u, const Bufs &b, uint32_t n,
                             hipStream_t st);
 hipError_t launch_count_pairs(const DevUniforms *u, const Bufs &b, uint32_t n,
-                              unsigned long long *out2, hipStream_t st);
+                              unsigned long long *out4, hipStream_t st);
 
 // Host-side parameter helpers (params.cpp).
 void fill_uniforms(DevUniforms &u, const GpuSphParams3D &p,

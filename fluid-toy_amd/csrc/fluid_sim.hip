@@ -61,10 +61,11 @@ struct fluid_sim {
   std::vector<float> extra;
   hipGraphExec_t graph = nullptr;
   uint32_t graph_n = 0, graph_iters = 0;
-  int graph_lanes = 0;
+  int graph_lanes = 0, graph_solve_lanes = 0;
   int cursor = -1;  // last completed stage of a partial step, -1 = none
   GpuRigidBodyAccum last_accum{};
-  SweepCfg cfg{1, 256};
+  SweepCfg cfg{1, 1};
+  bool tuned = false;  // cfg chosen by fluid_set_sweep_lanes / fluid_autotune
   uint32_t ntiles = 0;
 };
 
@@ -80,12 +81,15 @@ void *dev_alloc(fluid_sim *s, size_t bytes, hipError_t *err) {
   return p;
 }
 
-SweepCfg pick_cfg(uint32_t n) {
-  // Enough lanes in flight to cover 256 CUs x 4 SIMDs x ~8 waves.
-  SweepCfg c{1, 256};
-  const uint64_t target = 524288;
-  while (c.lanes < 8 && (uint64_t)n * c.lanes < target) c.lanes *= 2;
-  return c;
+SweepCfg pick_cfg(uint32_t) {
+  // Default before tuning: 8 lanes per particle, the fastest setting the
+  // autotuner found at every size measured (8k to 1M particles, compressed
+  // and relaxed states).
+  return SweepCfg{8, 8};
+}
+
+void update_cfg(fluid_sim *s) {
+  if (!s->tuned) s->cfg = pick_cfg(s->n);
 }
 
 int push_uniforms(fluid_sim *s) {
@@ -161,7 +165,7 @@ void drop_graph(fluid_sim *s) {
 
 int ensure_graph(fluid_sim *s) {
   if (s->graph && s->graph_n == s->n && s->graph_iters == s->iters &&
-      s->graph_lanes == s->cfg.lanes)
+      s->graph_lanes == s->cfg.lanes && s->graph_solve_lanes == s->cfg.solve_lanes)
     return FLUID_OK;
   drop_graph(s);
   hipGraph_t g = nullptr;
@@ -182,6 +186,7 @@ int ensure_graph(fluid_sim *s) {
   s->graph_n = s->n;
   s->graph_iters = s->iters;
   s->graph_lanes = s->cfg.lanes;
+  s->graph_solve_lanes = s->cfg.solve_lanes;
   return FLUID_OK;
 }
 
@@ -315,7 +320,7 @@ int fluid_create(const fluid_create_info *info, fluid_sim **out) {
   ALLOC(err, uint32_t *, 1);
   ALLOC(accum, int32_t *, 8);
 #undef ALLOC
-  s->d_pairs = (unsigned long long *)dev_alloc(s, 16, &e);
+  s->d_pairs = (unsigned long long *)dev_alloc(s, 32, &e);
   if (e != hipSuccess) return bail(fail(FLUID_ERR_HIP, "hipMalloc pairs"));
   s->d_u = (DevUniforms *)dev_alloc(s, sizeof(DevUniforms), &e);
   if (e != hipSuccess) return bail(fail(FLUID_ERR_HIP, "hipMalloc uniforms"));
@@ -479,7 +484,7 @@ int fluid_spawn_particles(fluid_sim *s, const float position[3], uint32_t count,
   s->n += c;
   s->grid.num_particles = s->n;
   s->params.num_particles = s->n;
-  s->cfg = pick_cfg(s->n);
+  update_cfg(s);
   if (spawned) *spawned = c;
   return push_uniforms(s);
 }
@@ -531,7 +536,7 @@ int fluid_write_particles(fluid_sim *s, const SphParticle3D *in, uint32_t n) {
   s->n = n;
   s->grid.num_particles = n;
   s->params.num_particles = n;
-  s->cfg = pick_cfg(n);
+  update_cfg(s);
   return push_uniforms(s);
 }
 
@@ -696,16 +701,117 @@ int fluid_time_stages(fluid_sim *s, uint32_t steps, float *ms_per_stage) {
   return check_errors(s);
 }
 
-int fluid_count_pairs(fluid_sim *s, uint64_t *candidates, uint64_t *in_range) {
-  if (!s || !candidates || !in_range) return fail(FLUID_ERR_INVALID_ARG, "null argument");
+static bool valid_lanes(uint32_t l) { return l == 1 || l == 2 || l == 4 || l == 8; }
+
+int fluid_set_sweep_lanes(fluid_sim *s, uint32_t list_lanes, uint32_t solve_lanes) {
+  if (!s) return fail(FLUID_ERR_INVALID_ARG, "null handle");
+  if ((list_lanes && !valid_lanes(list_lanes)) || (solve_lanes && !valid_lanes(solve_lanes)))
+    return fail(FLUID_ERR_INVALID_ARG, "lanes must be 1, 2, 4 or 8");
+  if (s->cursor != -1) return fail(FLUID_ERR_STATE, "a partial step is in progress");
+  if (list_lanes) s->cfg.lanes = (int)list_lanes;
+  if (solve_lanes) s->cfg.solve_lanes = (int)solve_lanes;
+  s->tuned = true;
+  return FLUID_OK;
+}
+
+int fluid_get_sweep_lanes(const fluid_sim *s, uint32_t out[2]) {
+  if (!s || !out) return fail(FLUID_ERR_INVALID_ARG, "null argument");
+  out[0] = (uint32_t)s->cfg.lanes;
+  out[1] = (uint32_t)s->cfg.solve_lanes;
+  return FLUID_OK;
+}
+
+int fluid_autotune(fluid_sim *s, uint32_t trials, uint32_t out[2]) {
+  if (!s) return fail(FLUID_ERR_INVALID_ARG, "null handle");
+  if (s->cursor != -1) return fail(FLUID_ERR_STATE, "a partial step is in progress");
   HIP_TRY(hipSetDevice(s->device));
-  HIP_TRY(hipMemsetAsync(s->d_pairs, 0, 16, s->stream));
-  HIP_TRY(launch_count_pairs(s->d_u, s->b, s->n, s->d_pairs, s->stream));
-  unsigned long long h[2] = {0, 0};
-  HIP_TRY(hipMemcpyAsync(h, s->d_pairs, 16, hipMemcpyDeviceToHost, s->stream));
+  trials = std::max(trials, 1u);
+  const size_t n = s->n;
+  if (n == 0) return FLUID_OK;
+  // Saved state: canonical position, velocity, force and the slot map.
+  float4 *save = nullptr;
+  uint32_t *save_sid = nullptr;
+  HIP_TRY(hipMalloc(&save, 3 * n * sizeof(float4)));
+  if (hipMalloc(&save_sid, n * 4) != hipSuccess) {
+    (void)hipFree(save);
+    return fail(FLUID_ERR_HIP, "hipMalloc (autotune)");
+  }
+  auto copy = [&](bool to_save) -> hipError_t {
+    const hipMemcpyKind k = hipMemcpyDeviceToDevice;
+    float4 *a[3] = {s->b.pos, s->b.vel, s->b.frc};
+    for (int i = 0; i < 3; ++i) {
+      hipError_t e = to_save ? hipMemcpyAsync(save + i * n, a[i], n * 16, k, s->stream)
+                             : hipMemcpyAsync(a[i], save + i * n, n * 16, k, s->stream);
+      if (e != hipSuccess) return e;
+    }
+    return to_save ? hipMemcpyAsync(save_sid, s->b.sid, n * 4, k, s->stream)
+                   : hipMemcpyAsync(s->b.sid, save_sid, n * 4, k, s->stream);
+  };
+  int32_t accum_keep[8];
+  HIP_TRY(hipMemcpyAsync(accum_keep, s->b.accum, 32, hipMemcpyDeviceToHost, s->stream));
   HIP_TRY(hipStreamSynchronize(s->stream));
-  *candidates = h[0];
-  *in_range = h[1];
+  const SweepCfg keep = s->cfg;
+  const int lanes[4] = {1, 2, 4, 8};
+  float best_list = 1e30f, best_solve = 1e30f;
+  SweepCfg best{keep.lanes, keep.solve_lanes};
+  int rc = FLUID_OK;
+  hipError_t e = copy(true);
+  for (int li = 0; li < 4 && e == hipSuccess && rc == FLUID_OK; ++li) {
+    s->cfg.lanes = lanes[li];
+    s->cfg.solve_lanes = lanes[li];
+    float ms[FLUID_STAGE_COUNT];
+    float tl = 0.f, ts = 0.f;
+    for (uint32_t t = 0; t < trials && rc == FLUID_OK; ++t) {
+      if ((e = copy(false)) != hipSuccess) break;
+      rc = fluid_time_stages(s, 1, ms);
+      tl += ms[FLUID_STAGE_DENSITY] + ms[FLUID_STAGE_XSPH] + ms[FLUID_STAGE_FORCE];
+      ts += ms[FLUID_STAGE_SOLVE];
+    }
+    if (tl < best_list) { best_list = tl; best.lanes = lanes[li]; }
+    if (ts < best_solve) { best_solve = ts; best.solve_lanes = lanes[li]; }
+  }
+  if (e == hipSuccess) e = copy(false);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(s->b.accum, accum_keep, 32, hipMemcpyHostToDevice, s->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+  (void)hipFree(save);
+  (void)hipFree(save_sid);
+  if (rc) {
+    s->cfg = keep;
+    return rc;
+  }
+  if (e != hipSuccess) {
+    s->cfg = keep;
+    return fail(FLUID_ERR_HIP, std::string("autotune: ") + hipGetErrorString(e));
+  }
+  s->cfg = best;
+  s->tuned = true;
+  if (out) {
+    out[0] = (uint32_t)best.lanes;
+    out[1] = (uint32_t)best.solve_lanes;
+  }
+  return FLUID_OK;
+}
+
+int fluid_pair_stats(fluid_sim *s, uint64_t out[4]) {
+  if (!s || !out) return fail(FLUID_ERR_INVALID_ARG, "null argument");
+  HIP_TRY(hipSetDevice(s->device));
+  HIP_TRY(hipMemsetAsync(s->d_pairs, 0, 32, s->stream));
+  HIP_TRY(launch_count_pairs(s->d_u, s->b, s->n, s->d_pairs, s->stream));
+  unsigned long long h[4] = {0, 0, 0, 0};
+  HIP_TRY(hipMemcpyAsync(h, s->d_pairs, 32, hipMemcpyDeviceToHost, s->stream));
+  HIP_TRY(hipStreamSynchronize(s->stream));
+  for (int k = 0; k < 4; ++k) out[k] = h[k];
+  return FLUID_OK;
+}
+
+int fluid_count_pairs(fluid_sim *s, uint64_t *candidates, uint64_t *in_range) {
+  if (!candidates || !in_range) return fail(FLUID_ERR_INVALID_ARG, "null argument");
+  uint64_t st[4];
+  const int rc = fluid_pair_stats(s, st);
+  if (rc) return rc;
+  *candidates = st[0];
+  *in_range = st[1];
   return FLUID_OK;
 }
 

@@ -260,16 +260,41 @@ __global__ __launch_bounds__(256) void k_scatter(Bufs b, uint32_t n) {
 }
 
 // (b) rank every entry by original index within its cell segment and copy
-// the particle (position, velocity, force) to its final sorted slot.
+// the particle (position, velocity, force) to its final sorted slot.  A
+// workgroup's 256 consecutive entries cover consecutive cell segments; their
+// original indices are staged in LDS when the covered range fits, so the
+// O(count) rank loop reads broadcast LDS words.
+constexpr uint32_t kRankLds = 4096;
+
 __global__ __launch_bounds__(256) void k_rank_gather(Bufs b, uint32_t n) {
-  const uint32_t u = blockIdx.x * blockDim.x + threadIdx.x;
+  __shared__ uint32_t s_ids[kRankLds];
+  __shared__ uint32_t s_lo, s_hi;
+  const uint32_t u0 = blockIdx.x * blockDim.x;
+  const uint32_t u = u0 + threadIdx.x;
+  const uint32_t last = min(n, u0 + blockDim.x) - 1;
+  uint2 e = make_uint2(0, 0);
+  uint32_t start = 0, end = 0;
+  if (u < n) {
+    e = b.tmp[u];
+    const uint32_t ci = b.cell_of[e.x];
+    end = b.starts[ci];
+    start = end - b.counts[ci];
+    if (u == u0) s_lo = start;
+    if (u == last) s_hi = end;
+  }
+  __syncthreads();
+  const uint32_t lo = s_lo, len = s_hi - s_lo;
+  const bool staged = len <= kRankLds;
+  if (staged)
+    for (uint32_t v = threadIdx.x; v < len; v += blockDim.x) s_ids[v] = b.tmp[lo + v].y;
+  __syncthreads();
   if (u >= n) return;
-  const uint2 e = b.tmp[u];
-  const uint32_t ci = b.cell_of[e.x];
-  const uint32_t end = b.starts[ci];
-  const uint32_t start = end - b.counts[ci];
   uint32_t rank = 0;
-  for (uint32_t v = start; v < end; ++v) rank += (b.tmp[v].y < e.y) ? 1u : 0u;
+  if (staged) {
+    for (uint32_t v = start - lo; v < end - lo; ++v) rank += (s_ids[v] < e.y) ? 1u : 0u;
+  } else {
+    for (uint32_t v = start; v < end; ++v) rank += (b.tmp[v].y < e.y) ? 1u : 0u;
+  }
   const uint32_t dest = start + rank;
   b.sid[dest] = e.y;
   b.spos[dest] = b.pos[e.x];
@@ -345,9 +370,20 @@ __global__ __launch_bounds__(256) void k_density(const DevUniforms *__restrict__
   rec[2] = make_float4(nx * sc, ny * sc, nz * sc, 0.f);
 }
 
+// Direction of a pair closer than 0.1 h: normalize(sin(d), cos(2d), sin(3d))
+// of the sorted-slot difference d = s - j (sph_force_3d_grid.wgsl:163-172).
+// Kept out of line: it is rare, and inlined it would hold registers the
+// common path needs.
+__device__ __noinline__ float3 close_dir(uint32_t s, uint32_t j) {
+  const float idx = (float)s - (float)j;
+  const float a = sinf(idx * 1.0f), c = cosf(idx * 2.0f), e = sinf(idx * 3.0f);
+  const float il = __builtin_amdgcn_rsqf(dot3(a, c, e, a, c, e));
+  return make_float3(a * il, c * il, e * il);
+}
+
 // Walk lane g's in-range list, or, if it overflowed, the same candidate
 // subset again with the same test; f(j, p_j) sees every pair once.
-template <int L, class F>
+template <int L, int UNROLL, class F>
 __device__ __forceinline__ void for_each_neighbour(const DevUniforms *__restrict__ U,
                                                    const Bufs &b, uint32_t n, uint32_t g,
                                                    uint32_t sub, float4 pi, F &&f) {
@@ -358,13 +394,26 @@ __device__ __forceinline__ void for_each_neighbour(const DevUniforms *__restrict
     const uint64_t stride = (uint64_t)n * L;
     const uint32_t *__restrict__ nbr = b.nbr + g;
     uint32_t k = 0;
-    for (; k + 1 < cnt; k += 2) {
-      const uint32_t j0 = nbr[(uint64_t)k * stride], j1 = nbr[(uint64_t)(k + 1) * stride];
-      const float4 p0 = srec[3 * (size_t)j0], p1 = srec[3 * (size_t)j1];
-      f(j0, p0);
-      f(j1, p1);
+    if (UNROLL == 4) {
+      for (; k + 3 < cnt; k += 4) {
+        const uint32_t j0 = nbr[(uint64_t)k * stride], j1 = nbr[(uint64_t)(k + 1) * stride];
+        const uint32_t j2 = nbr[(uint64_t)(k + 2) * stride], j3 = nbr[(uint64_t)(k + 3) * stride];
+        const float4 p0 = srec[3 * (size_t)j0], p1 = srec[3 * (size_t)j1];
+        const float4 p2 = srec[3 * (size_t)j2], p3 = srec[3 * (size_t)j3];
+        f(j0, p0);
+        f(j1, p1);
+        f(j2, p2);
+        f(j3, p3);
+      }
+    } else {
+      for (; k + 1 < cnt; k += 2) {
+        const uint32_t j0 = nbr[(uint64_t)k * stride], j1 = nbr[(uint64_t)(k + 1) * stride];
+        const float4 p0 = srec[3 * (size_t)j0], p1 = srec[3 * (size_t)j1];
+        f(j0, p0);
+        f(j1, p1);
+      }
     }
-    if (k < cnt) {
+    for (; k < cnt; ++k) {
       const uint32_t j0 = nbr[(uint64_t)k * stride];
       f(j0, srec[3 * (size_t)j0]);
     }
@@ -401,7 +450,7 @@ __global__ __launch_bounds__(256) void k_xsph(const DevUniforms *__restrict__ U,
   const float h2 = U->h2, mass = U->mass, ps = U->poly6_scale;
   float cxv = 0.f, cyv = 0.f, czv = 0.f;
   const float4 *__restrict__ srec = b.srec;
-  for_each_neighbour<L>(U, b, n, g, sub, pi, [&](uint32_t j, float4 pj) {
+  for_each_neighbour<L, 4>(U, b, n, g, sub, pi, [&](uint32_t j, float4 pj) {
     const float rx = pi.x - pj.x, ry = pi.y - pj.y, rz = pi.z - pj.z;
     const float r2 = dot3(rx, ry, rz, rx, ry, rz);
     const float4 vj = srec[3 * (size_t)j + 1];
@@ -451,7 +500,7 @@ __global__ __launch_bounds__(256) void k_force(const DevUniforms *__restrict__ U
   float apx = 0.f, apy = 0.f, apz = 0.f;
   float acx = 0.f, acy = 0.f, acz = 0.f;
   const float4 *__restrict__ srec = b.srec;
-  for_each_neighbour<L>(U, b, n, g, sub, pi, [&](uint32_t j, float4 pj) {
+  for_each_neighbour<L, 2>(U, b, n, g, sub, pi, [&](uint32_t j, float4 pj) {
     const float rx = pi.x - pj.x, ry = pi.y - pj.y, rz = pi.z - pj.z;
     const float r2 = dot3(rx, ry, rz, rx, ry, rz);
     const float4 vj = srec[3 * (size_t)j + 1];
@@ -461,10 +510,8 @@ __global__ __launch_bounds__(256) void k_force(const DevUniforms *__restrict__ U
     const float r = fast_sqrt(r2);
     float dx, dy, dz;
     if (close) {
-      const float idx = (float)s - (float)j;
-      const float a = sinf(idx * 1.0f), c = cosf(idx * 2.0f), e = sinf(idx * 3.0f);
-      const float il = __builtin_amdgcn_rsqf(dot3(a, c, e, a, c, e));
-      dx = a * il; dy = c * il; dz = e * il;
+      const float3 d3 = close_dir(s, j);
+      dx = d3.x; dy = d3.y; dz = d3.z;
     } else {
       const float il = __builtin_amdgcn_rcpf(r);
       dx = -rx * il; dy = -ry * il; dz = -rz * il;
@@ -801,10 +848,11 @@ __global__ __launch_bounds__(256) void k_integrate(const DevUniforms *__restrict
                          dot3(fw[2][0], fw[2][1], fw[2][2], mvx, mvy, mvz), 0.f);
 }
 
-// Candidate / in-range pair totals of the density stencil.
+// Candidate / in-range pair totals (and per-particle maxima) of the density
+// stencil: out4 = {sum candidates, sum in range, max candidates, max in range}.
 __global__ __launch_bounds__(256) void k_count_pairs(const DevUniforms *__restrict__ U,
                                                      Bufs b, uint32_t n,
-                                                     unsigned long long *out2) {
+                                                     unsigned long long *out4) {
   const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
   unsigned long long cand = 0, inr = 0;
   if (s < n) {
@@ -819,13 +867,18 @@ __global__ __launch_bounds__(256) void k_count_pairs(const DevUniforms *__restri
       if (dot3(rx, ry, rz, rx, ry, rz) < h2) ++inr;
     });
   }
+  unsigned long long mc = cand, mi = inr;
   for (int o = 32; o >= 1; o >>= 1) {
     cand += __shfl_xor(cand, o);
     inr += __shfl_xor(inr, o);
+    mc = max(mc, (unsigned long long)__shfl_xor(mc, o));
+    mi = max(mi, (unsigned long long)__shfl_xor(mi, o));
   }
   if ((threadIdx.x & 63) == 0) {
-    atomicAdd(&out2[0], cand);
-    atomicAdd(&out2[1], inr);
+    atomicAdd(&out4[0], cand);
+    atomicAdd(&out4[1], inr);
+    atomicMax(&out4[2], mc);
+    atomicMax(&out4[3], mi);
   }
 }
 
@@ -903,7 +956,8 @@ hipError_t launch_solve(const DevUniforms *u, const Bufs &b, uint32_t n, int in_
                         SweepCfg c, hipStream_t st) {
   if (n == 0) return hipSuccess;
   const PredBuf in = b.pred[in_buf], out = b.pred[1 - in_buf];
-  FLUID_DISPATCH_L(k_solve, c, n, u, b, n, in, out);
+  const SweepCfg cs{c.solve_lanes, c.solve_lanes};
+  FLUID_DISPATCH_L(k_solve, cs, n, u, b, n, in, out);
   return hipGetLastError();
 }
 
@@ -922,10 +976,10 @@ hipError_t launch_integrate(const DevUniforms *u, const Bufs &b, uint32_t n, hip
 }
 
 hipError_t launch_count_pairs(const DevUniforms *u, const Bufs &b, uint32_t n,
-                              unsigned long long *out2, hipStream_t st) {
+                              unsigned long long *out4, hipStream_t st) {
   if (n == 0) return hipSuccess;
   hipLaunchKernelGGL(k_count_pairs, dim3(blocks_for(n, 256)), dim3(256), 0, st, u, b, n,
-                     out2);
+                     out4);
   return hipGetLastError();
 }
 

@@ -229,6 +229,26 @@ class SphSimulation3DGrid:
         self._chk(self._lib.fluid_time_stages(self._h, steps, arr))
         return {name: arr[i] for i, name in enumerate(abi.STAGES)}
 
+    def set_sweep_lanes(self, list_lanes: int = 0, solve_lanes: int = 0) -> None:
+        self._chk(self._lib.fluid_set_sweep_lanes(self._h, list_lanes, solve_lanes))
+
+    def sweep_lanes(self) -> tuple[int, int]:
+        arr = (C.c_uint32 * 2)()
+        self._chk(self._lib.fluid_get_sweep_lanes(self._h, arr))
+        return arr[0], arr[1]
+
+    def autotune(self, trials: int = 2) -> tuple[int, int]:
+        arr = (C.c_uint32 * 2)()
+        self._chk(self._lib.fluid_autotune(self._h, trials, arr))
+        return arr[0], arr[1]
+
+    def pair_stats(self) -> dict:
+        arr = (C.c_uint64 * 4)()
+        self._chk(self._lib.fluid_pair_stats(self._h, arr))
+        n = max(self.num_particles(), 1)
+        return {"candidates": arr[0] / n, "in_range": arr[1] / n,
+                "max_candidates": arr[2], "max_in_range": arr[3]}
+
     def count_pairs(self) -> tuple[int, int]:
         a, b = C.c_uint64(0), C.c_uint64(0)
         self._chk(self._lib.fluid_count_pairs(self._h, C.byref(a), C.byref(b)))

@@ -321,11 +321,30 @@ int fluid_time_steps(fluid_sim *sim, uint32_t warmup, uint32_t steps,
  * iterations). */
 int fluid_time_stages(fluid_sim *sim, uint32_t steps, float *ms_per_stage);
 
+/* Lanes cooperating on one particle in the neighbour sweeps: `list_lanes`
+ * for density/XSPH/force (they share the neighbour-list layout), and
+ * `solve_lanes` for the PCISPH solve; each 1, 2, 4 or 8.  0 keeps the
+ * current value. */
+int fluid_set_sweep_lanes(fluid_sim *sim, uint32_t list_lanes,
+                          uint32_t solve_lanes);
+int fluid_get_sweep_lanes(const fluid_sim *sim, uint32_t out[2]);
+
+/* Autotuner: times every lane configuration on the current state (each
+ * trial runs one step from a saved copy of the state, which is restored
+ * afterwards, so the simulation does not advance) and keeps the fastest.
+ * out[2] receives the chosen list/solve lanes (may be null). */
+int fluid_autotune(fluid_sim *sim, uint32_t trials, uint32_t out[2]);
+
 /* Candidate and in-range neighbour pair totals of the current sorted state
  * (the 27-cell sweep of sph_density_3d_grid.wgsl:129-167 counted, not
  * summed); used to turn stage times into flop rates. */
 int fluid_count_pairs(fluid_sim *sim, uint64_t *candidates,
                       uint64_t *in_range);
+
+/* As fluid_count_pairs, plus the largest per-particle counts:
+ * out[0] candidates, out[1] in-range pairs (self included),
+ * out[2] max candidates of one particle, out[3] max in-range of one. */
+int fluid_pair_stats(fluid_sim *sim, uint64_t out[4]);
 
 #ifdef __cplusplus
 }

@@ -66,21 +66,24 @@ def report(name, err):
 
 
 def check_fp(name, got, want, want64=None, tol=TOL):
-    """99.9% of particles within `tol` of the float32 oracle; the maximum
-    within max(tol, 2 x the float32 oracle's own maximum distance from
-    float64), and the same bound against float64."""
+    """The HIP error against the float32 oracle, at the 99.9th percentile
+    and at the maximum, within max(tol, 2 x the float32 oracle's own error
+    against float64 at that same statistic); the same maximum bound against
+    float64."""
     err = rowwise_rel_err(got, want)
     report(name, err)
-    bound = tol
+    bound_q = bound_max = tol
     if want64 is not None:
-        noise = rowwise_rel_err(want, want64).max()
-        bound = max(tol, 2.0 * noise)
+        noise = rowwise_rel_err(want, want64)
+        bound_q = max(tol, 2.0 * np.quantile(noise, 0.999))
+        bound_max = max(tol, 2.0 * noise.max())
         e64 = rowwise_rel_err(got, want64).max()
-        print(f"    f32-oracle noise {noise:.3e}  gpu-vs-f64 {e64:.3e}  bound {bound:.3e}",
-              flush=True)
-        assert e64 <= bound, f"{name}: gpu vs f64 {e64:.3e} > {bound:.3e}"
-    assert np.quantile(err, 0.999) <= tol, f"{name}: p99.9 rel err > {tol}"
-    assert err.max() <= bound, f"{name}: max rel err {err.max():.3e} > {bound:.3e}"
+        print(f"    f32-oracle noise p99.9 {np.quantile(noise, 0.999):.3e} max {noise.max():.3e}"
+              f"  gpu-vs-f64 max {e64:.3e}", flush=True)
+        assert e64 <= bound_max, f"{name}: gpu vs f64 {e64:.3e} > {bound_max:.3e}"
+    q = np.quantile(err, 0.999)
+    assert q <= bound_q, f"{name}: p99.9 rel err {q:.3e} > {bound_q:.3e}"
+    assert err.max() <= bound_max, f"{name}: max rel err {err.max():.3e} > {bound_max:.3e}"
 
 
 def compare_grid(sim, o):
