@@ -159,8 +159,8 @@ PROTOTYPES = {
     "fluid_time_stages": (C.c_int, [P, u32, C.POINTER(C.c_float)]),
     "fluid_count_pairs": (C.c_int, [P, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
     "fluid_pair_stats": (C.c_int, [P, C.POINTER(C.c_uint64)]),
-    "fluid_set_sweep_lanes": (C.c_int, [P, u32, u32]),
-    "fluid_get_sweep_lanes": (C.c_int, [P, C.POINTER(u32)]),
+    "fluid_set_sweep_config": (C.c_int, [P, C.POINTER(u32)]),
+    "fluid_get_sweep_config": (C.c_int, [P, C.POINTER(u32)]),
     "fluid_autotune": (C.c_int, [P, u32, C.POINTER(u32)]),
 }
 

@@ -118,7 +118,7 @@ constexpr uint32_t kScanItems = 4;
 constexpr uint32_t kScanTile = kScanThreads * kScanItems;
 
 // In-range neighbour entries stored per particle (split over its lanes).
-constexpr uint32_t kListCapacity = 384;
+constexpr uint32_t kListCapacity = 1024;
 
 // Error bits written by kernels into Bufs::err.
 constexpr uint32_t kErrScanSpin = 1u;
@@ -127,6 +127,8 @@ constexpr uint32_t kErrScanSpin = 1u;
 struct SweepCfg {
   int lanes;        // lanes per particle in density / XSPH / force
   int solve_lanes;  // lanes per particle in the PCISPH solve
+  int xcd_tiles;    // 1: each XCD takes one contiguous run of tiles;
+                    // 0: hardware round-robin order
 };
 
 // Host-side launchers (sph_kernels.hip).  All enqueue on `st`.

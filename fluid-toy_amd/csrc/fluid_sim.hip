@@ -61,10 +61,10 @@ struct fluid_sim {
   std::vector<float> extra;
   hipGraphExec_t graph = nullptr;
   uint32_t graph_n = 0, graph_iters = 0;
-  int graph_lanes = 0, graph_solve_lanes = 0;
+  int graph_lanes = 0, graph_solve_lanes = 0, graph_xcd = 0;
   int cursor = -1;  // last completed stage of a partial step, -1 = none
   GpuRigidBodyAccum last_accum{};
-  SweepCfg cfg{1, 1};
+  SweepCfg cfg{8, 8, 0};
   bool tuned = false;  // cfg chosen by fluid_set_sweep_lanes / fluid_autotune
   uint32_t ntiles = 0;
 };
@@ -85,7 +85,7 @@ SweepCfg pick_cfg(uint32_t) {
   // Default before tuning: 8 lanes per particle, the fastest setting the
   // autotuner found at every size measured (8k to 1M particles, compressed
   // and relaxed states).
-  return SweepCfg{8, 8};
+  return SweepCfg{8, 8, 0};
 }
 
 void update_cfg(fluid_sim *s) {
@@ -165,7 +165,8 @@ void drop_graph(fluid_sim *s) {
 
 int ensure_graph(fluid_sim *s) {
   if (s->graph && s->graph_n == s->n && s->graph_iters == s->iters &&
-      s->graph_lanes == s->cfg.lanes && s->graph_solve_lanes == s->cfg.solve_lanes)
+      s->graph_lanes == s->cfg.lanes && s->graph_solve_lanes == s->cfg.solve_lanes &&
+      s->graph_xcd == s->cfg.xcd_tiles)
     return FLUID_OK;
   drop_graph(s);
   hipGraph_t g = nullptr;
@@ -187,6 +188,7 @@ int ensure_graph(fluid_sim *s) {
   s->graph_iters = s->iters;
   s->graph_lanes = s->cfg.lanes;
   s->graph_solve_lanes = s->cfg.solve_lanes;
+  s->graph_xcd = s->cfg.xcd_tiles;
   return FLUID_OK;
 }
 
@@ -703,25 +705,28 @@ int fluid_time_stages(fluid_sim *s, uint32_t steps, float *ms_per_stage) {
 
 static bool valid_lanes(uint32_t l) { return l == 1 || l == 2 || l == 4 || l == 8; }
 
-int fluid_set_sweep_lanes(fluid_sim *s, uint32_t list_lanes, uint32_t solve_lanes) {
-  if (!s) return fail(FLUID_ERR_INVALID_ARG, "null handle");
-  if ((list_lanes && !valid_lanes(list_lanes)) || (solve_lanes && !valid_lanes(solve_lanes)))
+int fluid_set_sweep_config(fluid_sim *s, const uint32_t cfg[3]) {
+  if (!s || !cfg) return fail(FLUID_ERR_INVALID_ARG, "null argument");
+  if ((cfg[0] && !valid_lanes(cfg[0])) || (cfg[1] && !valid_lanes(cfg[1])))
     return fail(FLUID_ERR_INVALID_ARG, "lanes must be 1, 2, 4 or 8");
+  if (cfg[2] > 1) return fail(FLUID_ERR_INVALID_ARG, "xcd mapping must be 0 or 1");
   if (s->cursor != -1) return fail(FLUID_ERR_STATE, "a partial step is in progress");
-  if (list_lanes) s->cfg.lanes = (int)list_lanes;
-  if (solve_lanes) s->cfg.solve_lanes = (int)solve_lanes;
+  if (cfg[0]) s->cfg.lanes = (int)cfg[0];
+  if (cfg[1]) s->cfg.solve_lanes = (int)cfg[1];
+  s->cfg.xcd_tiles = (int)cfg[2];
   s->tuned = true;
   return FLUID_OK;
 }
 
-int fluid_get_sweep_lanes(const fluid_sim *s, uint32_t out[2]) {
+int fluid_get_sweep_config(const fluid_sim *s, uint32_t out[3]) {
   if (!s || !out) return fail(FLUID_ERR_INVALID_ARG, "null argument");
   out[0] = (uint32_t)s->cfg.lanes;
   out[1] = (uint32_t)s->cfg.solve_lanes;
+  out[2] = (uint32_t)s->cfg.xcd_tiles;
   return FLUID_OK;
 }
 
-int fluid_autotune(fluid_sim *s, uint32_t trials, uint32_t out[2]) {
+int fluid_autotune(fluid_sim *s, uint32_t trials, uint32_t out[3]) {
   if (!s) return fail(FLUID_ERR_INVALID_ARG, "null handle");
   if (s->cursor != -1) return fail(FLUID_ERR_STATE, "a partial step is in progress");
   HIP_TRY(hipSetDevice(s->device));
@@ -753,22 +758,28 @@ int fluid_autotune(fluid_sim *s, uint32_t trials, uint32_t out[2]) {
   const SweepCfg keep = s->cfg;
   const int lanes[4] = {1, 2, 4, 8};
   float best_list = 1e30f, best_solve = 1e30f;
-  SweepCfg best{keep.lanes, keep.solve_lanes};
+  SweepCfg best = keep;
   int rc = FLUID_OK;
   hipError_t e = copy(true);
-  for (int li = 0; li < 4 && e == hipSuccess && rc == FLUID_OK; ++li) {
-    s->cfg.lanes = lanes[li];
-    s->cfg.solve_lanes = lanes[li];
-    float ms[FLUID_STAGE_COUNT];
-    float tl = 0.f, ts = 0.f;
-    for (uint32_t t = 0; t < trials && rc == FLUID_OK; ++t) {
-      if ((e = copy(false)) != hipSuccess) break;
-      rc = fluid_time_stages(s, 1, ms);
-      tl += ms[FLUID_STAGE_DENSITY] + ms[FLUID_STAGE_XSPH] + ms[FLUID_STAGE_FORCE];
-      ts += ms[FLUID_STAGE_SOLVE];
+  // Lanes per particle (density/XSPH/force and solve timed independently)
+  // crossed with the two workgroup-to-XCD mappings.
+  for (int xi = 0; xi < 2 && e == hipSuccess && rc == FLUID_OK; ++xi) {
+    for (int li = 0; li < 4 && e == hipSuccess && rc == FLUID_OK; ++li) {
+      s->cfg.lanes = lanes[li];
+      s->cfg.solve_lanes = lanes[li];
+      s->cfg.xcd_tiles = xi;
+      float ms[FLUID_STAGE_COUNT];
+      float tl = 0.f, ts = 0.f;
+      for (uint32_t t = 0; t < trials && rc == FLUID_OK; ++t) {
+        if ((e = copy(false)) != hipSuccess) break;
+        rc = fluid_time_stages(s, 1, ms);
+        tl += ms[FLUID_STAGE_DENSITY] + ms[FLUID_STAGE_XSPH] + ms[FLUID_STAGE_FORCE];
+        ts += ms[FLUID_STAGE_SOLVE];
+      }
+      if (tl + ts < best_list + best_solve) best.xcd_tiles = xi;
+      if (tl < best_list) { best_list = tl; best.lanes = lanes[li]; }
+      if (ts < best_solve) { best_solve = ts; best.solve_lanes = lanes[li]; }
     }
-    if (tl < best_list) { best_list = tl; best.lanes = lanes[li]; }
-    if (ts < best_solve) { best_solve = ts; best.solve_lanes = lanes[li]; }
   }
   if (e == hipSuccess) e = copy(false);
   if (e == hipSuccess)
@@ -789,6 +800,7 @@ int fluid_autotune(fluid_sim *s, uint32_t trials, uint32_t out[2]) {
   if (out) {
     out[0] = (uint32_t)best.lanes;
     out[1] = (uint32_t)best.solve_lanes;
+    out[2] = (uint32_t)best.xcd_tiles;
   }
   return FLUID_OK;
 }

@@ -66,8 +66,9 @@ __device__ __forceinline__ float group_sum(float v) {
 // Workgroup index remapped so that each XCD (workgroups are dealt to the 8
 // XCDs round-robin) receives one contiguous run of tiles: neighbouring
 // particles then share that XCD's L2.
-__device__ __forceinline__ uint32_t xcd_block() {
+__device__ __forceinline__ uint32_t xcd_block(uint32_t remap) {
   const uint32_t nb = gridDim.x, i = blockIdx.x;
+  if (!remap) return i;
   const uint32_t q = nb >> 3, r = nb & 7u, x = i & 7u, k = i >> 3;
   return x * q + min(x, r) + k;
 }
@@ -310,9 +311,9 @@ __global__ __launch_bounds__(256) void k_rank_gather(Bufs b, uint32_t n) {
 // later gathers.
 template <int L>
 __global__ __launch_bounds__(256) void k_density(const DevUniforms *__restrict__ U,
-                                                 Bufs b, uint32_t n) {
+                                                 Bufs b, uint32_t n, uint32_t xcd) {
   constexpr uint32_t kCap = kListCapacity / L;
-  const uint32_t g = xcd_block() * blockDim.x + threadIdx.x;
+  const uint32_t g = xcd_block(xcd) * blockDim.x + threadIdx.x;
   const uint32_t s = g / L, sub = g % L;
   if (s >= n) return;
   const float4 pi = b.spos[s];
@@ -405,13 +406,22 @@ __device__ __forceinline__ void for_each_neighbour(const DevUniforms *__restrict
         f(j2, p2);
         f(j3, p3);
       }
-    } else {
-      for (; k + 1 < cnt; k += 2) {
-        const uint32_t j0 = nbr[(uint64_t)k * stride], j1 = nbr[(uint64_t)(k + 1) * stride];
+    } else if (cnt >= 2) {
+      // two pairs per trip; the next trip's list entries are loaded one trip
+      // ahead so only the record gather waits on memory
+      uint32_t j0 = nbr[0], j1 = nbr[stride];
+      for (; k + 3 < cnt; k += 2) {
+        const uint32_t n0 = nbr[(uint64_t)(k + 2) * stride], n1 = nbr[(uint64_t)(k + 3) * stride];
         const float4 p0 = srec[3 * (size_t)j0], p1 = srec[3 * (size_t)j1];
         f(j0, p0);
         f(j1, p1);
+        j0 = n0;
+        j1 = n1;
       }
+      const float4 p0 = srec[3 * (size_t)j0], p1 = srec[3 * (size_t)j1];
+      f(j0, p0);
+      f(j1, p1);
+      k += 2;
     }
     for (; k < cnt; ++k) {
       const uint32_t j0 = nbr[(uint64_t)k * stride];
@@ -436,8 +446,8 @@ __device__ __forceinline__ void for_each_neighbour(const DevUniforms *__restrict
 // copy, which is what is written then.
 template <int L>
 __global__ __launch_bounds__(256) void k_xsph(const DevUniforms *__restrict__ U,
-                                              Bufs b, uint32_t n) {
-  const uint32_t g = xcd_block() * blockDim.x + threadIdx.x;
+                                              Bufs b, uint32_t n, uint32_t xcd) {
+  const uint32_t g = xcd_block(xcd) * blockDim.x + threadIdx.x;
   const uint32_t s = g / L, sub = g % L;
   if (s >= n) return;
   const float eps = U->eps;
@@ -477,8 +487,8 @@ __global__ __launch_bounds__(256) void k_xsph(const DevUniforms *__restrict__ U,
 // the exact bits the oracle uses.
 template <int L>
 __global__ __launch_bounds__(256) void k_force(const DevUniforms *__restrict__ U,
-                                               Bufs b, uint32_t n) {
-  const uint32_t g = xcd_block() * blockDim.x + threadIdx.x;
+                                               Bufs b, uint32_t n, uint32_t xcd) {
+  const uint32_t g = xcd_block(xcd) * blockDim.x + threadIdx.x;
   const uint32_t s = g / L, sub = g % L;
   if (s >= n) return;
   const float4 *rec_i = b.srec + 3 * (size_t)s;
@@ -571,8 +581,8 @@ __global__ __launch_bounds__(256) void k_predict(const DevUniforms *__restrict__
 template <int L>
 __global__ __launch_bounds__(256) void k_solve(const DevUniforms *__restrict__ U,
                                                Bufs b, uint32_t n, PredBuf in,
-                                               PredBuf out) {
-  const uint32_t g = xcd_block() * blockDim.x + threadIdx.x;
+                                               PredBuf out, uint32_t xcd) {
+  const uint32_t g = xcd_block(xcd) * blockDim.x + threadIdx.x;
   const uint32_t s = g / L, sub = g % L;
   if (s >= n) return;
   const float4 xi = in.xq[s];
@@ -928,21 +938,21 @@ hipError_t launch_reorder(const DevUniforms *, const Bufs &b, uint32_t n, hipStr
 hipError_t launch_density(const DevUniforms *u, const Bufs &b, uint32_t n, SweepCfg c,
                           hipStream_t st) {
   if (n == 0) return hipSuccess;
-  FLUID_DISPATCH_L(k_density, c, n, u, b, n);
+  FLUID_DISPATCH_L(k_density, c, n, u, b, n, (uint32_t)c.xcd_tiles);
   return hipGetLastError();
 }
 
 hipError_t launch_xsph(const DevUniforms *u, const Bufs &b, uint32_t n, SweepCfg c,
                        hipStream_t st) {
   if (n == 0) return hipSuccess;
-  FLUID_DISPATCH_L(k_xsph, c, n, u, b, n);
+  FLUID_DISPATCH_L(k_xsph, c, n, u, b, n, (uint32_t)c.xcd_tiles);
   return hipGetLastError();
 }
 
 hipError_t launch_force(const DevUniforms *u, const Bufs &b, uint32_t n, SweepCfg c,
                         hipStream_t st) {
   if (n == 0) return hipSuccess;
-  FLUID_DISPATCH_L(k_force, c, n, u, b, n);
+  FLUID_DISPATCH_L(k_force, c, n, u, b, n, (uint32_t)c.xcd_tiles);
   return hipGetLastError();
 }
 
@@ -956,8 +966,8 @@ hipError_t launch_solve(const DevUniforms *u, const Bufs &b, uint32_t n, int in_
                         SweepCfg c, hipStream_t st) {
   if (n == 0) return hipSuccess;
   const PredBuf in = b.pred[in_buf], out = b.pred[1 - in_buf];
-  const SweepCfg cs{c.solve_lanes, c.solve_lanes};
-  FLUID_DISPATCH_L(k_solve, cs, n, u, b, n, in, out);
+  const SweepCfg cs{c.solve_lanes, c.solve_lanes, c.xcd_tiles};
+  FLUID_DISPATCH_L(k_solve, cs, n, u, b, n, in, out, (uint32_t)c.xcd_tiles);
   return hipGetLastError();
 }
 

@@ -229,18 +229,20 @@ class SphSimulation3DGrid:
         self._chk(self._lib.fluid_time_stages(self._h, steps, arr))
         return {name: arr[i] for i, name in enumerate(abi.STAGES)}
 
-    def set_sweep_lanes(self, list_lanes: int = 0, solve_lanes: int = 0) -> None:
-        self._chk(self._lib.fluid_set_sweep_lanes(self._h, list_lanes, solve_lanes))
+    def set_sweep_config(self, list_lanes: int = 0, solve_lanes: int = 0,
+                         xcd_tiles: int = 0) -> None:
+        arr = (C.c_uint32 * 3)(list_lanes, solve_lanes, xcd_tiles)
+        self._chk(self._lib.fluid_set_sweep_config(self._h, arr))
 
-    def sweep_lanes(self) -> tuple[int, int]:
-        arr = (C.c_uint32 * 2)()
-        self._chk(self._lib.fluid_get_sweep_lanes(self._h, arr))
-        return arr[0], arr[1]
+    def sweep_config(self) -> tuple[int, int, int]:
+        arr = (C.c_uint32 * 3)()
+        self._chk(self._lib.fluid_get_sweep_config(self._h, arr))
+        return tuple(arr)
 
-    def autotune(self, trials: int = 2) -> tuple[int, int]:
-        arr = (C.c_uint32 * 2)()
+    def autotune(self, trials: int = 2) -> tuple[int, int, int]:
+        arr = (C.c_uint32 * 3)()
         self._chk(self._lib.fluid_autotune(self._h, trials, arr))
-        return arr[0], arr[1]
+        return tuple(arr)
 
     def pair_stats(self) -> dict:
         arr = (C.c_uint64 * 4)()

@@ -321,19 +321,21 @@ int fluid_time_steps(fluid_sim *sim, uint32_t warmup, uint32_t steps,
  * iterations). */
 int fluid_time_stages(fluid_sim *sim, uint32_t steps, float *ms_per_stage);
 
-/* Lanes cooperating on one particle in the neighbour sweeps: `list_lanes`
- * for density/XSPH/force (they share the neighbour-list layout), and
- * `solve_lanes` for the PCISPH solve; each 1, 2, 4 or 8.  0 keeps the
- * current value. */
-int fluid_set_sweep_lanes(fluid_sim *sim, uint32_t list_lanes,
-                          uint32_t solve_lanes);
-int fluid_get_sweep_lanes(const fluid_sim *sim, uint32_t out[2]);
+/* Launch configuration of the neighbour sweeps, cfg[3]:
+ *   cfg[0] lanes cooperating on one particle in density/XSPH/force (they
+ *          share the neighbour-list layout), cfg[1] the same for the PCISPH
+ *          solve: 1, 2, 4 or 8 (0 keeps the current value);
+ *   cfg[2] workgroup-to-XCD mapping: 1 gives each of the 8 XCDs one
+ *          contiguous run of tiles (shared L2 between neighbours), 0 keeps
+ *          the hardware's round-robin order (spreads dense regions). */
+int fluid_set_sweep_config(fluid_sim *sim, const uint32_t cfg[3]);
+int fluid_get_sweep_config(const fluid_sim *sim, uint32_t out[3]);
 
-/* Autotuner: times every lane configuration on the current state (each
+/* Autotuner: times every launch configuration on the current state (each
  * trial runs one step from a saved copy of the state, which is restored
  * afterwards, so the simulation does not advance) and keeps the fastest.
- * out[2] receives the chosen list/solve lanes (may be null). */
-int fluid_autotune(fluid_sim *sim, uint32_t trials, uint32_t out[2]);
+ * out[3] receives the chosen configuration (may be null). */
+int fluid_autotune(fluid_sim *sim, uint32_t trials, uint32_t out[3]);
 
 /* Candidate and in-range neighbour pair totals of the current sorted state
  * (the 27-cell sweep of sph_density_3d_grid.wgsl:129-167 counted, not
