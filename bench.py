@@ -111,7 +111,19 @@ def main() -> int:
     ap.add_argument("--stage-steps", type=int, default=20)
     ap.add_argument("--no-1m", action="store_true", help="skip the 1M-particle block")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only run a 256 MiB device copy (scales the FETCH_SIZE/WRITE_SIZE "
+                         "counters of a profiling pass to bytes)")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        import torch
+        x = torch.ones(64 << 20, dtype=torch.float32, device="cuda")
+        y = torch.empty_like(x)
+        for _ in range(5):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read": x.numel() * 4, "copy_bytes_written": y.numel() * 4}))
+        return 0
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
               "the multi-GPU slab decomposition is a design, not an implementation.",
