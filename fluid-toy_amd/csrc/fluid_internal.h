@@ -12,10 +12,11 @@
 //     srec            per sorted slot, three float4 written by the density
 //                     pass and gathered by XSPH and force:
 //                       (pos, density), (vel, near_density), (normal, 0)
-//     nbr, ncnt       in-range neighbour lists of the density pass: entry k
-//                     of lane g (= slot * lanes + lane) at nbr[k * n * lanes
-//                     + g], so a wave's k-th entries are one contiguous run;
-//                     ncnt[g] > capacity marks an overflowed list
+//     nbr, ncnt       in-range neighbour lists of the density pass: entry e
+//                     of slot s at nbr[(e / lanes) * n * lanes + s * lanes
+//                     + e % lanes], i.e. lane e % lanes's share, so a wave's
+//                     k-th reads are one contiguous run; ncnt[s] is the
+//                     entry count, > kListCapacity marking an overflow
 //     pred[2]         sorted_predicted_{a,b} split into
 //                       xq = (x*, q) with q = p / max(rho*,1)^2,
 //                       pr = (p, rho*), v = (v*, 0)

@@ -77,8 +77,9 @@ __device__ __forceinline__ uint32_t xcd_block(uint32_t remap) {
 // x-rows: cells (x-1..x+1, y, z) are consecutive cell indices, so their
 // particles form one contiguous sorted range.  The visit order (dz, dy, dx,
 // then slot) is the reference's.  Lane `sub` of an L-lane group takes every
-// L-th candidate of each row; f(j, p_j) receives the candidate's float4 from
-// `pts`, loaded four at a time to keep several gathers in flight.
+// L-th candidate of each row.  The walk is group-synchronous: all L lanes of
+// a particle make the same trips (four candidates each, the tail predicated
+// by `valid`), so group-wide ballots inside f(j, p_j, valid) see every lane.
 template <int L, class F>
 __device__ __forceinline__ void sweep27(const DevUniforms *__restrict__ U,
                                         const uint32_t *__restrict__ counts,
@@ -100,15 +101,17 @@ __device__ __forceinline__ void sweep27(const DevUniforms *__restrict__ U,
       const uint32_t c0 = row + (uint32_t)x0, c1 = row + (uint32_t)x1;
       const uint32_t begin = starts[c0] - counts[c0];
       const uint32_t end = starts[c1];
-      uint32_t j = begin + sub;
-      for (; j + 3 * L < end; j += 4 * L) {
-        const float4 p0 = pts[j], p1 = pts[j + L], p2 = pts[j + 2 * L], p3 = pts[j + 3 * L];
-        f(j, p0);
-        f(j + L, p1);
-        f(j + 2 * L, p2);
-        f(j + 3 * L, p3);
+      if (begin >= end) continue;
+      const uint32_t last = end - 1;
+      for (uint32_t base = begin; base < end; base += 4 * L) {
+        const uint32_t j0 = base + sub, j1 = j0 + L, j2 = j1 + L, j3 = j2 + L;
+        const float4 p0 = pts[min(j0, last)], p1 = pts[min(j1, last)];
+        const float4 p2 = pts[min(j2, last)], p3 = pts[min(j3, last)];
+        f(j0, p0, j0 < end);
+        f(j1, p1, j1 < end);
+        f(j2, p2, j2 < end);
+        f(j3, p3, j3 < end);
       }
-      for (; j < end; j += L) f(j, pts[j]);
     }
   }
 }
@@ -322,29 +325,40 @@ __global__ __launch_bounds__(256) void k_density(const DevUniforms *__restrict__
   const int cz = cell_coord(pi.z, U->oz, U->inv_cell);
   const float h = U->h, h2 = U->h2, mass = U->mass;
   const float ps = U->poly6_scale, ns = U->near_scale, mr = U->m_over_rho0;
-  const uint64_t stride = (uint64_t)n * L;
-  uint32_t *__restrict__ nbr = b.nbr + g;
-  uint32_t cnt = 0;
+  const size_t stride = (size_t)n * L;
+  uint32_t *__restrict__ nbr = b.nbr + (size_t)s * L;
+  const uint32_t lane = threadIdx.x & 63u;
+  const uint32_t gshift = lane & ~(uint32_t)(L - 1);
+  const uint32_t below = (1u << sub) - 1u;
+  uint32_t run = 0;  // pairs recorded so far by the whole group
   float dens = 0.f, near = 0.f, nx = 0.f, ny = 0.f, nz = 0.f;
-  sweep27<L>(U, b.counts, b.starts, b.spos, cx, cy, cz, sub, [&](uint32_t j, float4 pj) {
+  // Branch-free body: a pair outside the support contributes exact zeros
+  // through selects, so the lanes of a wave stay converged.  The group's
+  // k-th recorded pair (ranked across lanes by a ballot, in a fixed order)
+  // goes to lane k % L's sub-list at depth k / L, so every lane of XSPH and
+  // force later walks an equal share.
+  sweep27<L>(U, b.counts, b.starts, b.spos, cx, cy, cz, sub,
+             [&](uint32_t j, float4 pj, bool valid) {
     const float rx = pi.x - pj.x, ry = pi.y - pj.y, rz = pi.z - pj.z;
     const float r2 = dot3(rx, ry, rz, rx, ry, rz);
-    if (r2 < h2) {
-      const float r = fast_sqrt(r2);
-      const float d = h2 - r2;
-      dens += mass * (ps * d * d * d);
-      const float dn = h - r;
-      near += mass * (ns * dn * dn * dn);
-      if (r2 > 1e-12f) {
-        nx += mr * rx * d * d;
-        ny += mr * ry * d * d;
-        nz += mr * rz * d * d;
-        if (cnt < kCap) nbr[(uint64_t)cnt * stride] = j;
-        ++cnt;
-      }
+    const bool inr = valid && r2 < h2;
+    const bool pair = inr && r2 > 1e-12f;
+    const float d = inr ? h2 - r2 : 0.f;
+    dens += mass * (ps * d * d * d);
+    const float dn = inr ? h - fast_sqrt(r2) : 0.f;
+    near += mass * (ns * dn * dn * dn);
+    const float fn = pair ? mr * d * d : 0.f;
+    nx += fn * rx;
+    ny += fn * ry;
+    nz += fn * rz;
+    const uint32_t gm = (uint32_t)(__ballot(pair) >> gshift) & ((1u << L) - 1u);
+    if (pair) {
+      const uint32_t e = run + (uint32_t)__popc(gm & below);
+      if (e < kListCapacity) nbr[(e / L) * stride + (e % L)] = j;
     }
+    run += (uint32_t)__popc(gm);
   });
-  b.ncnt[g] = cnt;
+  if (sub == 0) b.ncnt[s] = run;
   dens = group_sum<L>(dens);
   near = group_sum<L>(near);
   nx = group_sum<L>(nx);
@@ -384,21 +398,24 @@ __device__ __noinline__ float3 close_dir(uint32_t s, uint32_t j) {
 
 // Walk lane g's in-range list, or, if it overflowed, the same candidate
 // subset again with the same test; f(j, p_j) sees every pair once.
+// Walk this lane's share of particle s's in-range list (entries sub, sub+L,
+// ... of the group's list), or, if the list overflowed, the lane's candidate
+// subset again with the same test; f(j, p_j) sees every pair once.
 template <int L, int UNROLL, class F>
 __device__ __forceinline__ void for_each_neighbour(const DevUniforms *__restrict__ U,
-                                                   const Bufs &b, uint32_t n, uint32_t g,
+                                                   const Bufs &b, uint32_t n, uint32_t s,
                                                    uint32_t sub, float4 pi, F &&f) {
-  constexpr uint32_t kCap = kListCapacity / L;
-  const uint32_t cnt = b.ncnt[g];
+  const uint32_t total = b.ncnt[s];
   const float4 *__restrict__ srec = b.srec;
-  if (cnt <= kCap) {
-    const uint64_t stride = (uint64_t)n * L;
-    const uint32_t *__restrict__ nbr = b.nbr + g;
+  if (total <= kListCapacity) {
+    const uint32_t cnt = total > sub ? (total - 1 - sub) / L + 1 : 0;
+    const size_t stride = (size_t)n * L;
+    const uint32_t *__restrict__ nbr = b.nbr + (size_t)s * L + sub;
     uint32_t k = 0;
     if (UNROLL == 4) {
       for (; k + 3 < cnt; k += 4) {
-        const uint32_t j0 = nbr[(uint64_t)k * stride], j1 = nbr[(uint64_t)(k + 1) * stride];
-        const uint32_t j2 = nbr[(uint64_t)(k + 2) * stride], j3 = nbr[(uint64_t)(k + 3) * stride];
+        const uint32_t j0 = nbr[0], j1 = nbr[stride], j2 = nbr[2 * stride], j3 = nbr[3 * stride];
+        nbr += 4 * stride;
         const float4 p0 = srec[3 * (size_t)j0], p1 = srec[3 * (size_t)j1];
         const float4 p2 = srec[3 * (size_t)j2], p3 = srec[3 * (size_t)j3];
         f(j0, p0);
@@ -410,8 +427,10 @@ __device__ __forceinline__ void for_each_neighbour(const DevUniforms *__restrict
       // two pairs per trip; the next trip's list entries are loaded one trip
       // ahead so only the record gather waits on memory
       uint32_t j0 = nbr[0], j1 = nbr[stride];
+      nbr += 2 * stride;
       for (; k + 3 < cnt; k += 2) {
-        const uint32_t n0 = nbr[(uint64_t)(k + 2) * stride], n1 = nbr[(uint64_t)(k + 3) * stride];
+        const uint32_t n0 = nbr[0], n1 = nbr[stride];
+        nbr += 2 * stride;
         const float4 p0 = srec[3 * (size_t)j0], p1 = srec[3 * (size_t)j1];
         f(j0, p0);
         f(j1, p1);
@@ -424,7 +443,8 @@ __device__ __forceinline__ void for_each_neighbour(const DevUniforms *__restrict
       k += 2;
     }
     for (; k < cnt; ++k) {
-      const uint32_t j0 = nbr[(uint64_t)k * stride];
+      const uint32_t j0 = nbr[0];
+      nbr += stride;
       f(j0, srec[3 * (size_t)j0]);
     }
     return;
@@ -433,10 +453,11 @@ __device__ __forceinline__ void for_each_neighbour(const DevUniforms *__restrict
   const int cx = cell_coord(pi.x, U->ox, U->inv_cell);
   const int cy = cell_coord(pi.y, U->oy, U->inv_cell);
   const int cz = cell_coord(pi.z, U->oz, U->inv_cell);
-  sweep27<L>(U, b.counts, b.starts, b.spos, cx, cy, cz, sub, [&](uint32_t j, float4 pj) {
+  sweep27<L>(U, b.counts, b.starts, b.spos, cx, cy, cz, sub,
+             [&](uint32_t j, float4 pj, bool valid) {
     const float rx = pi.x - pj.x, ry = pi.y - pj.y, rz = pi.z - pj.z;
     const float r2 = dot3(rx, ry, rz, rx, ry, rz);
-    if (r2 < h2 && r2 > 1e-12f) f(j, srec[3 * (size_t)j]);
+    if (valid && r2 < h2 && r2 > 1e-12f) f(j, srec[3 * (size_t)j]);
   });
 }
 
@@ -460,7 +481,7 @@ __global__ __launch_bounds__(256) void k_xsph(const DevUniforms *__restrict__ U,
   const float h2 = U->h2, mass = U->mass, ps = U->poly6_scale;
   float cxv = 0.f, cyv = 0.f, czv = 0.f;
   const float4 *__restrict__ srec = b.srec;
-  for_each_neighbour<L, 4>(U, b, n, g, sub, pi, [&](uint32_t j, float4 pj) {
+  for_each_neighbour<L, 4>(U, b, n, s, sub, pi, [&](uint32_t j, float4 pj) {
     const float rx = pi.x - pj.x, ry = pi.y - pj.y, rz = pi.z - pj.z;
     const float r2 = dot3(rx, ry, rz, rx, ry, rz);
     const float4 vj = srec[3 * (size_t)j + 1];
@@ -510,7 +531,7 @@ __global__ __launch_bounds__(256) void k_force(const DevUniforms *__restrict__ U
   float apx = 0.f, apy = 0.f, apz = 0.f;
   float acx = 0.f, acy = 0.f, acz = 0.f;
   const float4 *__restrict__ srec = b.srec;
-  for_each_neighbour<L, 2>(U, b, n, g, sub, pi, [&](uint32_t j, float4 pj) {
+  for_each_neighbour<L, 2>(U, b, n, s, sub, pi, [&](uint32_t j, float4 pj) {
     const float rx = pi.x - pj.x, ry = pi.y - pj.y, rz = pi.z - pj.z;
     const float r2 = dot3(rx, ry, rz, rx, ry, rz);
     const float4 vj = srec[3 * (size_t)j + 1];
@@ -536,8 +557,7 @@ __global__ __launch_bounds__(256) void k_force(const DevUniforms *__restrict__ U
     // Monaghan viscosity (:193-202): -m * pi_visc * grad, only if v.x < 0
     const float vdx = dot3(vi.x - vj.x, vi.y - vj.y, vi.z - vj.z, rx, ry, rz);
     const float inv_dd = __builtin_amdgcn_rcpf(di + dj);
-    if (vdx < 0.0f)
-      sp += c_visc * vdx * inv_dd * __builtin_amdgcn_rcpf(r2 + veps) * grad;
+    sp += c_visc * fminf(vdx, 0.0f) * inv_dd * __builtin_amdgcn_rcpf(r2 + veps) * grad;
     // Akinci cohesion + curvature (:206-214)
     const float mk = c_k * inv_dd;
     const float r3 = r * r * r;
@@ -593,23 +613,22 @@ __global__ __launch_bounds__(256) void k_solve(const DevUniforms *__restrict__ U
   const float h = U->h, h2 = U->h2, mass = U->mass, ps = U->poly6_scale;
   const float gs = U->grad_scale;
   float pd = 0.f, ax = 0.f, ay = 0.f, az = 0.f;
-  sweep27<L>(U, b.counts, b.starts, in.xq, cx, cy, cz, sub, [&](uint32_t, float4 xj) {
+  sweep27<L>(U, b.counts, b.starts, in.xq, cx, cy, cz, sub,
+             [&](uint32_t, float4 xj, bool valid) {
     const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
     const float r2 = dot3(rx, ry, rz, rx, ry, rz);
-    if (r2 < h2) {
-      const float d = h2 - r2;
-      pd += mass * (ps * d * d * d);
-      if (r2 > 1e-12f) {
-        const float il = __builtin_amdgcn_rsqf(r2);
-        const float r = r2 * il;
-        const float dr = h - r;
-        // -m (q_i + q_j) grad * dir, dir = -r_vec / r
-        const float sc = mass * (xi.w + xj.w) * (gs * dr * dr) * il;
-        ax += sc * rx;
-        ay += sc * ry;
-        az += sc * rz;
-      }
-    }
+    const bool inr = valid && r2 < h2;
+    const float d = inr ? h2 - r2 : 0.f;
+    pd += mass * (ps * d * d * d);
+    // -m (q_i + q_j) grad * dir with dir = -r_vec / r; zero outside the
+    // support and for coincident pairs (selected, so no divergence)
+    const float il = __builtin_amdgcn_rsqf(r2);
+    const float dr = h - r2 * il;
+    const float sc0 = mass * (xi.w + xj.w) * (gs * dr * dr) * il;
+    const float sc = (inr && r2 > 1e-12f) ? sc0 : 0.f;
+    ax += sc * rx;
+    ay += sc * ry;
+    az += sc * rz;
   });
   pd = group_sum<L>(pd);
   ax = group_sum<L>(ax);
@@ -871,10 +890,11 @@ __global__ __launch_bounds__(256) void k_count_pairs(const DevUniforms *__restri
     const int cy = cell_coord(pi.y, U->oy, U->inv_cell);
     const int cz = cell_coord(pi.z, U->oz, U->inv_cell);
     const float h2 = U->h2;
-    sweep27<1>(U, b.counts, b.starts, b.spos, cx, cy, cz, 0, [&](uint32_t, float4 pj) {
+    sweep27<1>(U, b.counts, b.starts, b.spos, cx, cy, cz, 0,
+               [&](uint32_t, float4 pj, bool valid) {
       const float rx = pi.x - pj.x, ry = pi.y - pj.y, rz = pi.z - pj.z;
-      ++cand;
-      if (dot3(rx, ry, rz, rx, ry, rz) < h2) ++inr;
+      cand += valid ? 1 : 0;
+      if (valid && dot3(rx, ry, rz, rx, ry, rz) < h2) ++inr;
     });
   }
   unsigned long long mc = cand, mi = inr;
