@@ -77,10 +77,13 @@ __device__ __forceinline__ uint32_t xcd_block(uint32_t remap) {
 // x-rows: cells (x-1..x+1, y, z) are consecutive cell indices, so their
 // particles form one contiguous sorted range.  The visit order (dz, dy, dx,
 // then slot) is the reference's.  Lane `sub` of an L-lane group takes every
-// L-th candidate of each row.  The walk is group-synchronous: all L lanes of
-// a particle make the same trips (four candidates each, the tail predicated
-// by `valid`), so group-wide ballots inside f(j, p_j, valid) see every lane.
-template <int L, class F>
+// L-th candidate of each row, four gathers in flight per trip.
+//   SYNC = false: each lane stops at its own last candidate and f(j, p_j,
+//                 true) only sees real candidates.
+//   SYNC = true:  all L lanes of a particle make the same trips, the tail
+//                 predicated by `valid`, so group-wide ballots inside
+//                 f(j, p_j, valid) see every lane of the group.
+template <int L, bool SYNC, class F>
 __device__ __forceinline__ void sweep27(const DevUniforms *__restrict__ U,
                                         const uint32_t *__restrict__ counts,
                                         const uint32_t *__restrict__ starts,
@@ -101,16 +104,28 @@ __device__ __forceinline__ void sweep27(const DevUniforms *__restrict__ U,
       const uint32_t c0 = row + (uint32_t)x0, c1 = row + (uint32_t)x1;
       const uint32_t begin = starts[c0] - counts[c0];
       const uint32_t end = starts[c1];
-      if (begin >= end) continue;
-      const uint32_t last = end - 1;
-      for (uint32_t base = begin; base < end; base += 4 * L) {
-        const uint32_t j0 = base + sub, j1 = j0 + L, j2 = j1 + L, j3 = j2 + L;
-        const float4 p0 = pts[min(j0, last)], p1 = pts[min(j1, last)];
-        const float4 p2 = pts[min(j2, last)], p3 = pts[min(j3, last)];
-        f(j0, p0, j0 < end);
-        f(j1, p1, j1 < end);
-        f(j2, p2, j2 < end);
-        f(j3, p3, j3 < end);
+      if (SYNC) {
+        if (begin >= end) continue;
+        const uint32_t last = end - 1;
+        for (uint32_t base = begin; base < end; base += 4 * L) {
+          const uint32_t j0 = base + sub, j1 = j0 + L, j2 = j1 + L, j3 = j2 + L;
+          const float4 p0 = pts[min(j0, last)], p1 = pts[min(j1, last)];
+          const float4 p2 = pts[min(j2, last)], p3 = pts[min(j3, last)];
+          f(j0, p0, j0 < end);
+          f(j1, p1, j1 < end);
+          f(j2, p2, j2 < end);
+          f(j3, p3, j3 < end);
+        }
+      } else {
+        uint32_t j = begin + sub;
+        for (; j + 3 * L < end; j += 4 * L) {
+          const float4 p0 = pts[j], p1 = pts[j + L], p2 = pts[j + 2 * L], p3 = pts[j + 3 * L];
+          f(j, p0, true);
+          f(j + L, p1, true);
+          f(j + 2 * L, p2, true);
+          f(j + 3 * L, p3, true);
+        }
+        for (; j < end; j += L) f(j, pts[j], true);
       }
     }
   }
@@ -337,8 +352,8 @@ __global__ __launch_bounds__(256) void k_density(const DevUniforms *__restrict__
   // k-th recorded pair (ranked across lanes by a ballot, in a fixed order)
   // goes to lane k % L's sub-list at depth k / L, so every lane of XSPH and
   // force later walks an equal share.
-  sweep27<L>(U, b.counts, b.starts, b.spos, cx, cy, cz, sub,
-             [&](uint32_t j, float4 pj, bool valid) {
+  sweep27<L, true>(U, b.counts, b.starts, b.spos, cx, cy, cz, sub,
+                   [&](uint32_t j, float4 pj, bool valid) {
     const float rx = pi.x - pj.x, ry = pi.y - pj.y, rz = pi.z - pj.z;
     const float r2 = dot3(rx, ry, rz, rx, ry, rz);
     const bool inr = valid && r2 < h2;
@@ -453,11 +468,11 @@ __device__ __forceinline__ void for_each_neighbour(const DevUniforms *__restrict
   const int cx = cell_coord(pi.x, U->ox, U->inv_cell);
   const int cy = cell_coord(pi.y, U->oy, U->inv_cell);
   const int cz = cell_coord(pi.z, U->oz, U->inv_cell);
-  sweep27<L>(U, b.counts, b.starts, b.spos, cx, cy, cz, sub,
-             [&](uint32_t j, float4 pj, bool valid) {
+  sweep27<L, false>(U, b.counts, b.starts, b.spos, cx, cy, cz, sub,
+                    [&](uint32_t j, float4 pj, bool) {
     const float rx = pi.x - pj.x, ry = pi.y - pj.y, rz = pi.z - pj.z;
     const float r2 = dot3(rx, ry, rz, rx, ry, rz);
-    if (valid && r2 < h2 && r2 > 1e-12f) f(j, srec[3 * (size_t)j]);
+    if (r2 < h2 && r2 > 1e-12f) f(j, srec[3 * (size_t)j]);
   });
 }
 
@@ -613,22 +628,23 @@ __global__ __launch_bounds__(256) void k_solve(const DevUniforms *__restrict__ U
   const float h = U->h, h2 = U->h2, mass = U->mass, ps = U->poly6_scale;
   const float gs = U->grad_scale;
   float pd = 0.f, ax = 0.f, ay = 0.f, az = 0.f;
-  sweep27<L>(U, b.counts, b.starts, in.xq, cx, cy, cz, sub,
-             [&](uint32_t, float4 xj, bool valid) {
+  sweep27<L, false>(U, b.counts, b.starts, in.xq, cx, cy, cz, sub,
+                    [&](uint32_t, float4 xj, bool) {
     const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
     const float r2 = dot3(rx, ry, rz, rx, ry, rz);
-    const bool inr = valid && r2 < h2;
-    const float d = inr ? h2 - r2 : 0.f;
-    pd += mass * (ps * d * d * d);
-    // -m (q_i + q_j) grad * dir with dir = -r_vec / r; zero outside the
-    // support and for coincident pairs (selected, so no divergence)
-    const float il = __builtin_amdgcn_rsqf(r2);
-    const float dr = h - r2 * il;
-    const float sc0 = mass * (xi.w + xj.w) * (gs * dr * dr) * il;
-    const float sc = (inr && r2 > 1e-12f) ? sc0 : 0.f;
-    ax += sc * rx;
-    ay += sc * ry;
-    az += sc * rz;
+    if (r2 < h2) {
+      const float d = h2 - r2;
+      pd += mass * (ps * d * d * d);
+      if (r2 > 1e-12f) {
+        // -m (q_i + q_j) grad * dir with dir = -r_vec / r
+        const float il = __builtin_amdgcn_rsqf(r2);
+        const float dr = h - r2 * il;
+        const float sc = mass * (xi.w + xj.w) * (gs * dr * dr) * il;
+        ax += sc * rx;
+        ay += sc * ry;
+        az += sc * rz;
+      }
+    }
   });
   pd = group_sum<L>(pd);
   ax = group_sum<L>(ax);
@@ -890,8 +906,8 @@ __global__ __launch_bounds__(256) void k_count_pairs(const DevUniforms *__restri
     const int cy = cell_coord(pi.y, U->oy, U->inv_cell);
     const int cz = cell_coord(pi.z, U->oz, U->inv_cell);
     const float h2 = U->h2;
-    sweep27<1>(U, b.counts, b.starts, b.spos, cx, cy, cz, 0,
-               [&](uint32_t, float4 pj, bool valid) {
+    sweep27<1, false>(U, b.counts, b.starts, b.spos, cx, cy, cz, 0,
+                      [&](uint32_t, float4 pj, bool valid) {
       const float rx = pi.x - pj.x, ry = pi.y - pj.y, rz = pi.z - pj.z;
       cand += valid ? 1 : 0;
       if (valid && dot3(rx, ry, rz, rx, ry, rz) < h2) ++inr;
