@@ -84,6 +84,35 @@ def roofline_for(stages, n, cells, iters, cand, inr):
     return roof, per_stage
 
 
+def run_batched(pkg, count: int, steps: int = 20):
+    """Batched form against the same scenes stepped one after another by
+    plain handles: T(sum of one graph step per scene) / T(one batched step).
+    Each side is warmed up, autotuned (state restored) and timed with HIP
+    events on its own stream; the sequential side sums device times, so host
+    overhead between scenes is not counted against it."""
+    scenes = pkg.scenes.batch_scenes(count)
+    batch = pkg.make_batch(scenes)
+    batch.step(5)
+    cfg = batch.autotune(1)
+    k = max(5, steps)
+    t_batch = batch.time_steps(0, k) / k
+    stages = batch.time_stages(3)
+    batch.close()
+    t_seq = 0.0
+    for sc in scenes:
+        sim = pkg.make_sim(sc)
+        sim.step(5)
+        sim.autotune(1)
+        t_seq += sim.time_steps(0, 5) / 5
+        sim.close()
+    n = sum(sc.n for sc in scenes)
+    return {"scenes": count, "particles": n, "batched_ms_per_step": round(t_batch, 4),
+            "sequential_ms_per_step": round(t_seq, 4),
+            "speedup": round(t_seq / t_batch, 3),
+            "batched_particle_updates_per_s": round(n / (t_batch / 1e3), 1),
+            "autotuned": list(cfg), "stage_ms": {a: round(b, 4) for a, b in stages.items()}}
+
+
 def cpu_baseline(pkg, orc, scene, budget_s: float = 12.0, max_steps: int = 40):
     params, cont, g = pkg.scenes.make_params(scene, pkg.abi, pkg.derive_sph_params,
                                              pkg.derive_container_geometry)
@@ -111,6 +140,8 @@ def main() -> int:
     ap.add_argument("--stage-steps", type=int, default=20)
     ap.add_argument("--no-1m", action="store_true", help="skip the 1M-particle block")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched-form comparison")
+    ap.add_argument("--batch-scenes", type=int, default=256)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a 256 MiB device copy (scales the FETCH_SIZE/WRITE_SIZE "
                          "counters of a profiling pass to bytes)")
@@ -168,6 +199,8 @@ def main() -> int:
                             "ms_per_step": round(ms1 / k1, 4),
                             "stage_ms": {k: round(v, 4) for k, v in st1.items()},
                             "roofline": roof1}
+    if not args.no_batch:
+        line["batched"] = run_batched(pkg, args.batch_scenes)
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(pkg, ge.load_oracle(), scene)
     print(json.dumps(line), flush=True)

@@ -7,7 +7,7 @@ reference's struct of the same name (src/simulation/sph_3d_grid.rs), and
 there is no CPU fallback, and a missing library raises FileNotFoundError.
 """
 from . import abi, scenes
-from .sim import (SphSimulation3DGrid, derive_container_geometry, derive_sph_params,
+from .sim import (SphBatch, SphSimulation3DGrid, derive_container_geometry, derive_sph_params,
                   gravity)
 
 
@@ -23,5 +23,22 @@ def make_sim(scene, device: int = 0, max_particles: int = 0) -> SphSimulation3DG
     return sim
 
 
-__all__ = ["abi", "scenes", "SphSimulation3DGrid", "derive_sph_params",
-           "derive_container_geometry", "gravity", "make_sim"]
+def scene_tuple(scene):
+    """(particles, params, container, gravity, bounds_extent) of a Scene, the
+    per-scene description SphBatch takes."""
+    params, cont, g = scenes.make_params(scene, abi, derive_sph_params,
+                                         derive_container_geometry)
+    return scene.particles, params, cont, g, scene.bounds_extent
+
+
+def make_batch(scene_list, device: int = 0) -> SphBatch:
+    """One SphBatch over a list of scenes.Scene (all with the same PCISPH
+    iteration count)."""
+    its = {sc.pcisph_iterations for sc in scene_list}
+    if len(its) != 1:
+        raise ValueError("scenes of a batch share one PCISPH iteration count")
+    return SphBatch([scene_tuple(sc) for sc in scene_list], its.pop(), device=device)
+
+
+__all__ = ["abi", "scenes", "SphSimulation3DGrid", "SphBatch", "derive_sph_params",
+           "derive_container_geometry", "gravity", "make_sim", "make_batch", "scene_tuple"]

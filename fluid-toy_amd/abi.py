@@ -102,6 +102,12 @@ class FluidCreateInfo(C.Structure):
                 ("sdf_resolution", C.c_uint32)]
 
 
+class FluidSceneDesc(C.Structure):
+    _fields_ = [("particles", C.c_void_p), ("num_particles", C.c_uint32),
+                ("sph_params", GpuSphParams3D), ("container", GpuContainerGeometry),
+                ("gravity", GpuGravity), ("bounds_extent", C.c_float)]
+
+
 def default_mouse_force() -> GpuMouseForce:
     """GpuMouseForce::default (src/state/interaction.rs:113-126)."""
     m = GpuMouseForce()
@@ -159,6 +165,12 @@ PROTOTYPES = {
     "fluid_time_stages": (C.c_int, [P, u32, C.POINTER(C.c_float)]),
     "fluid_count_pairs": (C.c_int, [P, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
     "fluid_pair_stats": (C.c_int, [P, C.POINTER(C.c_uint64)]),
+    "fluid_batch_create": (C.c_int, [C.POINTER(FluidSceneDesc), u32, u32, C.c_int, C.POINTER(P)]),
+    "fluid_batch_num_scenes": (u32, [P]),
+    "fluid_batch_scene_info": (C.c_int, [P, u32, C.POINTER(u32), C.POINTER(u32), C.POINTER(u32),
+                                         C.POINTER(GpuGridParams)]),
+    "fluid_batch_update_scene": (C.c_int, [P, u32, C.POINTER(GpuSphParams3D),
+                                           C.POINTER(GpuContainerGeometry), C.POINTER(GpuGravity)]),
     "fluid_set_sweep_config": (C.c_int, [P, C.POINTER(u32)]),
     "fluid_get_sweep_config": (C.c_int, [P, C.POINTER(u32)]),
     "fluid_autotune": (C.c_int, [P, u32, C.POINTER(u32)]),

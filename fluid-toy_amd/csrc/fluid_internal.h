@@ -75,8 +75,9 @@ struct DevUniforms {
   float interact_range;  // h * 0.7                                      :303
   float md2_lo, md2_hi;  // (0.1 h)^2 (1 -/+ 1e-6): r < 0.1h is decided on
                          // r^2 outside this band, on a rounded sqrt inside
-  // GpuGridParams
-  uint32_t gx, gy, gz, cells, gxy;
+  // GpuGridParams (cell_offset: first cell of this grid in a batch's
+  // concatenated cell array, 0 for a single scene)
+  uint32_t gx, gy, gz, cells, gxy, cell_offset;
   float inv_cell, ox, oy, oz;
   // GpuContainerGeometry
   float hw, hh, hd, cy;
@@ -88,6 +89,14 @@ struct DevUniforms {
   uint32_t sdf_res;
   GpuMouseForce mouse;
   GpuRigidBody rb;
+};
+
+// Uniform source of a launch: one block, or (batched handles) one block per
+// scene indexed by the slot's scene.  Scenes keep fixed slot ranges, since
+// each owns a contiguous run of cells in the concatenated grid.
+struct Uni {
+  const DevUniforms *u;
+  const uint16_t *scene_of;  // null for a single scene
 };
 
 struct PredBuf {
@@ -135,34 +144,35 @@ struct SweepCfg {
 // Host-side launchers (sph_kernels.hip).  All enqueue on `st`.
 hipError_t launch_clear(const Bufs &b, uint32_t cells, uint32_t ntiles,
                         hipStream_t st);
-hipError_t launch_build(const DevUniforms *u, const Bufs &b, uint32_t n,
+hipError_t launch_build(const Uni &u, const Bufs &b, uint32_t n,
                         hipStream_t st);
-hipError_t launch_scan(const DevUniforms *u, const Bufs &b, uint32_t cells,
+hipError_t launch_scan(const Uni &u, const Bufs &b, uint32_t cells,
                        hipStream_t st);
-hipError_t launch_reorder(const DevUniforms *u, const Bufs &b, uint32_t n,
+hipError_t launch_reorder(const Uni &u, const Bufs &b, uint32_t n,
                           hipStream_t st);
-hipError_t launch_density(const DevUniforms *u, const Bufs &b, uint32_t n,
+hipError_t launch_density(const Uni &u, const Bufs &b, uint32_t n,
                           SweepCfg c, hipStream_t st);
-hipError_t launch_xsph(const DevUniforms *u, const Bufs &b, uint32_t n,
+hipError_t launch_xsph(const Uni &u, const Bufs &b, uint32_t n,
                        SweepCfg c, hipStream_t st);
-hipError_t launch_force(const DevUniforms *u, const Bufs &b, uint32_t n,
+hipError_t launch_force(const Uni &u, const Bufs &b, uint32_t n,
                         SweepCfg c, hipStream_t st);
-hipError_t launch_predict(const DevUniforms *u, const Bufs &b, uint32_t n,
+hipError_t launch_predict(const Uni &u, const Bufs &b, uint32_t n,
                           hipStream_t st);
-hipError_t launch_solve(const DevUniforms *u, const Bufs &b, uint32_t n,
+hipError_t launch_solve(const Uni &u, const Bufs &b, uint32_t n,
                         int in_buf, SweepCfg c, hipStream_t st);
-hipError_t launch_finalize(const DevUniforms *u, const Bufs &b, uint32_t n,
+hipError_t launch_finalize(const Uni &u, const Bufs &b, uint32_t n,
                            int fin_buf, hipStream_t st);
-hipError_t launch_integrate(const DevUniforms *u, const Bufs &b, uint32_t n,
+hipError_t launch_integrate(const Uni &u, const Bufs &b, uint32_t n,
                             hipStream_t st);
-hipError_t launch_count_pairs(const DevUniforms *u, const Bufs &b, uint32_t n,
+hipError_t launch_count_pairs(const Uni &u, const Bufs &b, uint32_t n,
                               unsigned long long *out4, hipStream_t st);
 
 // Host-side parameter helpers (params.cpp).
 void fill_uniforms(DevUniforms &u, const GpuSphParams3D &p,
                    const GpuGridParams &g, const GpuContainerGeometry &c,
                    const GpuGravity &grav, const GpuMouseForce &mouse,
-                   const GpuRigidBody &rb, uint32_t sdf_res);
+                   const GpuRigidBody &rb, uint32_t sdf_res,
+                   uint32_t cell_offset = 0);
 GpuGridParams make_grid(float h, float bounds_extent, uint32_t n);
 
 }  // namespace fluid

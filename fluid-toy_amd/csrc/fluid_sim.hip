@@ -37,19 +37,28 @@ constexpr int kUniformRing = 16;
 
 }  // namespace
 
-struct fluid_sim {
-  int device = 0;
-  hipStream_t stream = nullptr;
-  uint32_t n = 0, max_n = 0, iters = 5;
+// One scene of a handle: its uniforms, its dense grid's place in the
+// concatenated cell array, and its fixed range of particle slots.
+struct SceneState {
   GpuSphParams3D params{};
   GpuGridParams grid{};
   GpuContainerGeometry cont{};
   GpuGravity grav{};
+  uint32_t cell_offset = 0, first = 0, count = 0;
+};
+
+struct fluid_sim {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  uint32_t n = 0, max_n = 0, iters = 5;
+  std::vector<SceneState> sc;  // one entry for a plain handle
+  uint32_t total_cells = 0;
+  uint16_t *d_scene_of = nullptr;  // slot -> scene (batched handles only)
   GpuMouseForce mouse{};
   GpuRigidBody rb{};
   uint32_t sdf_res = 0;
-  DevUniforms *d_u = nullptr;
-  DevUniforms *h_ring = nullptr;  // pinned staging ring for uniform copies
+  DevUniforms *d_u = nullptr;     // one block per scene
+  DevUniforms *h_ring = nullptr;  // pinned staging ring: kUniformRing x scenes
   hipEvent_t ring_ev[kUniformRing] = {};
   bool ring_used[kUniformRing] = {};
   int ring_pos = 0;
@@ -67,6 +76,9 @@ struct fluid_sim {
   SweepCfg cfg{8, 8, 0};
   bool tuned = false;  // cfg chosen by fluid_set_sweep_lanes / fluid_autotune
   uint32_t ntiles = 0;
+
+  Uni uni() const { return Uni{d_u, d_scene_of}; }
+  bool batched() const { return sc.size() > 1; }
 };
 
 namespace {
@@ -96,10 +108,15 @@ int push_uniforms(fluid_sim *s) {
   const int slot = s->ring_pos;
   s->ring_pos = (s->ring_pos + 1) % kUniformRing;
   if (s->ring_used[slot]) HIP_TRY(hipEventSynchronize(s->ring_ev[slot]));
-  fill_uniforms(s->h_ring[slot], s->params, s->grid, s->cont, s->grav, s->mouse, s->rb,
-                s->sdf_res);
-  HIP_TRY(hipMemcpyAsync(s->d_u, &s->h_ring[slot], sizeof(DevUniforms),
-                         hipMemcpyHostToDevice, s->stream));
+  const size_t S = s->sc.size();
+  DevUniforms *h = s->h_ring + (size_t)slot * S;
+  for (size_t k = 0; k < S; ++k) {
+    const SceneState &c = s->sc[k];
+    fill_uniforms(h[k], c.params, c.grid, c.cont, c.grav, s->mouse, s->rb, s->sdf_res,
+                  c.cell_offset);
+  }
+  HIP_TRY(hipMemcpyAsync(s->d_u, h, S * sizeof(DevUniforms), hipMemcpyHostToDevice,
+                         s->stream));
   HIP_TRY(hipEventRecord(s->ring_ev[slot], s->stream));
   s->ring_used[slot] = true;
   return FLUID_OK;
@@ -111,38 +128,38 @@ int enqueue_stage(fluid_sim *s, uint32_t stage) {
   const Bufs &b = s->b;
   switch (stage) {
     case FLUID_STAGE_CLEAR:
-      HIP_TRY(launch_clear(b, s->grid.total_cells, s->ntiles, st));
+      HIP_TRY(launch_clear(b, s->total_cells, s->ntiles, st));
       break;
     case FLUID_STAGE_BUILD:
-      HIP_TRY(launch_build(s->d_u, b, n, st));
+      HIP_TRY(launch_build(s->uni(), b, n, st));
       break;
     case FLUID_STAGE_SCAN:
-      HIP_TRY(launch_scan(s->d_u, b, s->grid.total_cells, st));
+      HIP_TRY(launch_scan(s->uni(), b, s->total_cells, st));
       break;
     case FLUID_STAGE_REORDER:
-      HIP_TRY(launch_reorder(s->d_u, b, n, st));
+      HIP_TRY(launch_reorder(s->uni(), b, n, st));
       break;
     case FLUID_STAGE_DENSITY:
-      HIP_TRY(launch_density(s->d_u, b, n, s->cfg, st));
+      HIP_TRY(launch_density(s->uni(), b, n, s->cfg, st));
       break;
     case FLUID_STAGE_XSPH:
-      HIP_TRY(launch_xsph(s->d_u, b, n, s->cfg, st));
+      HIP_TRY(launch_xsph(s->uni(), b, n, s->cfg, st));
       break;
     case FLUID_STAGE_FORCE:
-      HIP_TRY(launch_force(s->d_u, b, n, s->cfg, st));
+      HIP_TRY(launch_force(s->uni(), b, n, s->cfg, st));
       break;
     case FLUID_STAGE_PREDICT:
-      HIP_TRY(launch_predict(s->d_u, b, n, st));
+      HIP_TRY(launch_predict(s->uni(), b, n, st));
       break;
     case FLUID_STAGE_SOLVE:
       for (uint32_t it = 0; it < s->iters; ++it)
-        HIP_TRY(launch_solve(s->d_u, b, n, (int)(it % 2), s->cfg, st));
+        HIP_TRY(launch_solve(s->uni(), b, n, (int)(it % 2), s->cfg, st));
       break;
     case FLUID_STAGE_FINALIZE:
-      HIP_TRY(launch_finalize(s->d_u, b, n, (s->iters % 2 == 0) ? 0 : 1, st));
+      HIP_TRY(launch_finalize(s->uni(), b, n, (s->iters % 2 == 0) ? 0 : 1, st));
       break;
     case FLUID_STAGE_INTEGRATE:
-      HIP_TRY(launch_integrate(s->d_u, b, n, st));
+      HIP_TRY(launch_integrate(s->uni(), b, n, st));
       break;
     default:
       return fail(FLUID_ERR_INVALID_ARG, "unknown stage");
@@ -243,57 +260,58 @@ const char *fluid_last_error(void) { return g_last_error.c_str(); }
 
 int fluid_abi_version(void) { return FLUID_ABI_VERSION; }
 
-int fluid_create(const fluid_create_info *info, fluid_sim **out) {
-  if (!info || !out) return fail(FLUID_ERR_INVALID_ARG, "null argument");
-  *out = nullptr;
-  if (info->num_particles && !info->particles)
-    return fail(FLUID_ERR_INVALID_ARG, "particles is null");
-  const GpuSphParams3D &p = info->sph_params;
-  if (!(p.kernel_radius > 0.0f) || !(p.dt > 0.0f))
-    return fail(FLUID_ERR_INVALID_ARG, "kernel_radius and dt must be > 0");
-  const float extent = info->bounds_extent > 0.0f ? info->bounds_extent : 2.0f;
-  if (info->sdf_voxels && info->sdf_resolution == 0)
-    return fail(FLUID_ERR_INVALID_ARG, "sdf_resolution is 0");
+static int check_device(int device) {
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
     return fail(FLUID_ERR_NO_DEVICE, "no HIP device");
-  if (info->device < 0 || info->device >= ndev)
-    return fail(FLUID_ERR_NO_DEVICE, "device ordinal out of range");
+  if (device < 0 || device >= ndev) return fail(FLUID_ERR_NO_DEVICE, "device ordinal out of range");
   hipDeviceProp_t prop;
-  if (hipGetDeviceProperties(&prop, info->device) != hipSuccess)
+  if (hipGetDeviceProperties(&prop, device) != hipSuccess)
     return fail(FLUID_ERR_NO_DEVICE, "hipGetDeviceProperties failed");
   if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
-    return fail(FLUID_ERR_NO_DEVICE, std::string("library is built for gfx950, device is ") +
-                                         prop.gcnArchName);
-  const GpuGridParams grid = make_grid(p.kernel_radius, extent, info->num_particles);
-  const uint64_t cells64 = (uint64_t)grid.grid_size_x * grid.grid_size_y * grid.grid_size_z;
-  if (cells64 > (1ull << 28)) return fail(FLUID_ERR_CAPACITY, "grid exceeds 2^28 cells");
+    return fail(FLUID_ERR_NO_DEVICE,
+                std::string("library is built for gfx950, device is ") + prop.gcnArchName);
+  return FLUID_OK;
+}
+
+// Shared by fluid_create (one scene) and fluid_batch_create (many): the
+// scenes' grids are laid end to end in one cell array and their particles
+// (given concatenated, scene-major) end to end in one slot array.
+static int create_handle(int device, std::vector<SceneState> scenes,
+                         const SphParticle3D *particles, uint32_t n, uint32_t max_n,
+                         const float *sdf, uint32_t sdf_res, uint32_t iters, fluid_sim **out) {
+  int rc = check_device(device);
+  if (rc) return rc;
+  uint64_t cells = 0;
+  for (SceneState &c : scenes) {
+    c.cell_offset = (uint32_t)cells;
+    cells += (uint64_t)c.grid.grid_size_x * c.grid.grid_size_y * c.grid.grid_size_z;
+  }
+  if (cells > (1ull << 28)) return fail(FLUID_ERR_CAPACITY, "grid exceeds 2^28 cells");
 
   fluid_sim *s = new fluid_sim();
-  s->device = info->device;
-  s->n = info->num_particles;
-  s->max_n = std::max(info->max_particles, info->num_particles);
-  s->params = p;
-  s->params.num_particles = s->n;
-  s->grid = grid;
-  s->cont = info->container;
-  s->grav = GpuGravity{{0.0f, -9.8f, 0.0f}, 0.0f};  // sph_3d_grid.rs:277-280
+  s->device = device;
+  s->n = n;
+  s->max_n = std::max(max_n, n);
+  s->sc = std::move(scenes);
+  s->total_cells = (uint32_t)cells;
   s->mouse = GpuMouseForce{{0.f, 0.f, 0.f}, 0.5f, 30.0f, 0u, 0u, 0.f, {0.f, 0.f, -1.0f}, 0.f};
   s->rb = GpuRigidBody{{0.f, 0.f, 0.f}, 0.15f, {0.f, 0.f, 0.f}, 0u, 200.0f, 0u, 0.f, 0.f,
                        {1.f, 0.f, 0.f, 0.f}, {0.f, 1.f, 0.f, 0.f}, {0.f, 0.f, 1.f, 0.f}};
-  s->iters = 5;  // sph_3d_grid.rs:1253
+  s->iters = std::max(iters, 1u);
   s->cfg = pick_cfg(s->n);
-  s->ntiles = (grid.total_cells + kScanTile - 1) / kScanTile;
+  s->ntiles = (s->total_cells + kScanTile - 1) / kScanTile;
+  const size_t S = s->sc.size();
 
-  auto bail = [&](int rc) {
+  auto bail = [&](int code) {
     fluid_destroy(s);
-    return rc;
+    return code;
   };
   if (hipSetDevice(s->device) != hipSuccess) return bail(fail(FLUID_ERR_HIP, "hipSetDevice"));
   if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess)
     return bail(fail(FLUID_ERR_HIP, "hipStreamCreate"));
   hipError_t e = hipSuccess;
-  const size_t N = s->max_n, C = grid.total_cells;
+  const size_t N = s->max_n, C = s->total_cells;
 #define ALLOC(field, type, count)                                                 \
   s->b.field = (type)dev_alloc(s, (size_t)(count) * sizeof(*s->b.field), &e);     \
   if (e != hipSuccess) return bail(fail(FLUID_ERR_HIP, std::string("hipMalloc " #field ": ") + hipGetErrorString(e)));
@@ -305,7 +323,7 @@ int fluid_create(const fluid_create_info *info, fluid_sim **out) {
   ALLOC(sfrc, float4 *, N);
   ALLOC(srec, float4 *, 3 * N);
   ALLOC(nbr, uint32_t *, (size_t)kListCapacity * N);
-  ALLOC(ncnt, uint32_t *, 8 * N);
+  ALLOC(ncnt, uint32_t *, N);
   ALLOC(sid, uint32_t *, N);
   ALLOC(counts, uint32_t *, C);
   ALLOC(starts, uint32_t *, C);
@@ -324,30 +342,129 @@ int fluid_create(const fluid_create_info *info, fluid_sim **out) {
 #undef ALLOC
   s->d_pairs = (unsigned long long *)dev_alloc(s, 32, &e);
   if (e != hipSuccess) return bail(fail(FLUID_ERR_HIP, "hipMalloc pairs"));
-  s->d_u = (DevUniforms *)dev_alloc(s, sizeof(DevUniforms), &e);
+  s->d_u = (DevUniforms *)dev_alloc(s, S * sizeof(DevUniforms), &e);
   if (e != hipSuccess) return bail(fail(FLUID_ERR_HIP, "hipMalloc uniforms"));
-  if (info->sdf_voxels) {
-    const size_t nv = (size_t)info->sdf_resolution * info->sdf_resolution * info->sdf_resolution;
+  if (S > 1) {
+    std::vector<uint16_t> scene_of(N, 0);
+    for (size_t k = 0; k < S; ++k)
+      for (uint32_t i = 0; i < s->sc[k].count; ++i) scene_of[s->sc[k].first + i] = (uint16_t)k;
+    s->d_scene_of = (uint16_t *)dev_alloc(s, N * 2, &e);
+    if (e != hipSuccess) return bail(fail(FLUID_ERR_HIP, "hipMalloc scene map"));
+    if (hipMemcpy(s->d_scene_of, scene_of.data(), N * 2, hipMemcpyHostToDevice) != hipSuccess)
+      return bail(fail(FLUID_ERR_HIP, "scene map upload"));
+  }
+  if (sdf) {
+    const size_t nv = (size_t)sdf_res * sdf_res * sdf_res;
     float *d = (float *)dev_alloc(s, nv * 4, &e);
     if (e != hipSuccess) return bail(fail(FLUID_ERR_HIP, "hipMalloc sdf"));
-    if (hipMemcpy(d, info->sdf_voxels, nv * 4, hipMemcpyHostToDevice) != hipSuccess)
+    if (hipMemcpy(d, sdf, nv * 4, hipMemcpyHostToDevice) != hipSuccess)
       return bail(fail(FLUID_ERR_HIP, "sdf upload"));
     s->b.sdf = d;
-    s->sdf_res = info->sdf_resolution;
+    s->sdf_res = sdf_res;
   }
-  if (hipHostMalloc((void **)&s->h_ring, sizeof(DevUniforms) * kUniformRing) != hipSuccess)
+  if (hipHostMalloc((void **)&s->h_ring, sizeof(DevUniforms) * kUniformRing * S) != hipSuccess)
     return bail(fail(FLUID_ERR_HIP, "hipHostMalloc"));
   for (int k = 0; k < kUniformRing; ++k)
     if (hipEventCreateWithFlags(&s->ring_ev[k], hipEventDisableTiming) != hipSuccess)
       return bail(fail(FLUID_ERR_HIP, "hipEventCreate"));
   if (hipDeviceSynchronize() != hipSuccess) return bail(fail(FLUID_ERR_HIP, "memset"));
-  int rc = upload_canonical(s, info->particles, s->n);
+  rc = upload_canonical(s, particles, s->n);
   if (rc) return bail(rc);
   rc = push_uniforms(s);
   if (rc) return bail(rc);
   if (hipStreamSynchronize(s->stream) != hipSuccess) return bail(fail(FLUID_ERR_HIP, "sync"));
   *out = s;
   return FLUID_OK;
+}
+
+int fluid_create(const fluid_create_info *info, fluid_sim **out) {
+  if (!info || !out) return fail(FLUID_ERR_INVALID_ARG, "null argument");
+  *out = nullptr;
+  if (info->num_particles && !info->particles)
+    return fail(FLUID_ERR_INVALID_ARG, "particles is null");
+  const GpuSphParams3D &p = info->sph_params;
+  if (!(p.kernel_radius > 0.0f) || !(p.dt > 0.0f))
+    return fail(FLUID_ERR_INVALID_ARG, "kernel_radius and dt must be > 0");
+  if (info->sdf_voxels && info->sdf_resolution == 0)
+    return fail(FLUID_ERR_INVALID_ARG, "sdf_resolution is 0");
+  const float extent = info->bounds_extent > 0.0f ? info->bounds_extent : 2.0f;
+  SceneState c;
+  c.params = p;
+  c.params.num_particles = info->num_particles;
+  c.grid = make_grid(p.kernel_radius, extent, info->num_particles);
+  c.cont = info->container;
+  c.grav = GpuGravity{{0.0f, -9.8f, 0.0f}, 0.0f};  // sph_3d_grid.rs:277-280
+  c.first = 0;
+  c.count = info->num_particles;
+  return create_handle(info->device, {c}, info->particles, info->num_particles,
+                       info->max_particles, info->sdf_voxels, info->sdf_resolution,
+                       5 /* sph_3d_grid.rs:1253 */, out);
+}
+
+int fluid_batch_create(const fluid_scene_desc *scenes, uint32_t num_scenes,
+                       uint32_t pcisph_iterations, int device, fluid_sim **out) {
+  if (!scenes || !out || num_scenes == 0) return fail(FLUID_ERR_INVALID_ARG, "null argument");
+  *out = nullptr;
+  if (num_scenes > 65535) return fail(FLUID_ERR_CAPACITY, "at most 65535 scenes");
+  std::vector<SceneState> st(num_scenes);
+  uint64_t total = 0;
+  for (uint32_t k = 0; k < num_scenes; ++k) {
+    const fluid_scene_desc &d = scenes[k];
+    if (d.num_particles && !d.particles)
+      return fail(FLUID_ERR_INVALID_ARG, "scene particles is null");
+    if (!(d.sph_params.kernel_radius > 0.0f) || !(d.sph_params.dt > 0.0f))
+      return fail(FLUID_ERR_INVALID_ARG, "kernel_radius and dt must be > 0");
+    const float extent = d.bounds_extent > 0.0f ? d.bounds_extent : 2.0f;
+    st[k].params = d.sph_params;
+    st[k].params.num_particles = d.num_particles;
+    st[k].grid = make_grid(d.sph_params.kernel_radius, extent, d.num_particles);
+    st[k].cont = d.container;
+    st[k].grav = d.gravity;
+    st[k].first = (uint32_t)total;
+    st[k].count = d.num_particles;
+    total += d.num_particles;
+  }
+  if (total > 0xffffffffull) return fail(FLUID_ERR_CAPACITY, "too many particles");
+  std::vector<SphParticle3D> all((size_t)total);
+  for (uint32_t k = 0; k < num_scenes; ++k)
+    if (scenes[k].num_particles)
+      std::memcpy(all.data() + st[k].first, scenes[k].particles,
+                  sizeof(SphParticle3D) * scenes[k].num_particles);
+  return create_handle(device, std::move(st), all.data(), (uint32_t)total, (uint32_t)total,
+                       nullptr, 0, pcisph_iterations, out);
+}
+
+uint32_t fluid_batch_num_scenes(const fluid_sim *s) { return s ? (uint32_t)s->sc.size() : 0; }
+
+int fluid_batch_scene_info(const fluid_sim *s, uint32_t scene, uint32_t *first_particle,
+                           uint32_t *num_particles, uint32_t *first_cell,
+                           GpuGridParams *grid) {
+  if (!s || scene >= s->sc.size()) return fail(FLUID_ERR_INVALID_ARG, "bad scene");
+  const SceneState &c = s->sc[scene];
+  if (first_particle) *first_particle = c.first;
+  if (num_particles) *num_particles = c.count;
+  if (first_cell) *first_cell = c.cell_offset;
+  if (grid) *grid = c.grid;
+  return FLUID_OK;
+}
+
+int fluid_batch_update_scene(fluid_sim *s, uint32_t scene, const GpuSphParams3D *params,
+                             const GpuContainerGeometry *container, const GpuGravity *gravity) {
+  if (!s || scene >= s->sc.size()) return fail(FLUID_ERR_INVALID_ARG, "bad scene");
+  SceneState &c = s->sc[scene];
+  if (params) {
+    if (!(params->kernel_radius > 0.0f) || !(params->dt > 0.0f))
+      return fail(FLUID_ERR_INVALID_ARG, "kernel_radius and dt must be > 0");
+    if (params->kernel_radius != c.grid.cell_size)
+      return fail(FLUID_ERR_INVALID_ARG,
+                  "kernel_radius differs from the grid cell size fixed at creation");
+    c.params = *params;
+    c.params.num_particles = c.count;
+  }
+  if (container) c.cont = *container;
+  if (gravity) c.grav = *gravity;
+  HIP_TRY(hipSetDevice(s->device));
+  return push_uniforms(s);
 }
 
 int fluid_destroy(fluid_sim *s) {
@@ -395,19 +512,21 @@ int fluid_update_sph_params(fluid_sim *s, const GpuSphParams3D *p) {
   if (!s || !p) return fail(FLUID_ERR_INVALID_ARG, "null argument");
   if (!(p->kernel_radius > 0.0f) || !(p->dt > 0.0f))
     return fail(FLUID_ERR_INVALID_ARG, "kernel_radius and dt must be > 0");
-  if (p->kernel_radius != s->grid.cell_size)
+  if (s->batched()) return fail(FLUID_ERR_STATE, "batched handle: use fluid_batch_update_scene");
+  if (p->kernel_radius != s->sc[0].grid.cell_size)
     return fail(FLUID_ERR_INVALID_ARG,
                 "kernel_radius differs from the grid cell size fixed at creation");
   HIP_TRY(hipSetDevice(s->device));
-  s->params = *p;
-  s->params.num_particles = s->n;
+  s->sc[0].params = *p;
+  s->sc[0].params.num_particles = s->n;
   return push_uniforms(s);
 }
 
 int fluid_update_container_geometry(fluid_sim *s, const GpuContainerGeometry *g) {
   if (!s || !g) return fail(FLUID_ERR_INVALID_ARG, "null argument");
   HIP_TRY(hipSetDevice(s->device));
-  s->cont = *g;
+  if (s->batched()) return fail(FLUID_ERR_STATE, "batched handle: use fluid_batch_update_scene");
+  s->sc[0].cont = *g;
   return push_uniforms(s);
 }
 
@@ -421,7 +540,8 @@ int fluid_update_mouse_force(fluid_sim *s, const GpuMouseForce *m) {
 int fluid_update_gravity(fluid_sim *s, const GpuGravity *g) {
   if (!s || !g) return fail(FLUID_ERR_INVALID_ARG, "null argument");
   HIP_TRY(hipSetDevice(s->device));
-  s->grav = *g;
+  if (s->batched()) return fail(FLUID_ERR_STATE, "batched handle: use fluid_batch_update_scene");
+  s->sc[0].grav = *g;
   return push_uniforms(s);
 }
 
@@ -461,6 +581,7 @@ int fluid_spawn_particles(fluid_sim *s, const float position[3], uint32_t count,
   if (!s || !position) return fail(FLUID_ERR_INVALID_ARG, "null argument");
   if (spawned) *spawned = 0;
   if (s->cursor != -1) return fail(FLUID_ERR_STATE, "a partial step is in progress");
+  if (s->batched()) return fail(FLUID_ERR_STATE, "batched handle: scenes have fixed counts");
   const uint32_t avail = s->max_n - s->n;
   const uint32_t c = std::min(count, avail);
   if (c == 0) return FLUID_OK;
@@ -484,8 +605,9 @@ int fluid_spawn_particles(fluid_sim *s, const float position[3], uint32_t count,
   HIP_TRY(hipStreamSynchronize(s->stream));
   s->extra.resize((size_t)(s->n + c) * 5, 0.0f);
   s->n += c;
-  s->grid.num_particles = s->n;
-  s->params.num_particles = s->n;
+  s->sc[0].count = s->n;
+  s->sc[0].grid.num_particles = s->n;
+  s->sc[0].params.num_particles = s->n;
   update_cfg(s);
   if (spawned) *spawned = c;
   return push_uniforms(s);
@@ -531,13 +653,16 @@ int fluid_write_particles(fluid_sim *s, const SphParticle3D *in, uint32_t n) {
   if (!s || (!in && n)) return fail(FLUID_ERR_INVALID_ARG, "null argument");
   if (n > s->max_n) return fail(FLUID_ERR_CAPACITY, "more particles than max_particles");
   if (s->cursor != -1) return fail(FLUID_ERR_STATE, "a partial step is in progress");
+  if (s->batched() && n != s->n)
+    return fail(FLUID_ERR_STATE, "batched handle: write all scenes' particles at once");
   HIP_TRY(hipSetDevice(s->device));
   HIP_TRY(hipStreamSynchronize(s->stream));
   const int rc = upload_canonical(s, in, n);
   if (rc) return rc;
   s->n = n;
-  s->grid.num_particles = n;
-  s->params.num_particles = n;
+  s->sc[0].count = n;
+  s->sc[0].grid.num_particles = n;
+  s->sc[0].params.num_particles = n;
   update_cfg(s);
   return push_uniforms(s);
 }
@@ -580,12 +705,12 @@ static int read_u32(fluid_sim *s, const uint32_t *src, size_t count, uint32_t *o
 
 int fluid_read_cell_counts(fluid_sim *s, uint32_t *out, uint32_t cap) {
   if (!s) return fail(FLUID_ERR_INVALID_ARG, "null handle");
-  return read_u32(s, s->b.counts, s->grid.total_cells, out, cap);
+  return read_u32(s, s->b.counts, s->total_cells, out, cap);
 }
 
 int fluid_read_cell_starts(fluid_sim *s, uint32_t *out, uint32_t cap) {
   if (!s) return fail(FLUID_ERR_INVALID_ARG, "null handle");
-  return read_u32(s, s->b.starts, s->grid.total_cells, out, cap);
+  return read_u32(s, s->b.starts, s->total_cells, out, cap);
 }
 
 int fluid_read_sorted_to_orig(fluid_sim *s, uint32_t *out, uint32_t cap) {
@@ -629,19 +754,19 @@ int fluid_read_predicted(fluid_sim *s, uint32_t which, PredictedState *out, uint
 
 int fluid_read_grid_params(const fluid_sim *s, GpuGridParams *out) {
   if (!s || !out) return fail(FLUID_ERR_INVALID_ARG, "null argument");
-  *out = s->grid;
+  *out = s->sc[0].grid;
   return FLUID_OK;
 }
 
 int fluid_read_sph_params(const fluid_sim *s, GpuSphParams3D *out) {
   if (!s || !out) return fail(FLUID_ERR_INVALID_ARG, "null argument");
-  *out = s->params;
+  *out = s->sc[0].params;
   return FLUID_OK;
 }
 
 int fluid_read_container_geometry(const fluid_sim *s, GpuContainerGeometry *out) {
   if (!s || !out) return fail(FLUID_ERR_INVALID_ARG, "null argument");
-  *out = s->cont;
+  *out = s->sc[0].cont;
   return FLUID_OK;
 }
 
@@ -809,7 +934,7 @@ int fluid_pair_stats(fluid_sim *s, uint64_t out[4]) {
   if (!s || !out) return fail(FLUID_ERR_INVALID_ARG, "null argument");
   HIP_TRY(hipSetDevice(s->device));
   HIP_TRY(hipMemsetAsync(s->d_pairs, 0, 32, s->stream));
-  HIP_TRY(launch_count_pairs(s->d_u, s->b, s->n, s->d_pairs, s->stream));
+  HIP_TRY(launch_count_pairs(s->uni(), s->b, s->n, s->d_pairs, s->stream));
   unsigned long long h[4] = {0, 0, 0, 0};
   HIP_TRY(hipMemcpyAsync(h, s->d_pairs, 32, hipMemcpyDeviceToHost, s->stream));
   HIP_TRY(hipStreamSynchronize(s->stream));

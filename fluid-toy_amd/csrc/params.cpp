@@ -97,8 +97,9 @@ GpuGridParams make_grid(float h, float bounds_extent, uint32_t n) {
 void fill_uniforms(DevUniforms &u, const GpuSphParams3D &p,
                    const GpuGridParams &g, const GpuContainerGeometry &c,
                    const GpuGravity &grav, const GpuMouseForce &mouse,
-                   const GpuRigidBody &rb, uint32_t sdf_res) {
+                   const GpuRigidBody &rb, uint32_t sdf_res, uint32_t cell_offset) {
   std::memset(&u, 0, sizeof(u));
+  u.cell_offset = cell_offset;
   u.h = p.kernel_radius;
   u.h2 = p.kernel_radius_sq;
   u.h6 = p.kernel_radius_pow6;

@@ -63,6 +63,13 @@ __device__ __forceinline__ float group_sum(float v) {
   return v;
 }
 
+// The uniform block of slot s: the handle's only block, or its scene's.
+template <bool B>
+__device__ __forceinline__ const DevUniforms *uni_at(const Uni &un, uint32_t s) {
+  if constexpr (B) return un.u + un.scene_of[s];
+  else return un.u;
+}
+
 // Workgroup index remapped so that each XCD (workgroups are dealt to the 8
 // XCDs round-robin) receives one contiguous run of tiles: neighbouring
 // particles then share that XCD's L2.
@@ -100,7 +107,7 @@ __device__ __forceinline__ void sweep27(const DevUniforms *__restrict__ U,
   const int y0 = max(cy - 1, 0), y1 = min(cy + 1, gy - 1);
   for (int z = z0; z <= z1; ++z) {
     for (int y = y0; y <= y1; ++y) {
-      const uint32_t row = (uint32_t)y * U->gx + (uint32_t)z * U->gxy;
+      const uint32_t row = U->cell_offset + (uint32_t)y * U->gx + (uint32_t)z * U->gxy;
       const uint32_t c0 = row + (uint32_t)x0, c1 = row + (uint32_t)x1;
       const uint32_t begin = starts[c0] - counts[c0];
       const uint32_t end = starts[c1];
@@ -154,17 +161,18 @@ __global__ __launch_bounds__(256) void k_clear(uint32_t *__restrict__ counts,
 // same cell in a run (slots are spatially sorted from the previous step)
 // merge into one atomic; the returned base + rank in the run is the
 // particle's provisional in-cell slot, used only to scatter before ranking.
-__global__ __launch_bounds__(256) void k_build(const DevUniforms *__restrict__ U,
-                                               Bufs b, uint32_t n) {
+template <bool B>
+__global__ __launch_bounds__(256) void k_build(Uni un, Bufs b, uint32_t n) {
   const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
   const bool valid = t < n;
   uint32_t ci = 0xffffffffu;
   if (valid) {
+    const DevUniforms *__restrict__ U = uni_at<B>(un, t);
     const float4 p = b.pos[t];
     const int cx = min(max(cell_coord(p.x, U->ox, U->inv_cell), 0), (int)U->gx - 1);
     const int cy = min(max(cell_coord(p.y, U->oy, U->inv_cell), 0), (int)U->gy - 1);
     const int cz = min(max(cell_coord(p.z, U->oz, U->inv_cell), 0), (int)U->gz - 1);
-    ci = (uint32_t)cx + (uint32_t)cy * U->gx + (uint32_t)cz * U->gxy;
+    ci = U->cell_offset + (uint32_t)cx + (uint32_t)cy * U->gx + (uint32_t)cz * U->gxy;
     b.cell_of[t] = ci;
   }
   const int lane = threadIdx.x & 63;
@@ -327,13 +335,13 @@ __global__ __launch_bounds__(256) void k_rank_gather(Bufs b, uint32_t n) {
 // in-range neighbours it met (0 < r^2 < h^2, the pairs XSPH and force sum
 // over) in its list, and the particle's neighbour record is written for the
 // later gathers.
-template <int L>
-__global__ __launch_bounds__(256) void k_density(const DevUniforms *__restrict__ U,
-                                                 Bufs b, uint32_t n, uint32_t xcd) {
+template <int L, bool B>
+__global__ __launch_bounds__(256) void k_density(Uni un, Bufs b, uint32_t n, uint32_t xcd) {
   constexpr uint32_t kCap = kListCapacity / L;
   const uint32_t g = xcd_block(xcd) * blockDim.x + threadIdx.x;
   const uint32_t s = g / L, sub = g % L;
   if (s >= n) return;
+  const DevUniforms *__restrict__ U = uni_at<B>(un, s);
   const float4 pi = b.spos[s];
   const int cx = cell_coord(pi.x, U->ox, U->inv_cell);
   const int cy = cell_coord(pi.y, U->oy, U->inv_cell);
@@ -480,12 +488,12 @@ __device__ __forceinline__ void for_each_neighbour(const DevUniforms *__restrict
 // Stage 5 — xsph.wgsl:81-138.  Writes the canonical velocity; with
 // epsilon <= 0 the reference leaves it untouched, i.e. equal to the sorted
 // copy, which is what is written then.
-template <int L>
-__global__ __launch_bounds__(256) void k_xsph(const DevUniforms *__restrict__ U,
-                                              Bufs b, uint32_t n, uint32_t xcd) {
+template <int L, bool B>
+__global__ __launch_bounds__(256) void k_xsph(Uni un, Bufs b, uint32_t n, uint32_t xcd) {
   const uint32_t g = xcd_block(xcd) * blockDim.x + threadIdx.x;
   const uint32_t s = g / L, sub = g % L;
   if (s >= n) return;
+  const DevUniforms *__restrict__ U = uni_at<B>(un, s);
   const float eps = U->eps;
   const float4 vi = b.svel[s];
   if (!(eps > 0.0f)) {
@@ -521,12 +529,12 @@ __global__ __launch_bounds__(256) void k_xsph(const DevUniforms *__restrict__ U,
 // (sp = -m/(2 rho_i) (P_i/rho^n_j + k_near), the viscosity and K_ij share
 // 1/(rho_i + rho_j)); r < 0.1 h, which switches the direction, is decided on
 // the exact bits the oracle uses.
-template <int L>
-__global__ __launch_bounds__(256) void k_force(const DevUniforms *__restrict__ U,
-                                               Bufs b, uint32_t n, uint32_t xcd) {
+template <int L, bool B>
+__global__ __launch_bounds__(256) void k_force(Uni un, Bufs b, uint32_t n, uint32_t xcd) {
   const uint32_t g = xcd_block(xcd) * blockDim.x + threadIdx.x;
   const uint32_t s = g / L, sub = g % L;
   if (s >= n) return;
+  const DevUniforms *__restrict__ U = uni_at<B>(un, s);
   const float4 *rec_i = b.srec + 3 * (size_t)s;
   const float4 pi = rec_i[0];
   const float4 ni = rec_i[2];
@@ -597,10 +605,11 @@ __global__ __launch_bounds__(256) void k_force(const DevUniforms *__restrict__ U
 // ------------------------------------------------------------------------
 // Stage 7 — pcisph_predict.wgsl:50-73, run per sorted slot (the canonical
 // state already lives in slot order).
-__global__ __launch_bounds__(256) void k_predict(const DevUniforms *__restrict__ U,
-                                                 Bufs b, uint32_t n) {
+template <bool B>
+__global__ __launch_bounds__(256) void k_predict(Uni un, Bufs b, uint32_t n) {
   const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= n) return;
+  const DevUniforms *__restrict__ U = uni_at<B>(un, s);
   const float dt = U->dt;
   const float4 p = b.spos[s], v = b.vel[s], a = b.frc[s];
   const float vx = v.x + dt * a.x, vy = v.y + dt * a.y, vz = v.z + dt * a.z;
@@ -613,13 +622,13 @@ __global__ __launch_bounds__(256) void k_predict(const DevUniforms *__restrict__
 // Stage 8 — pcisph_solve.wgsl:172-208, one iteration, in -> out.  The
 // neighbour's p / max(rho*,1)^2 is precomputed by the producing iteration
 // (same operations as the reference evaluates per pair).
-template <int L>
-__global__ __launch_bounds__(256) void k_solve(const DevUniforms *__restrict__ U,
-                                               Bufs b, uint32_t n, PredBuf in,
+template <int L, bool B>
+__global__ __launch_bounds__(256) void k_solve(Uni un, Bufs b, uint32_t n, PredBuf in,
                                                PredBuf out, uint32_t xcd) {
   const uint32_t g = xcd_block(xcd) * blockDim.x + threadIdx.x;
   const uint32_t s = g / L, sub = g % L;
   if (s >= n) return;
+  const DevUniforms *__restrict__ U = uni_at<B>(un, s);
   const float4 xi = in.xq[s];
   const float4 p0 = b.spos[s];  // current position (grid lookup)
   const int cx = cell_coord(p0.x, U->ox, U->inv_cell);
@@ -715,10 +724,11 @@ __device__ float sdf_sample(const float *sdf, uint32_t res, float ux, float uy,
   return y0v + (y1v - y0v) * c;
 }
 
-__global__ __launch_bounds__(256) void k_integrate(const DevUniforms *__restrict__ U,
-                                                   Bufs b, uint32_t n) {
+template <bool B>
+__global__ __launch_bounds__(256) void k_integrate(Uni un, Bufs b, uint32_t n) {
   const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= n) return;
+  const DevUniforms *__restrict__ U = uni_at<B>(un, s);
   const float4 p4 = b.spos[s], v4 = b.vel[s];
   float px = p4.x, py = p4.y, pz = p4.z, vx = v4.x, vy = v4.y, vz = v4.z;
   const float(*iv)[4] = U->iv;
@@ -895,12 +905,13 @@ __global__ __launch_bounds__(256) void k_integrate(const DevUniforms *__restrict
 
 // Candidate / in-range pair totals (and per-particle maxima) of the density
 // stencil: out4 = {sum candidates, sum in range, max candidates, max in range}.
-__global__ __launch_bounds__(256) void k_count_pairs(const DevUniforms *__restrict__ U,
-                                                     Bufs b, uint32_t n,
+template <bool B>
+__global__ __launch_bounds__(256) void k_count_pairs(Uni un, Bufs b, uint32_t n,
                                                      unsigned long long *out4) {
   const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
   unsigned long long cand = 0, inr = 0;
   if (s < n) {
+    const DevUniforms *__restrict__ U = uni_at<B>(un, s);
     const float4 pi = b.spos[s];
     const int cx = cell_coord(pi.x, U->ox, U->inv_cell);
     const int cy = cell_coord(pi.y, U->oy, U->inv_cell);
@@ -941,73 +952,91 @@ hipError_t launch_clear(const Bufs &b, uint32_t cells, uint32_t ntiles, hipStrea
   return hipGetLastError();
 }
 
-hipError_t launch_build(const DevUniforms *u, const Bufs &b, uint32_t n, hipStream_t st) {
+#define FLUID_LAUNCH_B(KERNEL, GRID, UN, ...)                                             \
+  do {                                                                                    \
+    if ((UN).scene_of)                                                                    \
+      hipLaunchKernelGGL(KERNEL<true>, dim3(GRID), dim3(256), 0, st, UN, __VA_ARGS__);    \
+    else                                                                                  \
+      hipLaunchKernelGGL(KERNEL<false>, dim3(GRID), dim3(256), 0, st, UN, __VA_ARGS__);   \
+  } while (0)
+
+#define FLUID_CASE_L(KERNEL, LL, BB, GRID, UN, ...) \
+  case LL: hipLaunchKernelGGL((KERNEL<LL, BB>), dim3(GRID), dim3(256), 0, st, UN, __VA_ARGS__); break;
+
+#define FLUID_DISPATCH_L(KERNEL, LANES, N, UN, ...)                                        \
+  do {                                                                                     \
+    const uint32_t _grid = blocks_for((uint64_t)(N) * (LANES), 256);                       \
+    if ((UN).scene_of) {                                                                   \
+      switch (LANES) {                                                                     \
+        FLUID_CASE_L(KERNEL, 1, true, _grid, UN, __VA_ARGS__)                              \
+        FLUID_CASE_L(KERNEL, 2, true, _grid, UN, __VA_ARGS__)                              \
+        FLUID_CASE_L(KERNEL, 4, true, _grid, UN, __VA_ARGS__)                              \
+        default: hipLaunchKernelGGL((KERNEL<8, true>), dim3(_grid), dim3(256), 0, st, UN,  \
+                                    __VA_ARGS__); break;                                   \
+      }                                                                                    \
+    } else {                                                                               \
+      switch (LANES) {                                                                     \
+        FLUID_CASE_L(KERNEL, 1, false, _grid, UN, __VA_ARGS__)                             \
+        FLUID_CASE_L(KERNEL, 2, false, _grid, UN, __VA_ARGS__)                             \
+        FLUID_CASE_L(KERNEL, 4, false, _grid, UN, __VA_ARGS__)                             \
+        default: hipLaunchKernelGGL((KERNEL<8, false>), dim3(_grid), dim3(256), 0, st, UN, \
+                                    __VA_ARGS__); break;                                   \
+      }                                                                                    \
+    }                                                                                      \
+  } while (0)
+
+hipError_t launch_build(const Uni &u, const Bufs &b, uint32_t n, hipStream_t st) {
   if (n == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_build, dim3(blocks_for(n, 256)), dim3(256), 0, st, u, b, n);
+  FLUID_LAUNCH_B(k_build, blocks_for(n, 256), u, b, n);
   return hipGetLastError();
 }
 
-hipError_t launch_scan(const DevUniforms *, const Bufs &b, uint32_t cells, hipStream_t st) {
+hipError_t launch_scan(const Uni &, const Bufs &b, uint32_t cells, hipStream_t st) {
   hipLaunchKernelGGL(k_scan, dim3(blocks_for(cells, kScanTile)), dim3(kScanThreads), 0, st,
                      b, cells);
   return hipGetLastError();
 }
 
-hipError_t launch_reorder(const DevUniforms *, const Bufs &b, uint32_t n, hipStream_t st) {
+hipError_t launch_reorder(const Uni &, const Bufs &b, uint32_t n, hipStream_t st) {
   if (n == 0) return hipSuccess;
   hipLaunchKernelGGL(k_scatter, dim3(blocks_for(n, 256)), dim3(256), 0, st, b, n);
   hipLaunchKernelGGL(k_rank_gather, dim3(blocks_for(n, 256)), dim3(256), 0, st, b, n);
   return hipGetLastError();
 }
 
-#define FLUID_DISPATCH_L(KERNEL, CFG, N, ...)                                         \
-  do {                                                                                \
-    const uint32_t _grid = blocks_for((uint64_t)(N) * (CFG).lanes, 256);              \
-    switch ((CFG).lanes) {                                                            \
-      case 1: hipLaunchKernelGGL(KERNEL<1>, dim3(_grid), dim3(256), 0, st, __VA_ARGS__); break; \
-      case 2: hipLaunchKernelGGL(KERNEL<2>, dim3(_grid), dim3(256), 0, st, __VA_ARGS__); break; \
-      case 4: hipLaunchKernelGGL(KERNEL<4>, dim3(_grid), dim3(256), 0, st, __VA_ARGS__); break; \
-      default: hipLaunchKernelGGL(KERNEL<8>, dim3(_grid), dim3(256), 0, st, __VA_ARGS__); break; \
-    }                                                                                 \
-  } while (0)
-
-hipError_t launch_density(const DevUniforms *u, const Bufs &b, uint32_t n, SweepCfg c,
-                          hipStream_t st) {
+hipError_t launch_density(const Uni &u, const Bufs &b, uint32_t n, SweepCfg c, hipStream_t st) {
   if (n == 0) return hipSuccess;
-  FLUID_DISPATCH_L(k_density, c, n, u, b, n, (uint32_t)c.xcd_tiles);
+  FLUID_DISPATCH_L(k_density, c.lanes, n, u, b, n, (uint32_t)c.xcd_tiles);
   return hipGetLastError();
 }
 
-hipError_t launch_xsph(const DevUniforms *u, const Bufs &b, uint32_t n, SweepCfg c,
-                       hipStream_t st) {
+hipError_t launch_xsph(const Uni &u, const Bufs &b, uint32_t n, SweepCfg c, hipStream_t st) {
   if (n == 0) return hipSuccess;
-  FLUID_DISPATCH_L(k_xsph, c, n, u, b, n, (uint32_t)c.xcd_tiles);
+  FLUID_DISPATCH_L(k_xsph, c.lanes, n, u, b, n, (uint32_t)c.xcd_tiles);
   return hipGetLastError();
 }
 
-hipError_t launch_force(const DevUniforms *u, const Bufs &b, uint32_t n, SweepCfg c,
+hipError_t launch_force(const Uni &u, const Bufs &b, uint32_t n, SweepCfg c, hipStream_t st) {
+  if (n == 0) return hipSuccess;
+  FLUID_DISPATCH_L(k_force, c.lanes, n, u, b, n, (uint32_t)c.xcd_tiles);
+  return hipGetLastError();
+}
+
+hipError_t launch_predict(const Uni &u, const Bufs &b, uint32_t n, hipStream_t st) {
+  if (n == 0) return hipSuccess;
+  FLUID_LAUNCH_B(k_predict, blocks_for(n, 256), u, b, n);
+  return hipGetLastError();
+}
+
+hipError_t launch_solve(const Uni &u, const Bufs &b, uint32_t n, int in_buf, SweepCfg c,
                         hipStream_t st) {
   if (n == 0) return hipSuccess;
-  FLUID_DISPATCH_L(k_force, c, n, u, b, n, (uint32_t)c.xcd_tiles);
-  return hipGetLastError();
-}
-
-hipError_t launch_predict(const DevUniforms *u, const Bufs &b, uint32_t n, hipStream_t st) {
-  if (n == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_predict, dim3(blocks_for(n, 256)), dim3(256), 0, st, u, b, n);
-  return hipGetLastError();
-}
-
-hipError_t launch_solve(const DevUniforms *u, const Bufs &b, uint32_t n, int in_buf,
-                        SweepCfg c, hipStream_t st) {
-  if (n == 0) return hipSuccess;
   const PredBuf in = b.pred[in_buf], out = b.pred[1 - in_buf];
-  const SweepCfg cs{c.solve_lanes, c.solve_lanes, c.xcd_tiles};
-  FLUID_DISPATCH_L(k_solve, cs, n, u, b, n, in, out, (uint32_t)c.xcd_tiles);
+  FLUID_DISPATCH_L(k_solve, c.solve_lanes, n, u, b, n, in, out, (uint32_t)c.xcd_tiles);
   return hipGetLastError();
 }
 
-hipError_t launch_finalize(const DevUniforms *, const Bufs &b, uint32_t n, int fin_buf,
+hipError_t launch_finalize(const Uni &, const Bufs &b, uint32_t n, int fin_buf,
                            hipStream_t st) {
   if (n == 0) return hipSuccess;
   hipLaunchKernelGGL(k_finalize, dim3(blocks_for(n, 256)), dim3(256), 0, st, b, n,
@@ -1015,17 +1044,16 @@ hipError_t launch_finalize(const DevUniforms *, const Bufs &b, uint32_t n, int f
   return hipGetLastError();
 }
 
-hipError_t launch_integrate(const DevUniforms *u, const Bufs &b, uint32_t n, hipStream_t st) {
+hipError_t launch_integrate(const Uni &u, const Bufs &b, uint32_t n, hipStream_t st) {
   if (n == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_integrate, dim3(blocks_for(n, 256)), dim3(256), 0, st, u, b, n);
+  FLUID_LAUNCH_B(k_integrate, blocks_for(n, 256), u, b, n);
   return hipGetLastError();
 }
 
-hipError_t launch_count_pairs(const DevUniforms *u, const Bufs &b, uint32_t n,
+hipError_t launch_count_pairs(const Uni &u, const Bufs &b, uint32_t n,
                               unsigned long long *out4, hipStream_t st) {
   if (n == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_count_pairs, dim3(blocks_for(n, 256)), dim3(256), 0, st, u, b, n,
-                     out4);
+  FLUID_LAUNCH_B(k_count_pairs, blocks_for(n, 256), u, b, n, out4);
   return hipGetLastError();
 }
 

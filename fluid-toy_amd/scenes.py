@@ -154,6 +154,39 @@ def small_block(nx: int, ny: int, nz: int, seed: int = 7, box=(1.6, 1.2, 0.8)) -
     return Scene(f"block_{nx}x{ny}x{nz}", particles_from_positions(pos), box, 0.0)
 
 
+def batch_scenes(count: int = 256, seed: int = 6, n_min: int = 2048,
+                 n_max: int = 65536) -> list:
+    """Independent dam-break scenes of uneven size: particle counts
+    log-uniform in [n_min, n_max], box edges uniform in [0.8, 2.0] per axis,
+    and h, rest density, near stiffness, viscosity, surface tension, wall
+    stiffness and XSPH epsilon each uniform within +-30% of the defaults.
+    The block fills the box floor layer by layer at spacing 0.3125 h (the
+    default 0.025 / 0.08 ratio), closer where the box would be more than 60%
+    full.  The box is centred on the origin so the scene's cubic grid extent
+    is its largest half edge plus one cell."""
+    u = uniform01(seed, count * 12).reshape(count, 12)
+    out = []
+    for k in range(count):
+        r = u[k]
+        n = int(round(n_min * (n_max / n_min) ** r[0]))
+        box = tuple(float(0.8 + 1.2 * r[i]) for i in (1, 2, 3))
+        f = [0.7 + 0.6 * r[i] for i in range(4, 11)]
+        h = 0.08 * f[0]
+        s = min(0.3125 * h, (0.6 * box[0] * box[1] * box[2] / n) ** (1.0 / 3.0))
+        nx, nz = max(1, int(box[0] / s) - 1), max(1, int(box[2] / s) - 1)
+        ny = -(-n // (nx * nz))
+        pts = lattice(nx, ny, nz, s, count=n)
+        pos = pts + (uniform01(seed * 1000 + k, pts.size).reshape(pts.shape) * 2 - 1) * 0.08 * s
+        pos = pos + np.array([-box[0] / 2, -box[1] / 2, -box[2] / 2])
+        sc = Scene(f"batch{k}", particles_from_positions(pos), box, -box[1] / 2, h=h,
+                   rest_density=8000.0 * f[1], near_stiffness=0.45 * f[2],
+                   viscosity=15.0 * f[3], surface_tension=6.5 * f[4],
+                   wall_stiffness=250.0 * f[5], xsph_epsilon=0.305 * f[6],
+                   bounds_extent=max(box) / 2 + h, steps=200)
+        out.append(sc)
+    return out
+
+
 SCENES = {
     "dam_break_8k": dam_break_8k,
     "dam_break_50k": dam_break_50k,

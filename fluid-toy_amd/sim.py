@@ -255,3 +255,59 @@ class SphSimulation3DGrid:
         a, b = C.c_uint64(0), C.c_uint64(0)
         self._chk(self._lib.fluid_count_pairs(self._h, C.byref(a), C.byref(b)))
         return a.value, b.value
+
+
+class SphBatch(SphSimulation3DGrid):
+    """Many independent scenes advanced together (fluid_batch_create): one
+    launch per stage over all scenes.  Readbacks return all scenes
+    concatenated; `scene_slice(k)` selects scene k's rows."""
+
+    def __init__(self, scenes, pcisph_iterations: int = 4, device: int = 0):
+        """scenes: iterable of (particles (n,16) f32, GpuSphParams3D,
+        GpuContainerGeometry, GpuGravity, bounds_extent)."""
+        self._lib = abi.load_library()
+        scenes = list(scenes)
+        descs = (abi.FluidSceneDesc * len(scenes))()
+        self._keep = []
+        for d, (parts, params, cont, grav, extent) in zip(descs, scenes):
+            arr = np.ascontiguousarray(parts, dtype=np.float32).reshape(-1, 16)
+            self._keep.append(arr)
+            d.particles = arr.ctypes.data if len(arr) else None
+            d.num_particles = len(arr)
+            d.sph_params = params
+            d.container = cont
+            d.gravity = grav
+            d.bounds_extent = float(extent)
+        handle = C.c_void_p()
+        abi.check(self._lib, self._lib.fluid_batch_create(descs, len(scenes), pcisph_iterations,
+                                                          device, C.byref(handle)))
+        self._h = handle
+
+    def num_scenes(self) -> int:
+        return self._lib.fluid_batch_num_scenes(self._h)
+
+    def scene_info(self, k: int) -> dict:
+        first, count, cell0 = C.c_uint32(), C.c_uint32(), C.c_uint32()
+        grid = abi.GpuGridParams()
+        self._chk(self._lib.fluid_batch_scene_info(self._h, k, C.byref(first), C.byref(count),
+                                                   C.byref(cell0), C.byref(grid)))
+        return {"first": first.value, "count": count.value, "first_cell": cell0.value,
+                "cells": grid.total_cells, "grid": grid}
+
+    def scene_slice(self, k: int) -> slice:
+        i = self.scene_info(k)
+        return slice(i["first"], i["first"] + i["count"])
+
+    def update_scene(self, k: int, params=None, container=None, gravity=None) -> None:
+        self._chk(self._lib.fluid_batch_update_scene(
+            self._h, k, C.byref(params) if params is not None else None,
+            C.byref(container) if container is not None else None,
+            C.byref(gravity) if gravity is not None else None))
+
+    def read_cell_counts(self) -> np.ndarray:
+        total = sum(self.scene_info(k)["cells"] for k in range(self.num_scenes()))
+        return self._read_u32(self._lib.fluid_read_cell_counts, total)
+
+    def read_cell_starts(self) -> np.ndarray:
+        total = sum(self.scene_info(k)["cells"] for k in range(self.num_scenes()))
+        return self._read_u32(self._lib.fluid_read_cell_starts, total)

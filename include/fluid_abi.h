@@ -308,6 +308,40 @@ int fluid_sync(fluid_sim *sim);
 /* The handle's hipStream_t, as an opaque pointer (for interop). */
 void *fluid_stream(fluid_sim *sim);
 
+/* ---- batched form ------------------------------------------------------ */
+
+/* One scene of a batch: its particles, uniforms and grid extent (as in
+ * fluid_create_info; the scene's grid is make_grid(h, bounds_extent)). */
+typedef struct fluid_scene_desc {
+  const SphParticle3D *particles;
+  uint32_t num_particles;
+  GpuSphParams3D sph_params;
+  GpuContainerGeometry container;
+  GpuGravity gravity;
+  float bounds_extent;
+} fluid_scene_desc;
+
+/* Many independent scenes advanced together: every stage is one launch over
+ * all scenes' particles (concatenated scene-major, each scene keeping a
+ * fixed slot range) and one scan over all scenes' grids (laid end to end).
+ * The handle works with fluid_step/_n, fluid_time_*, fluid_autotune,
+ * fluid_pair_stats, fluid_sync and the readbacks, which return all scenes
+ * concatenated (original index = scene's first particle + index in scene).
+ * Per-scene uniforms change through fluid_batch_update_scene; the single-
+ * scene fluid_update_sph_params/_container_geometry/_gravity and
+ * fluid_spawn_particles return FLUID_ERR_STATE on a batch.  No counterpart
+ * in the reference (which simulates one scene per SphSimulation3DGrid). */
+int fluid_batch_create(const fluid_scene_desc *scenes, uint32_t num_scenes,
+                       uint32_t pcisph_iterations, int device, fluid_sim **out);
+uint32_t fluid_batch_num_scenes(const fluid_sim *sim);
+int fluid_batch_scene_info(const fluid_sim *sim, uint32_t scene,
+                           uint32_t *first_particle, uint32_t *num_particles,
+                           uint32_t *first_cell, GpuGridParams *grid);
+int fluid_batch_update_scene(fluid_sim *sim, uint32_t scene,
+                             const GpuSphParams3D *params,
+                             const GpuContainerGeometry *container,
+                             const GpuGravity *gravity);
+
 /* ---- measurement ------------------------------------------------------ */
 
 /* Time `steps` graph-launched steps with HIP events on the handle's stream

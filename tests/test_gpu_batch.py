@@ -1,0 +1,74 @@
+"""Batched form: many independent scenes in one handle (fluid_batch_create)
+against the same scenes stepped one by one by plain handles and by the CPU
+oracle.  Grid data and permutations are exact per scene; fields agree to
+the plain step's parity bar."""
+import numpy as np
+import pytest
+
+from conftest import rowwise_rel_err
+
+pytestmark = pytest.mark.gpu
+
+
+def small_batch(pkg, count=6, seed=11):
+    return pkg.scenes.batch_scenes(count, seed=seed, n_min=1000, n_max=6000)
+
+
+def test_batch_matches_plain_and_oracle(pkg, orc):
+    scenes = small_batch(pkg)
+    print(f"case: batch of {len(scenes)} scenes, n={[s.n for s in scenes]}", flush=True)
+    batch = pkg.make_batch(scenes)
+    assert batch.num_scenes() == len(scenes)
+    batch.step()
+    counts, starts = batch.read_cell_counts(), batch.read_cell_starts()
+    s2o = batch.read_sorted_to_orig()
+    parts = batch.read_particles()
+    for k, sc in enumerate(scenes):
+        info = batch.scene_info(k)
+        sl = slice(info["first"], info["first"] + info["count"])
+        cl = slice(info["first_cell"], info["first_cell"] + info["cells"])
+        plain = pkg.make_sim(sc)
+        plain.step()
+        np.testing.assert_array_equal(counts[cl], plain.read_cell_counts())
+        np.testing.assert_array_equal(starts[cl] - info["first"], plain.read_cell_starts())
+        np.testing.assert_array_equal(s2o[sl] - info["first"], plain.read_sorted_to_orig())
+        pp = plain.read_particles()
+        for name, cols in (("position", slice(0, 3)), ("velocity", slice(4, 7)),
+                           ("force", slice(8, 11))):
+            err = rowwise_rel_err(parts[sl, cols], pp[:, cols])
+            print(f"  scene {k} {name} vs plain: max {err.max():.3e}", flush=True)
+            assert err.max() <= 1e-5
+        params, cont, g = pkg.scenes.make_params(sc, pkg.abi, pkg.derive_sph_params,
+                                                 pkg.derive_container_geometry)
+        o = orc.from_scene(sc, pkg.abi, params, cont, g)
+        o.step()
+        np.testing.assert_array_equal(counts[cl], o.cell_counts())
+        op = o.particles()
+        err = rowwise_rel_err(parts[sl, 0:3], op[:, 0:3])
+        print(f"  scene {k} position vs oracle: max {err.max():.3e}", flush=True)
+        assert np.quantile(err, 0.999) <= 1e-4
+        plain.close()
+    batch.close()
+
+
+def test_batch_rollout_invariants(pkg):
+    scenes = pkg.scenes.batch_scenes(32, seed=12, n_min=2048, n_max=16384)
+    print(f"case: 32-scene rollout, total n={sum(s.n for s in scenes)}", flush=True)
+    batch = pkg.make_batch(scenes)
+    cfg = batch.autotune(1)
+    print(f"  autotuned {cfg}", flush=True)
+    batch.step(20)
+    counts = batch.read_cell_counts().astype(np.int64)
+    parts = batch.read_particles()
+    assert np.isfinite(parts).all()
+    for k, sc in enumerate(scenes):
+        info = batch.scene_info(k)
+        assert counts[info["first_cell"]:info["first_cell"] + info["cells"]].sum() == sc.n
+        p = parts[info["first"]:info["first"] + info["count"], 0:3]
+        half = np.array(sc.box) / 2 + 1e-4
+        # integrate's backstop keeps every particle inside its own box
+        assert np.all(np.abs(p[:, [0, 2]]) <= half[[0, 2]])
+        assert np.all(np.abs(p[:, 1]) <= half[1])
+    with pytest.raises(pkg.abi.FluidError):
+        batch.spawn_particles([0, 0, 0], 4, 0.1)
+    batch.close()
