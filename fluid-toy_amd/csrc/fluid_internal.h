@@ -122,9 +122,10 @@ struct Bufs {
   const float *sdf;
 };
 
-// Scan tiling: 256 threads x 4 cells.
+// Scan tiling: 256 threads x 16 cells (long look-back chains are the scan's
+// critical path on multi-million-cell batched grids).
 constexpr uint32_t kScanThreads = 256;
-constexpr uint32_t kScanItems = 4;
+constexpr uint32_t kScanItems = 16;
 constexpr uint32_t kScanTile = kScanThreads * kScanItems;
 
 // In-range neighbour entries stored per particle (split over its lanes).
@@ -154,16 +155,18 @@ hipError_t launch_density(const Uni &u, const Bufs &b, uint32_t n,
                           SweepCfg c, hipStream_t st);
 hipError_t launch_xsph(const Uni &u, const Bufs &b, uint32_t n,
                        SweepCfg c, hipStream_t st);
+// fuse_predict: also write pcisph_predict's output for each particle.
 hipError_t launch_force(const Uni &u, const Bufs &b, uint32_t n,
-                        SweepCfg c, hipStream_t st);
+                        SweepCfg c, bool fuse_predict, hipStream_t st);
 hipError_t launch_predict(const Uni &u, const Bufs &b, uint32_t n,
                           hipStream_t st);
 hipError_t launch_solve(const Uni &u, const Bufs &b, uint32_t n,
                         int in_buf, SweepCfg c, hipStream_t st);
 hipError_t launch_finalize(const Uni &u, const Bufs &b, uint32_t n,
                            int fin_buf, hipStream_t st);
+// fused_fin_buf >= 0: apply pcisph_finalize from that predicted buffer first.
 hipError_t launch_integrate(const Uni &u, const Bufs &b, uint32_t n,
-                            hipStream_t st);
+                            int fused_fin_buf, hipStream_t st);
 hipError_t launch_count_pairs(const Uni &u, const Bufs &b, uint32_t n,
                               unsigned long long *out4, hipStream_t st);
 

@@ -122,58 +122,63 @@ int push_uniforms(fluid_sim *s) {
   return FLUID_OK;
 }
 
-int enqueue_stage(fluid_sim *s, uint32_t stage) {
+// Enqueue stages [first, last].  Where the range holds both stages of a
+// pair, the pair runs as one kernel: force + predict (the force epilogue
+// writes the predicted state) and finalize + integrate.
+int enqueue_range(fluid_sim *s, uint32_t first, uint32_t last) {
   const uint32_t n = s->n;
   hipStream_t st = s->stream;
   const Bufs &b = s->b;
-  switch (stage) {
-    case FLUID_STAGE_CLEAR:
-      HIP_TRY(launch_clear(b, s->total_cells, s->ntiles, st));
-      break;
-    case FLUID_STAGE_BUILD:
-      HIP_TRY(launch_build(s->uni(), b, n, st));
-      break;
-    case FLUID_STAGE_SCAN:
-      HIP_TRY(launch_scan(s->uni(), b, s->total_cells, st));
-      break;
-    case FLUID_STAGE_REORDER:
-      HIP_TRY(launch_reorder(s->uni(), b, n, st));
-      break;
-    case FLUID_STAGE_DENSITY:
-      HIP_TRY(launch_density(s->uni(), b, n, s->cfg, st));
-      break;
-    case FLUID_STAGE_XSPH:
-      HIP_TRY(launch_xsph(s->uni(), b, n, s->cfg, st));
-      break;
-    case FLUID_STAGE_FORCE:
-      HIP_TRY(launch_force(s->uni(), b, n, s->cfg, st));
-      break;
-    case FLUID_STAGE_PREDICT:
-      HIP_TRY(launch_predict(s->uni(), b, n, st));
-      break;
-    case FLUID_STAGE_SOLVE:
-      for (uint32_t it = 0; it < s->iters; ++it)
-        HIP_TRY(launch_solve(s->uni(), b, n, (int)(it % 2), s->cfg, st));
-      break;
-    case FLUID_STAGE_FINALIZE:
-      HIP_TRY(launch_finalize(s->uni(), b, n, (s->iters % 2 == 0) ? 0 : 1, st));
-      break;
-    case FLUID_STAGE_INTEGRATE:
-      HIP_TRY(launch_integrate(s->uni(), b, n, st));
-      break;
-    default:
-      return fail(FLUID_ERR_INVALID_ARG, "unknown stage");
+  const Uni u = s->uni();
+  const bool fuse_fp = first <= FLUID_STAGE_FORCE && last >= FLUID_STAGE_PREDICT;
+  const bool fuse_fi = first <= FLUID_STAGE_FINALIZE && last >= FLUID_STAGE_INTEGRATE;
+  const int fin_buf = (s->iters % 2 == 0) ? 0 : 1;
+  for (uint32_t stage = first; stage <= last; ++stage) {
+    switch (stage) {
+      case FLUID_STAGE_CLEAR:
+        HIP_TRY(launch_clear(b, s->total_cells, s->ntiles, st));
+        break;
+      case FLUID_STAGE_BUILD:
+        HIP_TRY(launch_build(u, b, n, st));
+        break;
+      case FLUID_STAGE_SCAN:
+        HIP_TRY(launch_scan(u, b, s->total_cells, st));
+        break;
+      case FLUID_STAGE_REORDER:
+        HIP_TRY(launch_reorder(u, b, n, st));
+        break;
+      case FLUID_STAGE_DENSITY:
+        HIP_TRY(launch_density(u, b, n, s->cfg, st));
+        break;
+      case FLUID_STAGE_XSPH:
+        HIP_TRY(launch_xsph(u, b, n, s->cfg, st));
+        break;
+      case FLUID_STAGE_FORCE:
+        HIP_TRY(launch_force(u, b, n, s->cfg, fuse_fp, st));
+        break;
+      case FLUID_STAGE_PREDICT:
+        if (!fuse_fp) HIP_TRY(launch_predict(u, b, n, st));
+        break;
+      case FLUID_STAGE_SOLVE:
+        for (uint32_t it = 0; it < s->iters; ++it)
+          HIP_TRY(launch_solve(u, b, n, (int)(it % 2), s->cfg, st));
+        break;
+      case FLUID_STAGE_FINALIZE:
+        if (!fuse_fi) HIP_TRY(launch_finalize(u, b, n, fin_buf, st));
+        break;
+      case FLUID_STAGE_INTEGRATE:
+        HIP_TRY(launch_integrate(u, b, n, fuse_fi ? fin_buf : -1, st));
+        break;
+      default:
+        return fail(FLUID_ERR_INVALID_ARG, "unknown stage");
+    }
   }
   return FLUID_OK;
 }
 
-int enqueue_step(fluid_sim *s) {
-  for (uint32_t st = 0; st < FLUID_STAGE_COUNT; ++st) {
-    const int rc = enqueue_stage(s, st);
-    if (rc) return rc;
-  }
-  return FLUID_OK;
-}
+int enqueue_stage(fluid_sim *s, uint32_t stage) { return enqueue_range(s, stage, stage); }
+
+int enqueue_step(fluid_sim *s) { return enqueue_range(s, 0, FLUID_STAGE_COUNT - 1); }
 
 void drop_graph(fluid_sim *s) {
   if (s->graph) (void)hipGraphExecDestroy(s->graph);
@@ -500,10 +505,8 @@ int fluid_step_stages(fluid_sim *s, uint32_t first, uint32_t last) {
   if ((int)first != s->cursor + 1)
     return fail(FLUID_ERR_STATE, "stages must continue the current step in order");
   HIP_TRY(hipSetDevice(s->device));
-  for (uint32_t st = first; st <= last; ++st) {
-    const int rc = enqueue_stage(s, st);
-    if (rc) return rc;
-  }
+  const int rc = enqueue_range(s, first, last);
+  if (rc) return rc;
   s->cursor = (last == FLUID_STAGE_COUNT - 1) ? -1 : (int)last;
   return FLUID_OK;
 }

@@ -204,17 +204,20 @@ __global__ __launch_bounds__(256) void k_scan(Bufs b, uint32_t cells) {
   __syncthreads();
   const uint32_t tile = s_tile;
   const uint32_t base = tile * kScanTile + threadIdx.x * kScanItems;
-  uint32_t v0 = 0, v1 = 0, v2 = 0, v3 = 0;
-  if (base + 3 < cells) {
-    const uint4 x = *reinterpret_cast<const uint4 *>(b.counts + base);
-    v0 = x.x; v1 = x.y; v2 = x.z; v3 = x.w;
+  uint32_t v[kScanItems];
+  if (base + kScanItems <= cells) {
+#pragma unroll
+    for (uint32_t q = 0; q < kScanItems / 4; ++q) {
+      const uint4 x = *reinterpret_cast<const uint4 *>(b.counts + base + 4 * q);
+      v[4 * q] = x.x; v[4 * q + 1] = x.y; v[4 * q + 2] = x.z; v[4 * q + 3] = x.w;
+    }
   } else {
-    if (base < cells) v0 = b.counts[base];
-    if (base + 1 < cells) v1 = b.counts[base + 1];
-    if (base + 2 < cells) v2 = b.counts[base + 2];
+#pragma unroll
+    for (uint32_t q = 0; q < kScanItems; ++q) v[q] = base + q < cells ? b.counts[base + q] : 0u;
   }
-  v1 += v0; v2 += v1; v3 += v2;
-  const uint32_t tsum = v3;
+#pragma unroll
+  for (uint32_t q = 1; q < kScanItems; ++q) v[q] += v[q - 1];
+  const uint32_t tsum = v[kScanItems - 1];
   const uint32_t incl = wave_incl_scan(tsum);
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
   if (lane == 63) s_wave[wid] = incl;
@@ -265,13 +268,15 @@ __global__ __launch_bounds__(256) void k_scan(Bufs b, uint32_t cells) {
   }
   __syncthreads();
   const uint32_t off = s_prefix + wave_off + (incl - tsum);
-  if (base + 3 < cells) {
-    *reinterpret_cast<uint4 *>(b.starts + base) =
-        make_uint4(off + v0, off + v1, off + v2, off + v3);
+  if (base + kScanItems <= cells) {
+#pragma unroll
+    for (uint32_t q = 0; q < kScanItems / 4; ++q)
+      *reinterpret_cast<uint4 *>(b.starts + base + 4 * q) =
+          make_uint4(off + v[4 * q], off + v[4 * q + 1], off + v[4 * q + 2], off + v[4 * q + 3]);
   } else {
-    if (base < cells) b.starts[base] = off + v0;
-    if (base + 1 < cells) b.starts[base + 1] = off + v1;
-    if (base + 2 < cells) b.starts[base + 2] = off + v2;
+#pragma unroll
+    for (uint32_t q = 0; q < kScanItems; ++q)
+      if (base + q < cells) b.starts[base + q] = off + v[q];
   }
 }
 
@@ -530,7 +535,8 @@ __global__ __launch_bounds__(256) void k_xsph(Uni un, Bufs b, uint32_t n, uint32
 // 1/(rho_i + rho_j)); r < 0.1 h, which switches the direction, is decided on
 // the exact bits the oracle uses.
 template <int L, bool B>
-__global__ __launch_bounds__(256) void k_force(Uni un, Bufs b, uint32_t n, uint32_t xcd) {
+__global__ __launch_bounds__(256) void k_force(Uni un, Bufs b, uint32_t n, uint32_t xcd,
+                                               uint32_t fuse_predict) {
   const uint32_t g = xcd_block(xcd) * blockDim.x + threadIdx.x;
   const uint32_t s = g / L, sub = g % L;
   if (s >= n) return;
@@ -598,8 +604,17 @@ __global__ __launch_bounds__(256) void k_force(Uni un, Bufs b, uint32_t n, uint3
   acx = group_sum<L>(acx); acy = group_sum<L>(acy); acz = group_sum<L>(acz);
   if (sub != 0) return;
   const float st = U->st;
-  b.frc[s] = make_float4(apx + acx * st + U->grav[0], apy + acy * st + U->grav[1],
-                         apz + acz * st + U->grav[2], 0.f);
+  const float fx = apx + acx * st + U->grav[0], fy = apy + acy * st + U->grav[1],
+              fz = apz + acz * st + U->grav[2];
+  b.frc[s] = make_float4(fx, fy, fz, 0.f);
+  if (fuse_predict) {
+    // pcisph_predict.wgsl:57-72 for this particle, on the values in hand
+    const float dt = U->dt;
+    const float vx = vi.x + dt * fx, vy = vi.y + dt * fy, vz = vi.z + dt * fz;
+    b.pred[0].xq[s] = make_float4(pi.x + dt * vx, pi.y + dt * vy, pi.z + dt * vz, 0.f);
+    b.pred[0].pr[s] = make_float2(0.f, 0.f);
+    b.pred[0].v[s] = make_float4(vx, vy, vz, 0.f);
+  }
 }
 
 // ------------------------------------------------------------------------
@@ -725,11 +740,20 @@ __device__ float sdf_sample(const float *sdf, uint32_t res, float ux, float uy,
 }
 
 template <bool B>
-__global__ __launch_bounds__(256) void k_integrate(Uni un, Bufs b, uint32_t n) {
+__global__ __launch_bounds__(256) void k_integrate(Uni un, Bufs b, uint32_t n,
+                                                   const float4 *__restrict__ fin_v) {
   const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= n) return;
   const DevUniforms *__restrict__ U = uni_at<B>(un, s);
-  const float4 p4 = b.spos[s], v4 = b.vel[s];
+  const float4 p4 = b.spos[s];
+  float4 v4 = b.vel[s];
+  if (fin_v) {
+    // pcisph_finalize.wgsl:50-74 for this particle first: force <- v* - v,
+    // velocity <- v*
+    const float4 cv = fin_v[s];
+    b.frc[s] = make_float4(cv.x - v4.x, cv.y - v4.y, cv.z - v4.z, 0.f);
+    v4 = make_float4(cv.x, cv.y, cv.z, 0.f);
+  }
   float px = p4.x, py = p4.y, pz = p4.z, vx = v4.x, vy = v4.y, vz = v4.z;
   const float(*iv)[4] = U->iv;
   const float(*fw)[4] = U->fw;
@@ -1016,9 +1040,11 @@ hipError_t launch_xsph(const Uni &u, const Bufs &b, uint32_t n, SweepCfg c, hipS
   return hipGetLastError();
 }
 
-hipError_t launch_force(const Uni &u, const Bufs &b, uint32_t n, SweepCfg c, hipStream_t st) {
+hipError_t launch_force(const Uni &u, const Bufs &b, uint32_t n, SweepCfg c, bool fuse_predict,
+                        hipStream_t st) {
   if (n == 0) return hipSuccess;
-  FLUID_DISPATCH_L(k_force, c.lanes, n, u, b, n, (uint32_t)c.xcd_tiles);
+  FLUID_DISPATCH_L(k_force, c.lanes, n, u, b, n, (uint32_t)c.xcd_tiles,
+                   (uint32_t)(fuse_predict ? 1 : 0));
   return hipGetLastError();
 }
 
@@ -1044,9 +1070,11 @@ hipError_t launch_finalize(const Uni &, const Bufs &b, uint32_t n, int fin_buf,
   return hipGetLastError();
 }
 
-hipError_t launch_integrate(const Uni &u, const Bufs &b, uint32_t n, hipStream_t st) {
+hipError_t launch_integrate(const Uni &u, const Bufs &b, uint32_t n, int fused_fin_buf,
+                            hipStream_t st) {
   if (n == 0) return hipSuccess;
-  FLUID_LAUNCH_B(k_integrate, blocks_for(n, 256), u, b, n);
+  const float4 *fin = fused_fin_buf >= 0 ? b.pred[fused_fin_buf].v : nullptr;
+  FLUID_LAUNCH_B(k_integrate, blocks_for(n, 256), u, b, n, fin);
   return hipGetLastError();
 }
 
