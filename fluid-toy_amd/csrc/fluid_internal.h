@@ -160,8 +160,9 @@ hipError_t launch_force(const Uni &u, const Bufs &b, uint32_t n,
                         SweepCfg c, bool fuse_predict, hipStream_t st);
 hipError_t launch_predict(const Uni &u, const Bufs &b, uint32_t n,
                           hipStream_t st);
+// first_iter: the input holds predict's zero pressures (density sum only).
 hipError_t launch_solve(const Uni &u, const Bufs &b, uint32_t n,
-                        int in_buf, SweepCfg c, hipStream_t st);
+                        int in_buf, SweepCfg c, bool first_iter, hipStream_t st);
 hipError_t launch_finalize(const Uni &u, const Bufs &b, uint32_t n,
                            int fin_buf, hipStream_t st);
 // fused_fin_buf >= 0: apply pcisph_finalize from that predicted buffer first.

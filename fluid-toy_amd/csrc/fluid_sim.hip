@@ -161,7 +161,7 @@ int enqueue_range(fluid_sim *s, uint32_t first, uint32_t last) {
         break;
       case FLUID_STAGE_SOLVE:
         for (uint32_t it = 0; it < s->iters; ++it)
-          HIP_TRY(launch_solve(u, b, n, (int)(it % 2), s->cfg, st));
+          HIP_TRY(launch_solve(u, b, n, (int)(it % 2), s->cfg, it == 0, st));
         break;
       case FLUID_STAGE_FINALIZE:
         if (!fuse_fi) HIP_TRY(launch_finalize(u, b, n, fin_buf, st));

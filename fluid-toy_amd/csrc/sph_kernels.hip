@@ -639,7 +639,7 @@ __global__ __launch_bounds__(256) void k_predict(Uni un, Bufs b, uint32_t n) {
 // (same operations as the reference evaluates per pair).
 template <int L, bool B>
 __global__ __launch_bounds__(256) void k_solve(Uni un, Bufs b, uint32_t n, PredBuf in,
-                                               PredBuf out, uint32_t xcd) {
+                                               PredBuf out, uint32_t xcd, uint32_t first_iter) {
   const uint32_t g = xcd_block(xcd) * blockDim.x + threadIdx.x;
   const uint32_t s = g / L, sub = g % L;
   if (s >= n) return;
@@ -652,6 +652,18 @@ __global__ __launch_bounds__(256) void k_solve(Uni un, Bufs b, uint32_t n, PredB
   const float h = U->h, h2 = U->h2, mass = U->mass, ps = U->poly6_scale;
   const float gs = U->grad_scale;
   float pd = 0.f, ax = 0.f, ay = 0.f, az = 0.f;
+  if (first_iter) {
+    // First iteration: every pressure of the input is 0 (pcisph_predict.wgsl
+    // :69), so each pressure term is a signed zero and a_total = a_np
+    // exactly; only the predicted density needs the sum.
+    sweep27<L, false>(U, b.counts, b.starts, in.xq, cx, cy, cz, sub,
+                      [&](uint32_t, float4 xj, bool) {
+      const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
+      const float r2 = dot3(rx, ry, rz, rx, ry, rz);
+      const float d = r2 < h2 ? h2 - r2 : 0.f;
+      pd += mass * (ps * d * d * d);
+    });
+  } else {
   sweep27<L, false>(U, b.counts, b.starts, in.xq, cx, cy, cz, sub,
                     [&](uint32_t, float4 xj, bool) {
     const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
@@ -670,6 +682,7 @@ __global__ __launch_bounds__(256) void k_solve(Uni un, Bufs b, uint32_t n, PredB
       }
     }
   });
+  }
   pd = group_sum<L>(pd);
   ax = group_sum<L>(ax);
   ay = group_sum<L>(ay);
@@ -1055,10 +1068,11 @@ hipError_t launch_predict(const Uni &u, const Bufs &b, uint32_t n, hipStream_t s
 }
 
 hipError_t launch_solve(const Uni &u, const Bufs &b, uint32_t n, int in_buf, SweepCfg c,
-                        hipStream_t st) {
+                        bool first_iter, hipStream_t st) {
   if (n == 0) return hipSuccess;
   const PredBuf in = b.pred[in_buf], out = b.pred[1 - in_buf];
-  FLUID_DISPATCH_L(k_solve, c.solve_lanes, n, u, b, n, in, out, (uint32_t)c.xcd_tiles);
+  FLUID_DISPATCH_L(k_solve, c.solve_lanes, n, u, b, n, in, out, (uint32_t)c.xcd_tiles,
+                   (uint32_t)(first_iter ? 1 : 0));
   return hipGetLastError();
 }
 
