@@ -111,6 +111,7 @@ struct Bufs {
   float4 *srec;
   uint32_t *nbr, *ncnt;
   uint32_t *sid;
+  uint32_t *gmap;  // new slot -> previous slot of the last reorder, or null
   uint32_t *counts, *starts;
   uint32_t *cell_of, *slot_of;
   uint2 *tmp;
@@ -161,13 +162,16 @@ hipError_t launch_force(const Uni &u, const Bufs &b, uint32_t n,
 hipError_t launch_predict(const Uni &u, const Bufs &b, uint32_t n,
                           hipStream_t st);
 // first_iter: the input holds predict's zero pressures (density sum only).
-hipError_t launch_solve(const Uni &u, const Bufs &b, uint32_t n,
-                        int in_buf, SweepCfg c, bool first_iter, hipStream_t st);
+// ap_out (may be null): the iteration's pressure acceleration before the
+// CFL clamp, per slot.
+hipError_t launch_solve(const Uni &u, const Bufs &b, uint32_t n, const PredBuf &in,
+                        const PredBuf &out, float4 *ap_out, SweepCfg c, bool first_iter,
+                        hipStream_t st);
 hipError_t launch_finalize(const Uni &u, const Bufs &b, uint32_t n,
-                           int fin_buf, hipStream_t st);
-// fused_fin_buf >= 0: apply pcisph_finalize from that predicted buffer first.
+                           const PredBuf &fin, hipStream_t st);
+// fin_v non-null: apply pcisph_finalize from that predicted velocity first.
 hipError_t launch_integrate(const Uni &u, const Bufs &b, uint32_t n,
-                            int fused_fin_buf, hipStream_t st);
+                            const float4 *fin_v, hipStream_t st);
 hipError_t launch_count_pairs(const Uni &u, const Bufs &b, uint32_t n,
                               unsigned long long *out4, hipStream_t st);
 

@@ -13,6 +13,7 @@
 #include <string>
 #include <vector>
 
+#include "adjoint.h"
 #include "fluid_internal.h"
 
 using namespace fluid;
@@ -36,6 +37,22 @@ int fail(int code, const std::string &msg) {
 constexpr int kUniformRing = 16;
 
 }  // namespace
+
+// A forward rollout kept for its reverse: state checkpoints every `every`
+// steps, the final state, one segment's tape and the adjoint buffers.
+struct RolloutState {
+  bool valid = false;  // cleared by anything that changes the state or uniforms
+  uint32_t steps = 0, every = 0, n = 0, iters = 0;
+  SweepCfg cfg{};
+  // allocation sizes
+  uint32_t cap_n = 0, cap_iters = 0, cap_cells = 0, cap_every = 0, cap_ckpt = 0;
+  void *ckpt = nullptr;  // cap_ckpt x (pos, vel, sid)
+  void *fin = nullptr;   // pos, vel, frc, sid after the rollout
+  void *tape = nullptr;  // cap_every tape steps
+  void *adj = nullptr;   // adjoint buffers and staging
+  size_t ckpt_bytes = 0, tape_bytes = 0;
+  float ms[3] = {0.f, 0.f, 0.f};  // last backward: replay, reverse, total
+};
 
 // One scene of a handle: its uniforms, its dense grid's place in the
 // concatenated cell array, and its fixed range of particle slots.
@@ -76,6 +93,7 @@ struct fluid_sim {
   SweepCfg cfg{8, 8, 0};
   bool tuned = false;  // cfg chosen by fluid_set_sweep_lanes / fluid_autotune
   uint32_t ntiles = 0;
+  RolloutState roll;
 
   Uni uni() const { return Uni{d_u, d_scene_of}; }
   bool batched() const { return sc.size() > 1; }
@@ -105,6 +123,7 @@ void update_cfg(fluid_sim *s) {
 }
 
 int push_uniforms(fluid_sim *s) {
+  s->roll.valid = false;
   const int slot = s->ring_pos;
   s->ring_pos = (s->ring_pos + 1) % kUniformRing;
   if (s->ring_used[slot]) HIP_TRY(hipEventSynchronize(s->ring_ev[slot]));
@@ -126,6 +145,7 @@ int push_uniforms(fluid_sim *s) {
 // pair, the pair runs as one kernel: force + predict (the force epilogue
 // writes the predicted state) and finalize + integrate.
 int enqueue_range(fluid_sim *s, uint32_t first, uint32_t last) {
+  s->roll.valid = false;
   const uint32_t n = s->n;
   hipStream_t st = s->stream;
   const Bufs &b = s->b;
@@ -161,13 +181,14 @@ int enqueue_range(fluid_sim *s, uint32_t first, uint32_t last) {
         break;
       case FLUID_STAGE_SOLVE:
         for (uint32_t it = 0; it < s->iters; ++it)
-          HIP_TRY(launch_solve(u, b, n, (int)(it % 2), s->cfg, it == 0, st));
+          HIP_TRY(launch_solve(u, b, n, b.pred[it % 2], b.pred[1 - it % 2], nullptr, s->cfg,
+                               it == 0, st));
         break;
       case FLUID_STAGE_FINALIZE:
-        if (!fuse_fi) HIP_TRY(launch_finalize(u, b, n, fin_buf, st));
+        if (!fuse_fi) HIP_TRY(launch_finalize(u, b, n, b.pred[fin_buf], st));
         break;
       case FLUID_STAGE_INTEGRATE:
-        HIP_TRY(launch_integrate(u, b, n, fuse_fi ? fin_buf : -1, st));
+        HIP_TRY(launch_integrate(u, b, n, fuse_fi ? b.pred[fin_buf].v : nullptr, st));
         break;
       default:
         return fail(FLUID_ERR_INVALID_ARG, "unknown stage");
@@ -246,6 +267,246 @@ int upload_canonical(fluid_sim *s, const SphParticle3D *p, uint32_t n) {
     HIP_TRY(hipMemcpyAsync(s->b.sid, sid.data(), n * 4, hipMemcpyHostToDevice, s->stream));
     HIP_TRY(hipStreamSynchronize(s->stream));
   }
+  return FLUID_OK;
+}
+
+// ------------------------------------------------------------------------
+// Rollout storage.  Every piece of a block is 256-byte aligned; a Carve
+// with a null base only measures.
+struct Carve {
+  char *base = nullptr;
+  size_t off = 0;
+  template <class T>
+  T *take(size_t count) {
+    const size_t at = (off + 255) & ~(size_t)255;
+    off = at + count * sizeof(T);
+    return base ? reinterpret_cast<T *>(base + at) : nullptr;
+  }
+};
+
+struct CkptView {
+  float4 *pos, *vel, *frc;
+  uint32_t *sid;
+};
+
+CkptView carve_state(Carve &c, size_t n, bool with_frc) {
+  CkptView v{};
+  v.pos = c.take<float4>(n);
+  v.vel = c.take<float4>(n);
+  v.frc = with_frc ? c.take<float4>(n) : nullptr;
+  v.sid = c.take<uint32_t>(n);
+  return v;
+}
+
+// One step's tape: x, u, w, the density record, the final predicted
+// velocity, the grid, the slot map and every PCISPH iterate.
+struct TapeSlot {
+  float4 *x, *u, *w, *rec, *vK;
+  uint32_t *counts, *starts, *gmap;
+  std::vector<float4 *> xq;  // iters + 1
+  std::vector<float2 *> pr;  // iters + 1
+  std::vector<float4 *> ap;  // iters; ap[k] = iteration k's unclamped ap, ap[0] unused
+};
+
+TapeSlot carve_tape(Carve &c, size_t n, size_t cells, uint32_t iters) {
+  TapeSlot t;
+  t.x = c.take<float4>(n);
+  t.u = c.take<float4>(n);
+  t.w = c.take<float4>(n);
+  t.rec = c.take<float4>(3 * n);
+  t.vK = c.take<float4>(n);
+  t.counts = c.take<uint32_t>(cells);
+  t.starts = c.take<uint32_t>(cells);
+  t.gmap = c.take<uint32_t>(n);
+  for (uint32_t k = 0; k <= iters; ++k) {
+    t.xq.push_back(c.take<float4>(n));
+    t.pr.push_back(c.take<float2>(n));
+  }
+  t.ap.push_back(nullptr);
+  for (uint32_t k = 1; k < iters; ++k) t.ap.push_back(c.take<float4>(n));
+  return t;
+}
+
+struct AdjViews {
+  float4 *lam_x[2], *lam_v[2];
+  float4 *lx, *lw, *la, *lu, *ln;
+  float2 *lrho;
+  float4 *lap[2];
+  float *lP;
+  double *acc;
+  float4 *scratch_v;                  // non-final solve velocities of a replay
+  float *in_x, *in_v, *out_x, *out_v;  // canonical (3 floats per particle) staging
+};
+
+AdjViews carve_adj(Carve &c, size_t n) {
+  AdjViews a{};
+  for (int k = 0; k < 2; ++k) {
+    a.lam_x[k] = c.take<float4>(n);
+    a.lam_v[k] = c.take<float4>(n);
+  }
+  a.lx = c.take<float4>(n);
+  a.lw = c.take<float4>(n);
+  a.la = c.take<float4>(n);
+  a.lu = c.take<float4>(n);
+  a.ln = c.take<float4>(n);
+  a.lrho = c.take<float2>(n);
+  a.lap[0] = c.take<float4>(n);
+  a.lap[1] = c.take<float4>(n);
+  a.lP = c.take<float>(n);
+  a.acc = c.take<double>((size_t)kAccStride * n);
+  a.scratch_v = c.take<float4>(n);
+  a.in_x = c.take<float>(3 * n);
+  a.in_v = c.take<float>(3 * n);
+  a.out_x = c.take<float>(3 * n);
+  a.out_v = c.take<float>(3 * n);
+  return a;
+}
+
+void free_rollout(fluid_sim *s) {
+  RolloutState &R = s->roll;
+  for (void *p : {R.ckpt, R.fin, R.tape, R.adj})
+    if (p) (void)hipFree(p);
+  R.ckpt = R.fin = R.tape = R.adj = nullptr;
+  R.cap_n = R.cap_iters = R.cap_cells = R.cap_every = R.cap_ckpt = 0;
+  R.valid = false;
+}
+
+CkptView ckpt_view(fluid_sim *s, uint32_t k) {
+  Carve c{(char *)s->roll.ckpt + (size_t)k * s->roll.ckpt_bytes};
+  return carve_state(c, s->roll.cap_n, false);
+}
+
+CkptView fin_view(fluid_sim *s) {
+  Carve c{(char *)s->roll.fin};
+  return carve_state(c, s->roll.cap_n, true);
+}
+
+TapeSlot tape_slot(fluid_sim *s, uint32_t t) {
+  Carve c{(char *)s->roll.tape + (size_t)t * s->roll.tape_bytes};
+  return carve_tape(c, s->roll.cap_n, s->roll.cap_cells, s->roll.cap_iters);
+}
+
+AdjViews adj_views(fluid_sim *s) {
+  Carve c{(char *)s->roll.adj};
+  return carve_adj(c, s->roll.cap_n);
+}
+
+// Size the rollout storage for the handle's current n / iterations / grid.
+int ensure_rollout(fluid_sim *s, uint32_t every, uint32_t nckpt) {
+  RolloutState &R = s->roll;
+  const uint32_t n = s->n, iters = s->iters, cells = s->total_cells;
+  if (R.adj && R.cap_n == n && R.cap_iters == iters && R.cap_cells == cells &&
+      R.cap_every >= every && R.cap_ckpt >= nckpt)
+    return FLUID_OK;
+  free_rollout(s);
+  Carve m;
+  carve_state(m, n, false);
+  const size_t ck = (m.off + 255) & ~(size_t)255;
+  m = Carve{};
+  carve_state(m, n, true);
+  const size_t fin = m.off;
+  m = Carve{};
+  carve_tape(m, n, cells, iters);
+  const size_t tp = (m.off + 255) & ~(size_t)255;
+  m = Carve{};
+  carve_adj(m, n);
+  const size_t adj = m.off;
+  void *p[4] = {nullptr, nullptr, nullptr, nullptr};
+  const size_t bytes[4] = {ck * std::max(nckpt, 1u), fin, tp * std::max(every, 1u), adj};
+  for (int k = 0; k < 4; ++k) {
+    const hipError_t e = hipMalloc(&p[k], std::max(bytes[k], (size_t)256));
+    if (e != hipSuccess) {
+      for (int j = 0; j < k; ++j) (void)hipFree(p[j]);
+      return fail(FLUID_ERR_HIP, std::string("hipMalloc (rollout storage, ") +
+                                     std::to_string(bytes[k] >> 20) + " MiB): " +
+                                     hipGetErrorString(e));
+    }
+  }
+  R.ckpt = p[0];
+  R.fin = p[1];
+  R.tape = p[2];
+  R.adj = p[3];
+  R.ckpt_bytes = ck;
+  R.tape_bytes = tp;
+  R.cap_n = n;
+  R.cap_iters = iters;
+  R.cap_cells = cells;
+  R.cap_every = std::max(every, 1u);
+  R.cap_ckpt = std::max(nckpt, 1u);
+  return FLUID_OK;
+}
+
+hipError_t copy_state(fluid_sim *s, const CkptView &dst, const CkptView &src, size_t n) {
+  const hipMemcpyKind k = hipMemcpyDeviceToDevice;
+  hipError_t e = hipMemcpyAsync(dst.pos, src.pos, n * 16, k, s->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(dst.vel, src.vel, n * 16, k, s->stream);
+  if (e == hipSuccess && dst.frc && src.frc)
+    e = hipMemcpyAsync(dst.frc, src.frc, n * 16, k, s->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(dst.sid, src.sid, n * 4, k, s->stream);
+  return e;
+}
+
+CkptView live_state(fluid_sim *s) { return CkptView{s->b.pos, s->b.vel, s->b.frc, s->b.sid}; }
+
+// Replay one full step (the same kernels, fusions and sweep configuration
+// as the captured step) with its sorted arrays, grid, slot map and PCISPH
+// iterates redirected into tape slot T.
+int enqueue_tape_step(fluid_sim *s, const TapeSlot &T, float4 *scratch_v) {
+  const RolloutState &R = s->roll;
+  const uint32_t n = R.n, iters = R.iters;
+  const SweepCfg c = R.cfg;
+  const Uni u = s->uni();
+  hipStream_t st = s->stream;
+  Bufs rb = s->b;
+  rb.spos = T.x;
+  rb.svel = T.u;
+  rb.srec = T.rec;
+  rb.counts = T.counts;
+  rb.starts = T.starts;
+  rb.gmap = T.gmap;
+  HIP_TRY(launch_clear(rb, s->total_cells, s->ntiles, st));
+  HIP_TRY(launch_build(u, rb, n, st));
+  HIP_TRY(launch_scan(u, rb, s->total_cells, st));
+  HIP_TRY(launch_reorder(u, rb, n, st));
+  HIP_TRY(launch_density(u, rb, n, c, st));
+  Bufs mb = rb;  // XSPH's output w stays in the tape
+  mb.vel = T.w;
+  mb.pred[0] = PredBuf{T.xq[0], T.pr[0], scratch_v};
+  HIP_TRY(launch_xsph(u, mb, n, c, st));
+  HIP_TRY(launch_force(u, mb, n, c, true, st));
+  for (uint32_t k = 0; k < iters; ++k) {
+    const PredBuf in{T.xq[k], T.pr[k], scratch_v};
+    const PredBuf out{T.xq[k + 1], T.pr[k + 1], k + 1 == iters ? T.vK : scratch_v};
+    HIP_TRY(launch_solve(u, mb, n, in, out, k >= 1 ? T.ap[k] : nullptr, c, k == 0, st));
+  }
+  HIP_TRY(launch_integrate(u, rb, n, T.vK, st));
+  return FLUID_OK;
+}
+
+int enqueue_adjoint_step(fluid_sim *s, const TapeSlot &T, const AdjViews &A, int cur) {
+  const RolloutState &R = s->roll;
+  const uint32_t K = R.iters;
+  AdjBufs a{};
+  a.lin_x = A.lam_x[cur];
+  a.lin_v = A.lam_v[cur];
+  a.lout_x = A.lam_x[1 - cur];
+  a.lout_v = A.lam_v[1 - cur];
+  a.lx = A.lx;
+  a.lw = A.lw;
+  a.la = A.la;
+  a.lu = A.lu;
+  a.ln = A.ln;
+  a.lrho = A.lrho;
+  a.lap[0] = A.lap[0];
+  a.lap[1] = A.lap[1];
+  a.lP = A.lP;
+  a.acc = A.acc;
+  const TapeStep t{T.x, T.u, T.w, T.rec, T.vK, T.counts, T.starts, T.gmap};
+  std::vector<TapeIter> it(K);
+  for (uint32_t k = 0; k < K; ++k)
+    it[k] = TapeIter{T.xq[k], T.pr[k], k >= 2 ? T.ap[k - 1] : nullptr};
+  const TapeIter last{T.xq[K], T.pr[K], K >= 2 ? T.ap[K - 1] : nullptr};
+  HIP_TRY(launch_adjoint_step(s->uni(), a, t, it.data(), last, K, R.n, R.cfg.lanes, s->stream));
   return FLUID_OK;
 }
 
@@ -477,6 +738,7 @@ int fluid_destroy(fluid_sim *s) {
   (void)hipSetDevice(s->device);
   if (s->stream) (void)hipStreamSynchronize(s->stream);
   drop_graph(s);
+  free_rollout(s);
   for (void *p : s->allocs) (void)hipFree(p);
   for (int k = 0; k < kUniformRing; ++k)
     if (s->ring_ev[k]) (void)hipEventDestroy(s->ring_ev[k]);
@@ -494,6 +756,7 @@ int fluid_step_n(fluid_sim *s, uint32_t steps) {
   HIP_TRY(hipSetDevice(s->device));
   int rc = ensure_graph(s);
   if (rc) return rc;
+  s->roll.valid = false;
   for (uint32_t k = 0; k < steps; ++k) HIP_TRY(hipGraphLaunch(s->graph, s->stream));
   return FLUID_OK;
 }
@@ -558,6 +821,7 @@ int fluid_update_rigid_body(fluid_sim *s, const GpuRigidBody *r) {
 int fluid_set_pcisph_iterations(fluid_sim *s, uint32_t it) {
   if (!s) return fail(FLUID_ERR_INVALID_ARG, "null handle");
   s->iters = std::max(it, 1u);
+  s->roll.valid = false;
   return FLUID_OK;
 }
 
@@ -843,6 +1107,7 @@ int fluid_set_sweep_config(fluid_sim *s, const uint32_t cfg[3]) {
   if (cfg[1]) s->cfg.solve_lanes = (int)cfg[1];
   s->cfg.xcd_tiles = (int)cfg[2];
   s->tuned = true;
+  s->roll.valid = false;
   return FLUID_OK;
 }
 
@@ -952,6 +1217,133 @@ int fluid_count_pairs(fluid_sim *s, uint64_t *candidates, uint64_t *in_range) {
   if (rc) return rc;
   *candidates = st[0];
   *in_range = st[1];
+  return FLUID_OK;
+}
+
+// ------------------------------------------------------------------------
+// Differentiable rollout.
+int fluid_forward_rollout(fluid_sim *s, uint32_t steps, uint32_t checkpoint_every) {
+  if (!s) return fail(FLUID_ERR_INVALID_ARG, "null handle");
+  if (s->cursor != -1) return fail(FLUID_ERR_STATE, "a partial step is in progress");
+  if (s->mouse.is_active == 1u || s->rb.is_active != 0u)
+    return fail(FLUID_ERR_STATE, "mouse force and rigid body are not differentiated: "
+                                 "deactivate them before a rollout");
+  HIP_TRY(hipSetDevice(s->device));
+  uint32_t every = checkpoint_every;
+  if (every == 0) every = std::max(1u, (uint32_t)std::ceil(std::sqrt((double)steps)));
+  every = std::min(every, std::max(steps, 1u));
+  const uint32_t nck = (steps + every - 1) / every;
+  int rc = ensure_rollout(s, every, nck);
+  if (rc) return rc;
+  rc = ensure_graph(s);
+  if (rc) return rc;
+  RolloutState &R = s->roll;
+  R.valid = false;
+  const CkptView live = live_state(s);
+  for (uint32_t t = 0; t < steps; ++t) {
+    if (t % every == 0) HIP_TRY(copy_state(s, ckpt_view(s, t / every), live, s->n));
+    HIP_TRY(hipGraphLaunch(s->graph, s->stream));
+  }
+  HIP_TRY(copy_state(s, fin_view(s), live, s->n));
+  rc = check_errors(s);
+  if (rc) return rc;
+  R.steps = steps;
+  R.every = every;
+  R.n = s->n;
+  R.iters = s->iters;
+  R.cfg = s->cfg;
+  R.valid = true;
+  return FLUID_OK;
+}
+
+int fluid_backward_rollout(fluid_sim *s, const float *dl_dpos, const float *dl_dvel,
+                           float *dl_dpos0, float *dl_dvel0, double *dl_dparams) {
+  if (!s) return fail(FLUID_ERR_INVALID_ARG, "null handle");
+  RolloutState &R = s->roll;
+  if (!R.valid)
+    return fail(FLUID_ERR_STATE, "no forward rollout to reverse (or the state, parameters or "
+                                 "configuration changed since)");
+  HIP_TRY(hipSetDevice(s->device));
+  const uint32_t n = R.n;
+  hipStream_t st = s->stream;
+  const AdjViews A = adj_views(s);
+  const uint32_t nseg = (R.steps + R.every - 1) / R.every;
+  std::vector<hipEvent_t> ev(2 + 3 * (size_t)nseg);
+  for (auto &e : ev) HIP_TRY(hipEventCreate(&e));
+  auto drop_events = [&]() {
+    for (auto &e : ev) (void)hipEventDestroy(e);
+  };
+  HIP_TRY(hipEventRecord(ev[0], st));
+  if (n) {
+    if (dl_dpos) HIP_TRY(hipMemcpyAsync(A.in_x, dl_dpos, (size_t)n * 12, hipMemcpyDefault, st));
+    if (dl_dvel) HIP_TRY(hipMemcpyAsync(A.in_v, dl_dvel, (size_t)n * 12, hipMemcpyDefault, st));
+    HIP_TRY(launch_adj_load(s->b.sid, dl_dpos ? A.in_x : nullptr, dl_dvel ? A.in_v : nullptr,
+                            A.lam_x[0], A.lam_v[0], n, st));
+    HIP_TRY(hipMemsetAsync(A.acc, 0, (size_t)n * kAccStride * sizeof(double), st));
+  }
+  int cur = 0;
+  const CkptView live = live_state(s);
+  for (int seg = (int)nseg - 1; seg >= 0; --seg) {
+    const uint32_t t0 = (uint32_t)seg * R.every;
+    const uint32_t len = std::min(R.every, R.steps - t0);
+    HIP_TRY(copy_state(s, live, ckpt_view(s, (uint32_t)seg), n));
+    HIP_TRY(hipEventRecord(ev[1 + 3 * seg], st));
+    for (uint32_t t = 0; t < len; ++t) {
+      const int rc = enqueue_tape_step(s, tape_slot(s, t), A.scratch_v);
+      if (rc) return drop_events(), rc;
+    }
+    HIP_TRY(hipEventRecord(ev[2 + 3 * seg], st));
+    for (int t = (int)len - 1; t >= 0; --t) {
+      const int rc = enqueue_adjoint_step(s, tape_slot(s, (uint32_t)t), A, cur);
+      if (rc) return drop_events(), rc;
+      cur = 1 - cur;
+    }
+    HIP_TRY(hipEventRecord(ev[3 + 3 * seg], st));
+  }
+  std::vector<double> acc;
+  if (n) {
+    const uint32_t *sid0 = nseg ? ckpt_view(s, 0).sid : s->b.sid;
+    HIP_TRY(launch_adj_store(sid0, A.lam_x[cur], A.lam_v[cur], A.out_x, A.out_v, n, st));
+    if (dl_dpos0)
+      HIP_TRY(hipMemcpyAsync(dl_dpos0, A.out_x, (size_t)n * 12, hipMemcpyDefault, st));
+    if (dl_dvel0)
+      HIP_TRY(hipMemcpyAsync(dl_dvel0, A.out_v, (size_t)n * 12, hipMemcpyDefault, st));
+    if (dl_dparams) {
+      acc.resize((size_t)n * kAccStride);
+      HIP_TRY(hipMemcpyAsync(acc.data(), A.acc, acc.size() * sizeof(double),
+                             hipMemcpyDeviceToHost, st));
+    }
+  }
+  if (nseg) HIP_TRY(copy_state(s, live, fin_view(s), n));
+  HIP_TRY(hipEventRecord(ev[1 + 3 * nseg], st));
+  HIP_TRY(hipStreamSynchronize(st));
+  float replay = 0.f, reverse = 0.f, total = 0.f, ms = 0.f;
+  for (uint32_t k = 0; k < nseg; ++k) {
+    HIP_TRY(hipEventElapsedTime(&ms, ev[1 + 3 * k], ev[2 + 3 * k]));
+    replay += ms;
+    HIP_TRY(hipEventElapsedTime(&ms, ev[2 + 3 * k], ev[3 + 3 * k]));
+    reverse += ms;
+  }
+  HIP_TRY(hipEventElapsedTime(&total, ev[0], ev[1 + 3 * nseg]));
+  drop_events();
+  R.ms[0] = replay;
+  R.ms[1] = reverse;
+  R.ms[2] = total;
+  if (dl_dparams) {
+    for (size_t k = 0; k < s->sc.size(); ++k) {
+      const SceneState &c = s->sc[k];
+      double g[kGradParams] = {0, 0, 0, 0, 0};
+      for (uint32_t i = c.first; i < c.first + c.count && !acc.empty(); ++i)
+        for (int q = 0; q < kGradParams; ++q) g[q] += acc[(size_t)i * kAccStride + q];
+      for (int q = 0; q < kGradParams; ++q) dl_dparams[k * kGradParams + q] = g[q];
+    }
+  }
+  return check_errors(s);
+}
+
+int fluid_rollout_timing(const fluid_sim *s, float ms[3]) {
+  if (!s || !ms) return fail(FLUID_ERR_INVALID_ARG, "null argument");
+  for (int k = 0; k < 3; ++k) ms[k] = s->roll.ms[k];
   return FLUID_OK;
 }
 

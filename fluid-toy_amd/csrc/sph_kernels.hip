@@ -215,6 +215,7 @@ __global__ __launch_bounds__(256) void k_rank_gather(Bufs b, uint32_t n) {
   }
   const uint32_t dest = start + rank;
   b.sid[dest] = e.y;
+  if (b.gmap) b.gmap[dest] = e.x;
   b.spos[dest] = b.pos[e.x];
   b.svel[dest] = b.vel[e.x];
   b.sfrc[dest] = b.frc[e.x];
@@ -228,7 +229,6 @@ __global__ __launch_bounds__(256) void k_rank_gather(Bufs b, uint32_t n) {
 // later gathers.
 template <int L, bool B>
 __global__ __launch_bounds__(256) void k_density(Uni un, Bufs b, uint32_t n, uint32_t xcd) {
-  constexpr uint32_t kCap = kListCapacity / L;
   const uint32_t g = xcd_block(xcd) * blockDim.x + threadIdx.x;
   const uint32_t s = g / L, sub = g % L;
   if (s >= n) return;
@@ -525,7 +525,8 @@ __global__ __launch_bounds__(256) void k_predict(Uni un, Bufs b, uint32_t n) {
 // (same operations as the reference evaluates per pair).
 template <int L, bool B>
 __global__ __launch_bounds__(256) void k_solve(Uni un, Bufs b, uint32_t n, PredBuf in,
-                                               PredBuf out, uint32_t xcd, uint32_t first_iter) {
+                                               PredBuf out, float4 *ap_out, uint32_t xcd,
+                                               uint32_t first_iter) {
   const uint32_t g = xcd_block(xcd) * blockDim.x + threadIdx.x;
   const uint32_t s = g / L, sub = g % L;
   if (s >= n) return;
@@ -577,6 +578,7 @@ __global__ __launch_bounds__(256) void k_solve(Uni un, Bufs b, uint32_t n, PredB
   const float old_p = in.pr[s].x;
   const float err = fmaxf(0.0f, pd - U->rho0);
   const float new_p = old_p + U->delta * err;
+  if (ap_out) ap_out[s] = make_float4(ax, ay, az, 0.f);
   const float amag = dot3(ax, ay, az, ax, ay, az);
   if (amag > U->max_accel_sq) {
     const float f = U->max_accel / sqrtf(amag);
@@ -953,28 +955,26 @@ hipError_t launch_predict(const Uni &u, const Bufs &b, uint32_t n, hipStream_t s
   return hipGetLastError();
 }
 
-hipError_t launch_solve(const Uni &u, const Bufs &b, uint32_t n, int in_buf, SweepCfg c,
-                        bool first_iter, hipStream_t st) {
+hipError_t launch_solve(const Uni &u, const Bufs &b, uint32_t n, const PredBuf &in,
+                        const PredBuf &out, float4 *ap_out, SweepCfg c, bool first_iter,
+                        hipStream_t st) {
   if (n == 0) return hipSuccess;
-  const PredBuf in = b.pred[in_buf], out = b.pred[1 - in_buf];
-  FLUID_DISPATCH_L(k_solve, c.solve_lanes, n, u, b, n, in, out, (uint32_t)c.xcd_tiles,
-                   (uint32_t)(first_iter ? 1 : 0));
+  FLUID_DISPATCH_L(k_solve, c.solve_lanes, n, u, b, n, in, out, ap_out,
+                   (uint32_t)c.xcd_tiles, (uint32_t)(first_iter ? 1 : 0));
   return hipGetLastError();
 }
 
-hipError_t launch_finalize(const Uni &, const Bufs &b, uint32_t n, int fin_buf,
+hipError_t launch_finalize(const Uni &, const Bufs &b, uint32_t n, const PredBuf &fin,
                            hipStream_t st) {
   if (n == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_finalize, dim3(blocks_for(n, 256)), dim3(256), 0, st, b, n,
-                     b.pred[fin_buf]);
+  hipLaunchKernelGGL(k_finalize, dim3(blocks_for(n, 256)), dim3(256), 0, st, b, n, fin);
   return hipGetLastError();
 }
 
-hipError_t launch_integrate(const Uni &u, const Bufs &b, uint32_t n, int fused_fin_buf,
+hipError_t launch_integrate(const Uni &u, const Bufs &b, uint32_t n, const float4 *fin_v,
                             hipStream_t st) {
   if (n == 0) return hipSuccess;
-  const float4 *fin = fused_fin_buf >= 0 ? b.pred[fused_fin_buf].v : nullptr;
-  FLUID_LAUNCH_B(k_integrate, blocks_for(n, 256), u, b, n, fin);
+  FLUID_LAUNCH_B(k_integrate, blocks_for(n, 256), u, b, n, fin_v);
   return hipGetLastError();
 }
 

@@ -139,12 +139,33 @@ def block_4m() -> Scene:
     return block(256, 3)
 
 
-def dam_break_32k(seed: int = 4) -> Scene:
-    pts = lattice(32, 32, 32, 0.025)
+def dam_break_block(n_side: int, seed: int) -> Scene:
+    """n_side^3 block in the corner of the 2.0x1.2x1.0 box (left wall and
+    floor inside its boundary layer), initial velocity N(0, 0.1^2) per axis."""
+    pts = lattice(n_side, n_side, n_side, 0.025)
     pos = _jittered_block(pts, seed, (-1.0, 0.0, -0.5))
     vel = normal(seed + 1000, pts.size).reshape(pts.shape) * 0.1
-    return Scene("dam_break_32k", particles_from_positions(pos, vel), (2.0, 1.2, 1.0), 0.0,
-                 steps=64)
+    return Scene(f"dam_break_block{n_side}", particles_from_positions(pos, vel),
+                 (2.0, 1.2, 1.0), 0.0, steps=64)
+
+
+def relaxed_block(n_side: int, seed: int) -> Scene:
+    """n_side^3 block at the reference's spawn spacing 0.6 h (app.rs:179),
+    where the density is near rest, in the corner of the 2.0x1.2x1.0 box;
+    initial velocity N(0, 0.1^2) per axis and rest density from the
+    reference's lattice rule (compute_rest_density, 8678.83 at h = 0.08)."""
+    s = 0.6 * 0.08
+    pts = lattice(n_side, n_side, n_side, s)
+    pos = _jittered_block(pts, seed, (-1.0 + 0.25 * s, 0.25 * s, -0.5 + 0.25 * s))
+    vel = normal(seed + 1000, pts.size).reshape(pts.shape) * 0.1
+    return Scene(f"relaxed_block{n_side}", particles_from_positions(pos, vel),
+                 (2.0, 1.2, 1.0), 0.0, rest_density=0.0, steps=64)
+
+
+def dam_break_32k(seed: int = 4) -> Scene:
+    sc = dam_break_block(32, seed)
+    sc.name = "dam_break_32k"
+    return sc
 
 
 def small_block(nx: int, ny: int, nz: int, seed: int = 7, box=(1.6, 1.2, 0.8)) -> Scene:

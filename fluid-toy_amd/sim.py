@@ -256,6 +256,54 @@ class SphSimulation3DGrid:
         self._chk(self._lib.fluid_count_pairs(self._h, C.byref(a), C.byref(b)))
         return a.value, b.value
 
+    # -- differentiable rollout ------------------------------------------
+    def forward_rollout(self, steps: int, checkpoint_every: int = 0) -> None:
+        """Advance `steps` steps (as step(steps)) keeping checkpoints for
+        backward_rollout (fluid_forward_rollout)."""
+        self._chk(self._lib.fluid_forward_rollout(self._h, int(steps), int(checkpoint_every)))
+
+    def backward_rollout(self, dl_dpos=None, dl_dvel=None, want_state: bool = True):
+        """Reverse the last forward_rollout (fluid_backward_rollout).
+
+        dl_dpos / dl_dvel: (n, 3) float32 arrays in original index order, or
+        None for zero.  Returns (dL/dpos0 (n,3), dL/dvel0 (n,3), dL/dparams
+        (num_scenes, 5): viscosity, surface tension, gravity x, y, z)."""
+        n = self.num_particles()
+        keep = []
+
+        def src(a):
+            if a is None:
+                return None
+            arr = np.ascontiguousarray(a, dtype=np.float32).reshape(n, 3)
+            keep.append(arr)
+            return arr.ctypes.data
+        gx = np.zeros((n, 3), np.float32) if want_state else None
+        gv = np.zeros((n, 3), np.float32) if want_state else None
+        scenes = self._lib.fluid_batch_num_scenes(self._h)
+        gp = np.zeros((max(scenes, 1), abi.FLUID_GRAD_PARAMS), np.float64)
+        self._chk(self._lib.fluid_backward_rollout(
+            self._h, src(dl_dpos), src(dl_dvel),
+            gx.ctypes.data if gx is not None else None,
+            gv.ctypes.data if gv is not None else None, gp.ctypes.data))
+        return gx, gv, gp
+
+    def backward_rollout_device(self, dl_dpos_ptr: int, dl_dvel_ptr: int,
+                                out_pos_ptr: int, out_vel_ptr: int) -> np.ndarray:
+        """backward_rollout on device buffers (addresses of (n, 3) float32
+        arrays on this handle's device; 0 = null).  Returns dL/dparams."""
+        scenes = self._lib.fluid_batch_num_scenes(self._h)
+        gp = np.zeros((max(scenes, 1), abi.FLUID_GRAD_PARAMS), np.float64)
+        self._chk(self._lib.fluid_backward_rollout(
+            self._h, dl_dpos_ptr or None, dl_dvel_ptr or None, out_pos_ptr or None,
+            out_vel_ptr or None, gp.ctypes.data))
+        return gp
+
+    def rollout_timing(self) -> dict:
+        """HIP-event timing of the last backward_rollout (ms)."""
+        arr = (C.c_float * 3)()
+        self._chk(self._lib.fluid_rollout_timing(self._h, arr))
+        return {"replay": arr[0], "reverse": arr[1], "total": arr[2]}
+
 
 class SphBatch(SphSimulation3DGrid):
     """Many independent scenes advanced together (fluid_batch_create): one

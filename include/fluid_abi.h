@@ -382,6 +382,45 @@ int fluid_count_pairs(fluid_sim *sim, uint64_t *candidates,
  * out[2] max candidates of one particle, out[3] max in-range of one. */
 int fluid_pair_stats(fluid_sim *sim, uint64_t out[4]);
 
+/* ---- Differentiable rollout (reverse mode) -----------------------------
+ * No counterpart in the reference, whose step has no gradients; these entry
+ * points serve optimisation and inverse-design callers of the step of
+ * sph_3d_grid.rs:1342-1408.  The reverse pass differentiates every stage's
+ * arithmetic (integrate with its wall forces and backstop, finalize, each
+ * PCISPH iteration including the CFL clamp and max(0, rho* - rho0),
+ * predict, force, XSPH, density with its wall factor); the grid, neighbour
+ * sets and sort order are piecewise constant in the state and are not
+ * differentiated.  Mouse force and rigid body must be inactive. */
+
+/* Gradient slots per scene in dl_dparams: viscosity, surface tension,
+ * gravity x, y, z. */
+#define FLUID_GRAD_PARAMS 5
+
+/* Advance `steps` steps exactly as fluid_step_n does, keeping a state
+ * checkpoint every `checkpoint_every` steps (0: ceil(sqrt(steps))) for the
+ * reverse pass.  Device memory for the checkpoints and for one segment's
+ * tape (about 300 bytes per particle per step at 4 PCISPH iterations) is
+ * allocated here and reused. */
+int fluid_forward_rollout(fluid_sim *sim, uint32_t steps,
+                          uint32_t checkpoint_every);
+
+/* Reverse the last fluid_forward_rollout.  Inputs: dL/d(final positions)
+ * and dL/d(final velocities), 3 floats per particle in original index
+ * order (host or device memory; null = zero).  Outputs (each may be null):
+ * dL/d(initial positions), dL/d(initial velocities) in the same layout, and
+ * dl_dparams[FLUID_GRAD_PARAMS * num_scenes] (double).  Each checkpoint
+ * segment is replayed to record its tape, then reversed.  The handle's
+ * state is the rollout's final state again on return.  Fails with
+ * FLUID_ERR_STATE when the state, parameters or sweep configuration changed
+ * after the forward rollout. */
+int fluid_backward_rollout(fluid_sim *sim, const float *dl_dpos,
+                           const float *dl_dvel, float *dl_dpos0,
+                           float *dl_dvel0, double *dl_dparams);
+
+/* Timing of the last fluid_backward_rollout on the handle's stream (HIP
+ * events): ms[0] replays, ms[1] reverse kernels, ms[2] whole call. */
+int fluid_rollout_timing(const fluid_sim *sim, float ms[3]);
+
 #ifdef __cplusplus
 }
 #endif
