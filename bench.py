@@ -20,6 +20,10 @@ from pathlib import Path
 ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
+# torch carries its own HIP runtime library; it must be the first one loaded
+# in the process (libfluidhip.so then binds to it), or torch finds no device.
+import torch  # noqa: E402,F401
+
 import __graft_entry__ as ge  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
@@ -113,6 +117,117 @@ def run_batched(pkg, count: int, steps: int = 20):
             "autotuned": list(cfg), "stage_ms": {a: round(b, 4) for a, b in stages.items()}}
 
 
+def run_adjoint(pkg, orc, steps: int = 64, every: int = 16, fd_coords: int = 64,
+                fd_eps: float = 1e-3):
+    """Differentiable rollout of the 32,768-particle dam break (initial
+    velocities N(0, 0.1^2), seed 4): loss = MSE between the final
+    particle-splatted density on a 32^3 grid and the same simulator's from
+    seed 5.  Reports forward and backward (replay + reverse) times, the
+    gradient's growth with the horizon, and a central-difference check of
+    dL/dv0 on `fd_coords` random coordinates against the FP64 oracle at a
+    one-step horizon (longer horizons amplify the f32 / f64 trajectory
+    difference past the check's resolution: see `growth`)."""
+    import numpy as np
+    sys.path.insert(0, str(ROOT / "tests"))
+    import grad_harness as gh
+    sc, tg = pkg.scenes.dam_break_32k(4), pkg.scenes.dam_break_32k(5)
+    lo, hi = gh.container_box(sc)
+    dev = "cuda"
+
+    def target_at(T):
+        t = pkg.make_sim(tg)
+        t.step(T)
+        pos = t.read_particles()[:, 0:3]
+        t.close()
+        return gh.splat_density(torch.tensor(pos, dtype=torch.float64, device=dev), lo, hi)
+
+    sim = pkg.make_sim(sc)
+    t_plain = sim.time_steps(2, steps)  # graph-launched plain steps, HIP events
+    sim.write_particles(sc.particles)
+    fwd, bwd, tim = [], [], []
+    lam = None
+    target = target_at(steps)
+    for rep in range(3):
+        sim.write_particles(sc.particles)
+        sim.sync()
+        t0 = time.perf_counter()
+        sim.forward_rollout(steps, every)
+        sim.sync()
+        fwd.append((time.perf_counter() - t0) * 1e3)
+        if lam is None:
+            pos = torch.tensor(sim.read_particles()[:, 0:3], dtype=torch.float64, device=dev,
+                               requires_grad=True)
+            gh.splat_mse(pos, target, lo, hi).backward()
+            lam = pos.grad.float().cpu().numpy()
+        t0 = time.perf_counter()
+        _, gv, gp = sim.backward_rollout(lam, None)
+        bwd.append((time.perf_counter() - t0) * 1e3)
+        tim.append(sim.rollout_timing())
+    finite = bool(np.isfinite(gv).all() and np.isfinite(gp).all())
+    f_ms, b_ms = min(fwd), min(bwd)
+    best = min(tim, key=lambda d: d["total"])
+    # gradient growth with the horizon
+    growth = []
+    for T in (1, 2, 4, 8):
+        tgt = target_at(T)
+        sim.write_particles(sc.particles)
+        sim.forward_rollout(T, every)
+        pos = torch.tensor(sim.read_particles()[:, 0:3], dtype=torch.float64, device=dev,
+                           requires_grad=True)
+        gh.splat_mse(pos, tgt, lo, hi).backward()
+        _, g1, p1 = sim.backward_rollout(pos.grad.float().cpu().numpy(), None)
+        growth.append({"steps": T, "grad_v0_norm": float(np.linalg.norm(g1.astype(np.float64))),
+                       "dL_dviscosity": float(p1[0, 0]), "finite": bool(np.isfinite(g1).all())})
+        if T == 1:
+            g_one, tgt_one = g1.astype(np.float64).ravel(), tgt
+    sim.close()
+    # FD of the one-step loss in FP64 (oracle) on random coordinates of v0
+    params, cont, g = pkg.scenes.make_params(sc, pkg.abi, pkg.derive_sph_params,
+                                             pkg.derive_container_geometry)
+    base = sc.particles.astype(np.float64)
+    rng = np.random.default_rng(0)
+    coords = rng.choice(sc.n * 3, fd_coords, replace=False)
+
+    def loss64(parts):
+        o = orc.from_scene(sc, pkg.abi, params, cont, g, fp64=True, particles=parts)
+        o.step(1)
+        x = o.particles()[:, 0:3]
+        o.close()
+        with torch.no_grad():
+            return float(gh.splat_mse(torch.tensor(x, device=dev), tgt_one, lo, hi))
+
+    t0 = time.perf_counter()
+    fd = np.zeros(fd_coords)
+    for k, c in enumerate(coords):
+        i, a = divmod(int(c), 3)
+        vals = []
+        for sgn in (1.0, -1.0):
+            b = base.copy()
+            b[i, 4 + a] += sgn * fd_eps
+            vals.append(loss64(b))
+        fd[k] = (vals[0] - vals[1]) / (2 * fd_eps)
+    ad = g_one[coords]
+    rel = np.abs(ad - fd) / np.maximum(np.abs(fd), 1e-30)
+    return {"workload": "dam_break_32k splat-MSE (32^3), seed 4 vs target seed 5",
+            "particles": sc.n, "steps": steps, "checkpoint_every": every,
+            "plain_steps_ms": round(t_plain, 3),
+            "forward_rollout_ms": round(f_ms, 3),
+            "backward_ms": round(b_ms, 3),
+            "backward_replay_ms": round(best["replay"], 3),
+            "backward_reverse_ms": round(best["reverse"], 3),
+            "adjoint_over_forward": round(b_ms / f_ms, 3),
+            "reverse_only_over_forward": round(best["reverse"] / f_ms, 3),
+            "gradient_finite_at_full_horizon": finite,
+            "dL_dstiffness": 0.0,
+            "growth": growth,
+            "fd_check": {"horizon_steps": 1, "eps": fd_eps, "coords": fd_coords,
+                         "reference": "FP64 oracle, central differences",
+                         "normwise_rel_err": float(np.linalg.norm(ad - fd) / np.linalg.norm(fd)),
+                         "median_rel_err": float(np.median(rel)),
+                         "within_1e-3": int((rel <= 1e-3).sum()),
+                         "seconds": round(time.perf_counter() - t0, 1)}}
+
+
 def cpu_baseline(pkg, orc, scene, budget_s: float = 12.0, max_steps: int = 40):
     params, cont, g = pkg.scenes.make_params(scene, pkg.abi, pkg.derive_sph_params,
                                              pkg.derive_container_geometry)
@@ -142,12 +257,13 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-form comparison")
     ap.add_argument("--batch-scenes", type=int, default=256)
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the differentiable rollout")
+    ap.add_argument("--fd-coords", type=int, default=64)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a 256 MiB device copy (scales the FETCH_SIZE/WRITE_SIZE "
                          "counters of a profiling pass to bytes)")
     args = ap.parse_args()
     if args.calibrate_copy:
-        import torch
         x = torch.ones(64 << 20, dtype=torch.float32, device="cuda")
         y = torch.empty_like(x)
         for _ in range(5):
@@ -201,6 +317,8 @@ def main() -> int:
                             "roofline": roof1}
     if not args.no_batch:
         line["batched"] = run_batched(pkg, args.batch_scenes)
+    if not args.no_adjoint:
+        line["adjoint"] = run_adjoint(pkg, ge.load_oracle(), fd_coords=args.fd_coords)
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(pkg, ge.load_oracle(), scene)
     print(json.dumps(line), flush=True)

@@ -174,6 +174,8 @@ PROTOTYPES = {
     "fluid_set_sweep_config": (C.c_int, [P, C.POINTER(u32)]),
     "fluid_get_sweep_config": (C.c_int, [P, C.POINTER(u32)]),
     "fluid_autotune": (C.c_int, [P, u32, C.POINTER(u32)]),
+    "fluid_set_tile_config": (C.c_int, [P, C.POINTER(u32)]),
+    "fluid_get_tile_config": (C.c_int, [P, C.POINTER(u32)]),
     "fluid_forward_rollout": (C.c_int, [P, u32, u32]),
     "fluid_backward_rollout": (C.c_int, [P, P, P, P, P, P]),
     "fluid_rollout_timing": (C.c_int, [P, C.POINTER(C.c_float)]),

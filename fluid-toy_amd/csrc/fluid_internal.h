@@ -99,6 +99,19 @@ struct Uni {
   const uint16_t *scene_of;  // null for a single scene
 };
 
+// Cell tiles of the LDS-staged sweeps (see sph_device.h).
+constexpr uint32_t kTileWMax = 16;
+struct TileMap {
+  const uint32_t *scene_first;  // first tile id of each scene, nscenes + 1 entries
+  uint32_t nscenes;
+  uint32_t total;               // tiles of all scenes
+  uint32_t w;                   // cells per tile along x
+  uint32_t cap;                 // staged entries per workgroup
+  uint32_t chunk;               // particles per work item (256 / lanes)
+  uint4 *list;                  // work items: (scene, scene-local tile, chunk, 0)
+  uint32_t *count;              // entries in list
+};
+
 struct PredBuf {
   float4 *xq;
   float2 *pr;
@@ -121,6 +134,7 @@ struct Bufs {
   uint32_t *err;
   int32_t *accum;
   const float *sdf;
+  TileMap tiles;
 };
 
 // Scan tiling: 256 threads x 16 cells (long look-back chains are the scan's
@@ -141,6 +155,10 @@ struct SweepCfg {
   int solve_lanes;  // lanes per particle in the PCISPH solve
   int xcd_tiles;    // 1: each XCD takes one contiguous run of tiles;
                     // 0: hardware round-robin order
+  int tiled;        // 1: density and solve stage cell tiles in LDS, both with
+                    // `lanes` lanes per particle, 256 / lanes particles per
+                    // work item (Bufs::tiles.chunk)
+  int tile_wgs;     // persistent workgroups of the tiled sweeps
 };
 
 // Host-side launchers (sph_kernels.hip).  All enqueue on `st`.
@@ -148,7 +166,8 @@ hipError_t launch_clear(const Bufs &b, uint32_t cells, uint32_t ntiles,
                         hipStream_t st);
 hipError_t launch_build(const Uni &u, const Bufs &b, uint32_t n,
                         hipStream_t st);
-hipError_t launch_scan(const Uni &u, const Bufs &b, uint32_t cells,
+// With c.tiled, also lists the non-empty cell tiles (b.tiles).
+hipError_t launch_scan(const Uni &u, const Bufs &b, uint32_t cells, SweepCfg c,
                        hipStream_t st);
 hipError_t launch_reorder(const Uni &u, const Bufs &b, uint32_t n,
                           hipStream_t st);

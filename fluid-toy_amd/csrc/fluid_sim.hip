@@ -87,7 +87,10 @@ struct fluid_sim {
   std::vector<float> extra;
   hipGraphExec_t graph = nullptr;
   uint32_t graph_n = 0, graph_iters = 0;
-  int graph_lanes = 0, graph_solve_lanes = 0, graph_xcd = 0;
+  int graph_lanes = 0, graph_solve_lanes = 0, graph_xcd = 0, graph_tiled = 0, graph_wgs = 0;
+  uint32_t graph_tile_w = 0;
+  uint32_t num_cu = 256;
+  uint32_t *d_scene_first = nullptr;  // first tile id of each scene (+ total)
   int cursor = -1;  // last completed stage of a partial step, -1 = none
   GpuRigidBodyAccum last_accum{};
   SweepCfg cfg{8, 8, 0};
@@ -114,12 +117,35 @@ void *dev_alloc(fluid_sim *s, size_t bytes, hipError_t *err) {
 SweepCfg pick_cfg(uint32_t) {
   // Default before tuning: 8 lanes per particle, the fastest setting the
   // autotuner found at every size measured (8k to 1M particles, compressed
-  // and relaxed states).
-  return SweepCfg{8, 8, 0};
+  // and relaxed states); global-memory sweeps.
+  return SweepCfg{8, 8, 0, 0, 1024};
+}
+
+// Per-scene tile counts for tiles of w cells along x; uploads the scene
+// offsets and sets b.tiles (the list buffer holds one entry per cell, the
+// w = 1 bound).
+int setup_tiles(fluid_sim *s, uint32_t w) {
+  std::vector<uint32_t> first(s->sc.size() + 1, 0);
+  uint64_t total = 0;
+  for (size_t k = 0; k < s->sc.size(); ++k) {
+    const GpuGridParams &g = s->sc[k].grid;
+    first[k] = (uint32_t)total;
+    total += (uint64_t)g.grid_size_y * g.grid_size_z * ((g.grid_size_x + w - 1) / w);
+  }
+  first[s->sc.size()] = (uint32_t)total;
+  HIP_TRY(hipMemcpy(s->d_scene_first, first.data(), first.size() * 4, hipMemcpyHostToDevice));
+  s->b.tiles.scene_first = s->d_scene_first;
+  s->b.tiles.nscenes = (uint32_t)s->sc.size();
+  s->b.tiles.total = (uint32_t)total;
+  s->b.tiles.w = w;
+  return FLUID_OK;
 }
 
 void update_cfg(fluid_sim *s) {
-  if (!s->tuned) s->cfg = pick_cfg(s->n);
+  if (s->tuned) return;
+  const int wgs = s->cfg.tile_wgs;
+  s->cfg = pick_cfg(s->n);
+  s->cfg.tile_wgs = wgs;
 }
 
 int push_uniforms(fluid_sim *s) {
@@ -146,6 +172,7 @@ int push_uniforms(fluid_sim *s) {
 // writes the predicted state) and finalize + integrate.
 int enqueue_range(fluid_sim *s, uint32_t first, uint32_t last) {
   s->roll.valid = false;
+  s->b.tiles.chunk = 256u / (uint32_t)s->cfg.lanes;
   const uint32_t n = s->n;
   hipStream_t st = s->stream;
   const Bufs &b = s->b;
@@ -162,7 +189,7 @@ int enqueue_range(fluid_sim *s, uint32_t first, uint32_t last) {
         HIP_TRY(launch_build(u, b, n, st));
         break;
       case FLUID_STAGE_SCAN:
-        HIP_TRY(launch_scan(u, b, s->total_cells, st));
+        HIP_TRY(launch_scan(u, b, s->total_cells, s->cfg, st));
         break;
       case FLUID_STAGE_REORDER:
         HIP_TRY(launch_reorder(u, b, n, st));
@@ -209,7 +236,8 @@ void drop_graph(fluid_sim *s) {
 int ensure_graph(fluid_sim *s) {
   if (s->graph && s->graph_n == s->n && s->graph_iters == s->iters &&
       s->graph_lanes == s->cfg.lanes && s->graph_solve_lanes == s->cfg.solve_lanes &&
-      s->graph_xcd == s->cfg.xcd_tiles)
+      s->graph_xcd == s->cfg.xcd_tiles && s->graph_tiled == s->cfg.tiled &&
+      s->graph_wgs == s->cfg.tile_wgs && s->graph_tile_w == s->b.tiles.w)
     return FLUID_OK;
   drop_graph(s);
   hipGraph_t g = nullptr;
@@ -232,6 +260,9 @@ int ensure_graph(fluid_sim *s) {
   s->graph_lanes = s->cfg.lanes;
   s->graph_solve_lanes = s->cfg.solve_lanes;
   s->graph_xcd = s->cfg.xcd_tiles;
+  s->graph_tiled = s->cfg.tiled;
+  s->graph_wgs = s->cfg.tile_wgs;
+  s->graph_tile_w = s->b.tiles.w;
   return FLUID_OK;
 }
 
@@ -458,6 +489,7 @@ int enqueue_tape_step(fluid_sim *s, const TapeSlot &T, float4 *scratch_v) {
   const Uni u = s->uni();
   hipStream_t st = s->stream;
   Bufs rb = s->b;
+  rb.tiles.chunk = 256u / (uint32_t)c.lanes;
   rb.spos = T.x;
   rb.svel = T.u;
   rb.srec = T.rec;
@@ -466,7 +498,7 @@ int enqueue_tape_step(fluid_sim *s, const TapeSlot &T, float4 *scratch_v) {
   rb.gmap = T.gmap;
   HIP_TRY(launch_clear(rb, s->total_cells, s->ntiles, st));
   HIP_TRY(launch_build(u, rb, n, st));
-  HIP_TRY(launch_scan(u, rb, s->total_cells, st));
+  HIP_TRY(launch_scan(u, rb, s->total_cells, c, st));
   HIP_TRY(launch_reorder(u, rb, n, st));
   HIP_TRY(launch_density(u, rb, n, c, st));
   Bufs mb = rb;  // XSPH's output w stays in the tape
@@ -605,7 +637,22 @@ static int create_handle(int device, std::vector<SceneState> scenes,
   ALLOC(tile_counter, uint32_t *, 1);
   ALLOC(err, uint32_t *, 1);
   ALLOC(accum, int32_t *, 8);
+  // work items: at most one per chunk of particles plus one per tile
+  ALLOC(tiles.list, uint4 *, N + std::max<size_t>(C, 1));
+  ALLOC(tiles.count, uint32_t *, 1);
 #undef ALLOC
+  s->d_scene_first = (uint32_t *)dev_alloc(s, (S + 1) * 4, &e);
+  if (e != hipSuccess) return bail(fail(FLUID_ERR_HIP, "hipMalloc tile offsets"));
+  s->b.tiles.cap = 2048;
+  {
+    int cus = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, s->device) ==
+            hipSuccess && cus > 0)
+      s->num_cu = (uint32_t)cus;
+    s->cfg.tile_wgs = (int)(4 * s->num_cu);
+  }
+  rc = setup_tiles(s, 4);
+  if (rc) return bail(rc);
   s->d_pairs = (unsigned long long *)dev_alloc(s, 32, &e);
   if (e != hipSuccess) return bail(fail(FLUID_ERR_HIP, "hipMalloc pairs"));
   s->d_u = (DevUniforms *)dev_alloc(s, S * sizeof(DevUniforms), &e);
@@ -1111,6 +1158,32 @@ int fluid_set_sweep_config(fluid_sim *s, const uint32_t cfg[3]) {
   return FLUID_OK;
 }
 
+int fluid_set_tile_config(fluid_sim *s, const uint32_t cfg[3]) {
+  if (!s || !cfg) return fail(FLUID_ERR_INVALID_ARG, "null argument");
+  if (cfg[0] > 1) return fail(FLUID_ERR_INVALID_ARG, "tiled must be 0 or 1");
+  if (cfg[1] > kTileWMax) return fail(FLUID_ERR_INVALID_ARG, "tile width above 16 cells");
+  if (s->cursor != -1) return fail(FLUID_ERR_STATE, "a partial step is in progress");
+  HIP_TRY(hipSetDevice(s->device));
+  s->cfg.tiled = (int)cfg[0];
+  if (cfg[1] && cfg[1] != s->b.tiles.w) {
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    const int rc = setup_tiles(s, cfg[1]);
+    if (rc) return rc;
+  }
+  if (cfg[2]) s->cfg.tile_wgs = (int)cfg[2];
+  s->tuned = true;
+  s->roll.valid = false;
+  return FLUID_OK;
+}
+
+int fluid_get_tile_config(const fluid_sim *s, uint32_t out[3]) {
+  if (!s || !out) return fail(FLUID_ERR_INVALID_ARG, "null argument");
+  out[0] = (uint32_t)s->cfg.tiled;
+  out[1] = s->b.tiles.w;
+  out[2] = (uint32_t)s->cfg.tile_wgs;
+  return FLUID_OK;
+}
+
 int fluid_get_sweep_config(const fluid_sim *s, uint32_t out[3]) {
   if (!s || !out) return fail(FLUID_ERR_INVALID_ARG, "null argument");
   out[0] = (uint32_t)s->cfg.lanes;
@@ -1149,18 +1222,32 @@ int fluid_autotune(fluid_sim *s, uint32_t trials, uint32_t out[3]) {
   HIP_TRY(hipMemcpyAsync(accum_keep, s->b.accum, 32, hipMemcpyDeviceToHost, s->stream));
   HIP_TRY(hipStreamSynchronize(s->stream));
   const SweepCfg keep = s->cfg;
+  const uint32_t keep_w = s->b.tiles.w;
   const int lanes[4] = {1, 2, 4, 8};
-  float best_list = 1e30f, best_solve = 1e30f;
+  // Sweep modes: global gathers with either workgroup-to-XCD mapping, and
+  // LDS-staged tiles of 2, 4 or 8 cells.  Within a mode, lanes per particle
+  // for density/XSPH/force and for the solve are timed independently.
+  struct Mode { int tiled, xcd; uint32_t w; };
+  const Mode modes[5] = {{0, 0, 0}, {0, 1, 0}, {1, 0, 2}, {1, 0, 4}, {1, 0, 8}};
+  float best_total = 1e30f;
   SweepCfg best = keep;
+  uint32_t best_w = keep_w;
   int rc = FLUID_OK;
   hipError_t e = copy(true);
-  // Lanes per particle (density/XSPH/force and solve timed independently)
-  // crossed with the two workgroup-to-XCD mappings.
-  for (int xi = 0; xi < 2 && e == hipSuccess && rc == FLUID_OK; ++xi) {
+  for (int mi = 0; mi < 5 && e == hipSuccess && rc == FLUID_OK; ++mi) {
+    const Mode &m = modes[mi];
+    if (m.tiled && m.w != s->b.tiles.w) {
+      e = hipStreamSynchronize(s->stream);
+      if (e != hipSuccess || (rc = setup_tiles(s, m.w)) != FLUID_OK) break;
+    }
+    float ml = 1e30f, ms_solve = 1e30f;
+    SweepCfg mc = keep;
+    mc.tiled = m.tiled;
+    mc.xcd_tiles = m.xcd;
     for (int li = 0; li < 4 && e == hipSuccess && rc == FLUID_OK; ++li) {
+      s->cfg = mc;
       s->cfg.lanes = lanes[li];
       s->cfg.solve_lanes = lanes[li];
-      s->cfg.xcd_tiles = xi;
       float ms[FLUID_STAGE_COUNT];
       float tl = 0.f, ts = 0.f;
       for (uint32_t t = 0; t < trials && rc == FLUID_OK; ++t) {
@@ -1169,10 +1256,27 @@ int fluid_autotune(fluid_sim *s, uint32_t trials, uint32_t out[3]) {
         tl += ms[FLUID_STAGE_DENSITY] + ms[FLUID_STAGE_XSPH] + ms[FLUID_STAGE_FORCE];
         ts += ms[FLUID_STAGE_SOLVE];
       }
-      if (tl + ts < best_list + best_solve) best.xcd_tiles = xi;
-      if (tl < best_list) { best_list = tl; best.lanes = lanes[li]; }
-      if (ts < best_solve) { best_solve = ts; best.solve_lanes = lanes[li]; }
+      if (m.tiled) {
+        // one lane count for density, XSPH, force and solve
+        if (tl + ts < ml + ms_solve) {
+          ml = tl;
+          ms_solve = ts;
+          mc.lanes = mc.solve_lanes = lanes[li];
+        }
+      } else {
+        if (tl < ml) { ml = tl; mc.lanes = lanes[li]; }
+        if (ts < ms_solve) { ms_solve = ts; mc.solve_lanes = lanes[li]; }
+      }
     }
+    if (ml + ms_solve < best_total) {
+      best_total = ml + ms_solve;
+      best = mc;
+      best_w = m.tiled ? m.w : keep_w;
+    }
+  }
+  if (e == hipSuccess && rc == FLUID_OK && best_w != s->b.tiles.w) {
+    e = hipStreamSynchronize(s->stream);
+    if (e == hipSuccess) rc = setup_tiles(s, best_w);
   }
   if (e == hipSuccess) e = copy(false);
   if (e == hipSuccess)
@@ -1180,12 +1284,10 @@ int fluid_autotune(fluid_sim *s, uint32_t trials, uint32_t out[3]) {
   if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
   (void)hipFree(save);
   (void)hipFree(save_sid);
-  if (rc) {
+  if (rc || e != hipSuccess) {
     s->cfg = keep;
-    return rc;
-  }
-  if (e != hipSuccess) {
-    s->cfg = keep;
+    if (s->b.tiles.w != keep_w) (void)setup_tiles(s, keep_w);
+    if (rc) return rc;
     return fail(FLUID_ERR_HIP, std::string("autotune: ") + hipGetErrorString(e));
   }
   s->cfg = best;

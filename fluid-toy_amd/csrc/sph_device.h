@@ -122,4 +122,126 @@ __device__ __forceinline__ void sweep27(const DevUniforms *__restrict__ U,
   }
 }
 
+// ------------------------------------------------------------------------
+// Cell tiles.  A tile is a run of up to w (<= kTileWMax) cells along x of
+// one grid row (y, z) of one scene.  Its particles' 27-cell stencils cover
+// nine x-rows (y-1..y+1, z-1..z+1) of cells xa = x0-1 .. xb = x1+1, each a
+// contiguous range of sorted slots; a workgroup stages those ranges in LDS
+// once and every particle of the tile sweeps its candidates from there.
+struct TileShared {
+  uint32_t goff[9][kTileWMax + 3];  // slot where each row cell starts; [nx] = row end
+  uint32_t rowl[10];                // staged index of each row's first slot; [9] = total
+  int x0, x1, y, z, xa, xb;
+  uint32_t own_lo, own_hi;          // the tile's own slots
+};
+
+// Decode scene-local tile `lt` and fill S (all threads of the workgroup
+// call this; it synchronises).  Rows outside the grid have length 0.
+__device__ __forceinline__ void tile_setup(const DevUniforms *__restrict__ U,
+                                           const uint32_t *__restrict__ counts,
+                                           const uint32_t *__restrict__ starts, uint32_t lt,
+                                           uint32_t w, TileShared &S) {
+  const int gx = (int)U->gx, gy = (int)U->gy, gz = (int)U->gz;
+  const uint32_t nxc = ((uint32_t)gx + w - 1) / w;
+  const int xc = (int)(lt % nxc), row = (int)(lt / nxc);
+  const int y = row % gy, z = row / gy;
+  const int x0 = xc * (int)w, x1 = min(x0 + (int)w, gx) - 1;
+  const int xa = max(x0 - 1, 0), xb = min(x1 + 1, gx - 1);
+  const int nx = xb - xa + 1;
+  for (int t = (int)threadIdx.x; t < 9 * (nx + 1); t += (int)blockDim.x) {
+    const int r = t / (nx + 1), k = t % (nx + 1);
+    const int yy = y + (r % 3) - 1, zz = z + (r / 3) - 1;
+    uint32_t v = 0;
+    if (yy >= 0 && yy < gy && zz >= 0 && zz < gz) {
+      const uint32_t rc = U->cell_offset + (uint32_t)yy * U->gx + (uint32_t)zz * U->gxy;
+      v = k < nx ? starts[rc + xa + k] - counts[rc + xa + k] : starts[rc + xb];
+    }
+    S.goff[r][k] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t acc = 0;
+    for (int r = 0; r < 9; ++r) {
+      S.rowl[r] = acc;
+      acc += S.goff[r][nx] - S.goff[r][0];
+    }
+    S.rowl[9] = acc;
+    S.x0 = x0;
+    S.x1 = x1;
+    S.y = y;
+    S.z = z;
+    S.xa = xa;
+    S.xb = xb;
+    S.own_lo = S.goff[4][x0 - xa];
+    S.own_hi = S.goff[4][x1 - xa + 1];
+  }
+  __syncthreads();
+}
+
+// Copy the nine rows of src into dst (LDS) when they fit in `cap`
+// entries; returns whether they were staged.
+template <class T>
+__device__ __forceinline__ bool tile_stage(const TileShared &S, const T *__restrict__ src,
+                                           T *dst, uint32_t cap) {
+  const uint32_t total = S.rowl[9];
+  if (total > cap) return false;
+  for (uint32_t i = threadIdx.x; i < total; i += blockDim.x) {
+    int r = 0;
+    while (i >= S.rowl[r + 1]) ++r;
+    dst[i] = src[S.goff[r][0] + (i - S.rowl[r])];
+  }
+  __syncthreads();
+  return true;
+}
+
+// Whether a particle with cell coordinates (cx, cy, cz) sweeps this tile's
+// rows: it lies in one of the tile's own cells (a particle outside the grid
+// lives in a border cell but sweeps around its own coordinates).
+__device__ __forceinline__ bool tile_owns(const TileShared &S, const DevUniforms *U, int cx,
+                                          int cy, int cz) {
+  cx = min(max(cx, -2), (int)U->gx + 1);
+  cy = min(max(cy, -2), (int)U->gy + 1);
+  cz = min(max(cz, -2), (int)U->gz + 1);
+  return cy == S.y && cz == S.z && cx >= S.x0 && cx <= S.x1;
+}
+
+// The same walk as sweep27 (same rows, same trips, same candidate order per
+// lane) over a tile's rows; pts is the staged copy (staged = true, indexed
+// from 0) or the global array (indexed by slot).  f(j, p_j, valid) receives
+// the global slot j.
+template <int L, bool SYNC, class T, class F>
+__device__ __forceinline__ void tile_sweep(const TileShared &S, const T *pts, bool staged,
+                                           int cx, uint32_t sub, F &&f) {
+  const int lo = max(cx - 1, S.xa) - S.xa, hi = min(cx + 1, S.xb) - S.xa + 1;
+  for (int r = 0; r < 9; ++r) {
+    const uint32_t begin = S.goff[r][lo], end = S.goff[r][hi];
+    if (begin >= end) continue;
+    // staged index of slot j: j + off (mod 2^32) = rowl[r] + (j - goff[r][0])
+    const uint32_t off = staged ? S.rowl[r] - S.goff[r][0] : 0u;
+    if (SYNC) {
+      const uint32_t last = end - 1;
+      for (uint32_t base = begin; base < end; base += 4 * L) {
+        const uint32_t j0 = base + sub, j1 = j0 + L, j2 = j1 + L, j3 = j2 + L;
+        const T p0 = pts[min(j0, last) + off], p1 = pts[min(j1, last) + off];
+        const T p2 = pts[min(j2, last) + off], p3 = pts[min(j3, last) + off];
+        f(j0, p0, j0 < end);
+        f(j1, p1, j1 < end);
+        f(j2, p2, j2 < end);
+        f(j3, p3, j3 < end);
+      }
+    } else {
+      uint32_t j = begin + sub;
+      for (; j + 3 * L < end; j += 4 * L) {
+        const T p0 = pts[j + off], p1 = pts[j + L + off];
+        const T p2 = pts[j + 2 * L + off], p3 = pts[j + 3 * L + off];
+        f(j, p0, true);
+        f(j + L, p1, true);
+        f(j + 2 * L, p2, true);
+        f(j + 3 * L, p3, true);
+      }
+      for (; j < end; j += L) f(j, pts[j + off], true);
+    }
+  }
+}
+
 }  // namespace fluid

@@ -32,13 +32,17 @@ __global__ __launch_bounds__(256) void k_clear(uint32_t *__restrict__ counts,
                                                uint32_t cells,
                                                unsigned long long *status,
                                                uint32_t ntiles,
-                                               uint32_t *tile_counter) {
+                                               uint32_t *tile_counter,
+                                               uint32_t *cell_tile_count) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   const uint32_t q = cells / 4;
   if (i < q) reinterpret_cast<uint4 *>(counts)[i] = make_uint4(0, 0, 0, 0);
   if (i < cells - 4 * q) counts[4 * q + i] = 0u;
   if (i < ntiles) status[i] = 0ull;
-  if (i == 0) *tile_counter = 0u;
+  if (i == 0) {
+    *tile_counter = 0u;
+    if (cell_tile_count) *cell_tile_count = 0u;
+  }
 }
 
 // ------------------------------------------------------------------------
@@ -167,6 +171,46 @@ __global__ __launch_bounds__(256) void k_scan(Bufs b, uint32_t cells) {
 }
 
 // ------------------------------------------------------------------------
+// After the scan: the work items of the LDS-staged sweeps.  Every
+// non-empty cell tile of every scene contributes one item per `chunk` of
+// its own particles (dense tiles spread over several workgroups, each
+// staging the same rows), appended with one atomic per wave.  The list order
+// is arbitrary; items are processed independently of each other.
+template <bool B>
+__global__ __launch_bounds__(256) void k_tiles(Uni un, Bufs b) {
+  const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+  const TileMap &tm = b.tiles;
+  uint32_t items = 0, scene = 0, lt = 0;
+  if (g < tm.total) {
+    if (B) {
+      uint32_t lo = 0, hi = tm.nscenes;  // last scene with scene_first <= g
+      while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) / 2;
+        if (tm.scene_first[mid] <= g) lo = mid; else hi = mid;
+      }
+      scene = lo;
+    }
+    lt = g - tm.scene_first[scene];
+    const DevUniforms *U = un.u + scene;
+    const uint32_t gx = U->gx, gy = U->gy;
+    const uint32_t nxc = (gx + tm.w - 1) / tm.w;
+    const uint32_t xc = lt % nxc, row = lt / nxc;
+    const uint32_t y = row % gy, z = row / gy;
+    const uint32_t x0 = xc * tm.w, x1 = min(x0 + tm.w, gx) - 1;
+    const uint32_t rc = U->cell_offset + y * gx + z * U->gxy;
+    const uint32_t own = b.starts[rc + x1] - (b.starts[rc + x0] - b.counts[rc + x0]);
+    items = (own + tm.chunk - 1) / tm.chunk;
+  }
+  const uint32_t incl = wave_incl_scan(items);
+  const uint32_t lane = threadIdx.x & 63u;
+  const uint32_t total = (uint32_t)__shfl((int)incl, 63);
+  if (total == 0) return;
+  uint32_t base = 0;
+  if (lane == 63) base = atomicAdd(tm.count, total);
+  base = (uint32_t)__shfl((int)base, 63) + incl - items;
+  for (uint32_t k = 0; k < items; ++k) tm.list[base + k] = make_uint4(scene, lt, k, 0u);
+}
+
 // Stage 3 — grid_reorder.wgsl:37-63 as a deterministic counting sort.
 // (a) scatter (old slot, original index) to cell start + provisional slot;
 __global__ __launch_bounds__(256) void k_scatter(Bufs b, uint32_t n) {
@@ -227,16 +271,12 @@ __global__ __launch_bounds__(256) void k_rank_gather(Bufs b, uint32_t n) {
 // in-range neighbours it met (0 < r^2 < h^2, the pairs XSPH and force sum
 // over) in its list, and the particle's neighbour record is written for the
 // later gathers.
-template <int L, bool B>
-__global__ __launch_bounds__(256) void k_density(Uni un, Bufs b, uint32_t n, uint32_t xcd) {
-  const uint32_t g = xcd_block(xcd) * blockDim.x + threadIdx.x;
-  const uint32_t s = g / L, sub = g % L;
-  if (s >= n) return;
-  const DevUniforms *__restrict__ U = uni_at<B>(un, s);
-  const float4 pi = b.spos[s];
-  const int cx = cell_coord(pi.x, U->ox, U->inv_cell);
-  const int cy = cell_coord(pi.y, U->oy, U->inv_cell);
-  const int cz = cell_coord(pi.z, U->oz, U->inv_cell);
+// Per-particle body of the density stage; sweep(f) walks the particle's
+// candidates group-synchronously (sweep27 or tile_sweep with SYNC).
+template <int L, class Sweep>
+__device__ __forceinline__ void density_particle(const DevUniforms *__restrict__ U,
+                                                 const Bufs &b, uint32_t n, uint32_t s,
+                                                 uint32_t sub, float4 pi, Sweep &&sweep) {
   const float h = U->h, h2 = U->h2, mass = U->mass;
   const float ps = U->poly6_scale, ns = U->near_scale, mr = U->m_over_rho0;
   const size_t stride = (size_t)n * L;
@@ -251,8 +291,7 @@ __global__ __launch_bounds__(256) void k_density(Uni un, Bufs b, uint32_t n, uin
   // k-th recorded pair (ranked across lanes by a ballot, in a fixed order)
   // goes to lane k % L's sub-list at depth k / L, so every lane of XSPH and
   // force later walks an equal share.
-  sweep27<L, true>(U, b.counts, b.starts, b.spos, cx, cy, cz, sub,
-                   [&](uint32_t j, float4 pj, bool valid) {
+  sweep([&](uint32_t j, float4 pj, bool valid) {
     const float rx = pi.x - pj.x, ry = pi.y - pj.y, rz = pi.z - pj.z;
     const float r2 = dot3(rx, ry, rz, rx, ry, rz);
     const bool inr = valid && r2 < h2;
@@ -297,6 +336,60 @@ __global__ __launch_bounds__(256) void k_density(Uni un, Bufs b, uint32_t n, uin
   rec[0] = make_float4(pi.x, pi.y, pi.z, dens / gamma);
   rec[1] = make_float4(vi.x, vi.y, vi.z, near);
   rec[2] = make_float4(nx * sc, ny * sc, nz * sc, 0.f);
+}
+
+template <int L, bool B>
+__global__ __launch_bounds__(256) void k_density(Uni un, Bufs b, uint32_t n, uint32_t xcd) {
+  const uint32_t g = xcd_block(xcd) * blockDim.x + threadIdx.x;
+  const uint32_t s = g / L, sub = g % L;
+  if (s >= n) return;
+  const DevUniforms *__restrict__ U = uni_at<B>(un, s);
+  const float4 pi = b.spos[s];
+  const int cx = cell_coord(pi.x, U->ox, U->inv_cell);
+  const int cy = cell_coord(pi.y, U->oy, U->inv_cell);
+  const int cz = cell_coord(pi.z, U->oz, U->inv_cell);
+  density_particle<L>(U, b, n, s, sub, pi, [&](auto &&f) {
+    sweep27<L, true>(U, b.counts, b.starts, b.spos, cx, cy, cz, sub, f);
+  });
+}
+
+// Tiled form: persistent workgroups take non-empty cell tiles in turn, stage
+// the tile's nine candidate rows in LDS and run the same per-particle body
+// over them (bitwise the same sums as k_density).  Particles whose stencil
+// is not the tile's (outside the grid) and tiles too dense to stage sweep
+// from global memory.
+template <int L, bool B>
+__global__ __launch_bounds__(256) void k_density_t(Uni un, Bufs b, uint32_t n) {
+  extern __shared__ float4 stage[];
+  __shared__ TileShared S;
+  const uint32_t count = *b.tiles.count;
+  for (uint32_t t = blockIdx.x; t < count; t += gridDim.x) {
+    const uint4 e = b.tiles.list[t];
+    const DevUniforms *__restrict__ U = B ? un.u + e.x : un.u;
+    tile_setup(U, b.counts, b.starts, e.y, b.tiles.w, S);
+    const bool staged = tile_stage(S, b.spos, stage, b.tiles.cap);
+    const uint32_t first = S.own_lo + e.z * b.tiles.chunk;
+    const uint32_t own = min(S.own_hi - first, b.tiles.chunk);
+    for (uint32_t p0 = 0; p0 < own; p0 += 256 / L) {
+      const uint32_t p = p0 + threadIdx.x / L, sub = threadIdx.x % L;
+      if (p >= own) continue;
+      const uint32_t s = first + p;
+      const float4 pi = b.spos[s];
+      const int cx = cell_coord(pi.x, U->ox, U->inv_cell);
+      const int cy = cell_coord(pi.y, U->oy, U->inv_cell);
+      const int cz = cell_coord(pi.z, U->oz, U->inv_cell);
+      if (staged && tile_owns(S, U, cx, cy, cz)) {
+        density_particle<L>(U, b, n, s, sub, pi, [&](auto &&f) {
+          tile_sweep<L, true>(S, stage, true, cx, sub, f);
+        });
+      } else {
+        density_particle<L>(U, b, n, s, sub, pi, [&](auto &&f) {
+          sweep27<L, true>(U, b.counts, b.starts, b.spos, cx, cy, cz, sub, f);
+        });
+      }
+    }
+    __syncthreads();
+  }
 }
 
 // Direction of a pair closer than 0.1 h: normalize(sin(d), cos(2d), sin(3d))
@@ -522,20 +615,15 @@ __global__ __launch_bounds__(256) void k_predict(Uni un, Bufs b, uint32_t n) {
 // ------------------------------------------------------------------------
 // Stage 8 — pcisph_solve.wgsl:172-208, one iteration, in -> out.  The
 // neighbour's p / max(rho*,1)^2 is precomputed by the producing iteration
-// (same operations as the reference evaluates per pair).
-template <int L, bool B>
-__global__ __launch_bounds__(256) void k_solve(Uni un, Bufs b, uint32_t n, PredBuf in,
-                                               PredBuf out, float4 *ap_out, uint32_t xcd,
-                                               uint32_t first_iter) {
-  const uint32_t g = xcd_block(xcd) * blockDim.x + threadIdx.x;
-  const uint32_t s = g / L, sub = g % L;
-  if (s >= n) return;
-  const DevUniforms *__restrict__ U = uni_at<B>(un, s);
-  const float4 xi = in.xq[s];
-  const float4 p0 = b.spos[s];  // current position (grid lookup)
-  const int cx = cell_coord(p0.x, U->ox, U->inv_cell);
-  const int cy = cell_coord(p0.y, U->oy, U->inv_cell);
-  const int cz = cell_coord(p0.z, U->oz, U->inv_cell);
+// (same operations as the reference evaluates per pair).  sweep(f) walks
+// the candidates of the particle's cell (the grid of the current positions)
+// over the iteration's predicted positions, lane-independently.
+template <int L, class Sweep>
+__device__ __forceinline__ void solve_particle(const DevUniforms *__restrict__ U, const Bufs &b,
+                                               const PredBuf &in, const PredBuf &out,
+                                               float4 *ap_out, uint32_t first_iter, uint32_t s,
+                                               uint32_t sub, float4 xi, float4 p0,
+                                               Sweep &&sweep) {
   const float h = U->h, h2 = U->h2, mass = U->mass, ps = U->poly6_scale;
   const float gs = U->grad_scale;
   float pd = 0.f, ax = 0.f, ay = 0.f, az = 0.f;
@@ -543,32 +631,30 @@ __global__ __launch_bounds__(256) void k_solve(Uni un, Bufs b, uint32_t n, PredB
     // First iteration: every pressure of the input is 0 (pcisph_predict.wgsl
     // :69), so each pressure term is a signed zero and a_total = a_np
     // exactly; only the predicted density needs the sum.
-    sweep27<L, false>(U, b.counts, b.starts, in.xq, cx, cy, cz, sub,
-                      [&](uint32_t, float4 xj, bool) {
+    sweep([&](uint32_t, float4 xj, bool) {
       const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
       const float r2 = dot3(rx, ry, rz, rx, ry, rz);
       const float d = r2 < h2 ? h2 - r2 : 0.f;
       pd += mass * (ps * d * d * d);
     });
   } else {
-  sweep27<L, false>(U, b.counts, b.starts, in.xq, cx, cy, cz, sub,
-                    [&](uint32_t, float4 xj, bool) {
-    const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
-    const float r2 = dot3(rx, ry, rz, rx, ry, rz);
-    if (r2 < h2) {
-      const float d = h2 - r2;
-      pd += mass * (ps * d * d * d);
-      if (r2 > 1e-12f) {
-        // -m (q_i + q_j) grad * dir with dir = -r_vec / r
-        const float il = __builtin_amdgcn_rsqf(r2);
-        const float dr = h - r2 * il;
-        const float sc = mass * (xi.w + xj.w) * (gs * dr * dr) * il;
-        ax += sc * rx;
-        ay += sc * ry;
-        az += sc * rz;
+    sweep([&](uint32_t, float4 xj, bool) {
+      const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
+      const float r2 = dot3(rx, ry, rz, rx, ry, rz);
+      if (r2 < h2) {
+        const float d = h2 - r2;
+        pd += mass * (ps * d * d * d);
+        if (r2 > 1e-12f) {
+          // -m (q_i + q_j) grad * dir with dir = -r_vec / r
+          const float il = __builtin_amdgcn_rsqf(r2);
+          const float dr = h - r2 * il;
+          const float sc = mass * (xi.w + xj.w) * (gs * dr * dr) * il;
+          ax += sc * rx;
+          ay += sc * ry;
+          az += sc * rz;
+        }
       }
-    }
-  });
+    });
   }
   pd = group_sum<L>(pd);
   ax = group_sum<L>(ax);
@@ -594,6 +680,63 @@ __global__ __launch_bounds__(256) void k_solve(Uni un, Bufs b, uint32_t n, PredB
                           new_p / (nd * nd));
   out.pr[s] = make_float2(new_p, pd);
   out.v[s] = make_float4(vx, vy, vz, 0.f);
+}
+
+template <int L, bool B>
+__global__ __launch_bounds__(256) void k_solve(Uni un, Bufs b, uint32_t n, PredBuf in,
+                                               PredBuf out, float4 *ap_out, uint32_t xcd,
+                                               uint32_t first_iter) {
+  const uint32_t g = xcd_block(xcd) * blockDim.x + threadIdx.x;
+  const uint32_t s = g / L, sub = g % L;
+  if (s >= n) return;
+  const DevUniforms *__restrict__ U = uni_at<B>(un, s);
+  const float4 xi = in.xq[s];
+  const float4 p0 = b.spos[s];  // current position (grid lookup)
+  const int cx = cell_coord(p0.x, U->ox, U->inv_cell);
+  const int cy = cell_coord(p0.y, U->oy, U->inv_cell);
+  const int cz = cell_coord(p0.z, U->oz, U->inv_cell);
+  solve_particle<L>(U, b, in, out, ap_out, first_iter, s, sub, xi, p0, [&](auto &&f) {
+    sweep27<L, false>(U, b.counts, b.starts, in.xq, cx, cy, cz, sub, f);
+  });
+}
+
+// Tiled form of k_solve (see k_density_t): the tile's candidate rows of the
+// predicted state are staged in LDS.
+template <int L, bool B>
+__global__ __launch_bounds__(256) void k_solve_t(Uni un, Bufs b, uint32_t n, PredBuf in,
+                                                 PredBuf out, float4 *ap_out,
+                                                 uint32_t first_iter) {
+  extern __shared__ float4 stage[];
+  __shared__ TileShared S;
+  const uint32_t count = *b.tiles.count;
+  for (uint32_t t = blockIdx.x; t < count; t += gridDim.x) {
+    const uint4 e = b.tiles.list[t];
+    const DevUniforms *__restrict__ U = B ? un.u + e.x : un.u;
+    tile_setup(U, b.counts, b.starts, e.y, b.tiles.w, S);
+    const bool staged = tile_stage(S, in.xq, stage, b.tiles.cap);
+    const uint32_t first = S.own_lo + e.z * b.tiles.chunk;
+    const uint32_t own = min(S.own_hi - first, b.tiles.chunk);
+    for (uint32_t q0 = 0; q0 < own; q0 += 256 / L) {
+      const uint32_t q = q0 + threadIdx.x / L, sub = threadIdx.x % L;
+      if (q >= own) continue;
+      const uint32_t s = first + q;
+      const float4 xi = in.xq[s];
+      const float4 p0 = b.spos[s];
+      const int cx = cell_coord(p0.x, U->ox, U->inv_cell);
+      const int cy = cell_coord(p0.y, U->oy, U->inv_cell);
+      const int cz = cell_coord(p0.z, U->oz, U->inv_cell);
+      if (staged && tile_owns(S, U, cx, cy, cz)) {
+        solve_particle<L>(U, b, in, out, ap_out, first_iter, s, sub, xi, p0, [&](auto &&f) {
+          tile_sweep<L, false>(S, stage, true, cx, sub, f);
+        });
+      } else {
+        solve_particle<L>(U, b, in, out, ap_out, first_iter, s, sub, xi, p0, [&](auto &&f) {
+          sweep27<L, false>(U, b.counts, b.starts, in.xq, cx, cy, cz, sub, f);
+        });
+      }
+    }
+    __syncthreads();
+  }
 }
 
 // ------------------------------------------------------------------------
@@ -873,7 +1016,7 @@ inline uint32_t blocks_for(uint64_t threads, uint32_t per) {
 hipError_t launch_clear(const Bufs &b, uint32_t cells, uint32_t ntiles, hipStream_t st) {
   const uint32_t work = std::max(cells / 4 + 4, ntiles + 1);
   hipLaunchKernelGGL(k_clear, dim3(blocks_for(work, 256)), dim3(256), 0, st, b.counts,
-                     cells, b.tile_status, ntiles, b.tile_counter);
+                     cells, b.tile_status, ntiles, b.tile_counter, b.tiles.count);
   return hipGetLastError();
 }
 
@@ -916,9 +1059,12 @@ hipError_t launch_build(const Uni &u, const Bufs &b, uint32_t n, hipStream_t st)
   return hipGetLastError();
 }
 
-hipError_t launch_scan(const Uni &, const Bufs &b, uint32_t cells, hipStream_t st) {
+hipError_t launch_scan(const Uni &u, const Bufs &b, uint32_t cells, SweepCfg c,
+                       hipStream_t st) {
   hipLaunchKernelGGL(k_scan, dim3(blocks_for(cells, kScanTile)), dim3(kScanThreads), 0, st,
                      b, cells);
+  if (c.tiled && b.tiles.total)
+    FLUID_LAUNCH_B(k_tiles, blocks_for(b.tiles.total, 256), u, b);
   return hipGetLastError();
 }
 
@@ -929,9 +1075,34 @@ hipError_t launch_reorder(const Uni &, const Bufs &b, uint32_t n, hipStream_t st
   return hipGetLastError();
 }
 
+// Persistent tiled launches: c.tile_wgs workgroups, LDS for the staged rows.
+#define FLUID_DISPATCH_TILED(KERNEL, LANES, UN, ...)                                      \
+  do {                                                                                    \
+    const size_t _lds = (size_t)b.tiles.cap * sizeof(float4);                             \
+    const dim3 _g((uint32_t)c.tile_wgs);                                                  \
+    if ((UN).scene_of) {                                                                  \
+      switch (LANES) {                                                                    \
+        case 1: hipLaunchKernelGGL((KERNEL<1, true>), _g, dim3(256), _lds, st, UN, __VA_ARGS__); break; \
+        case 2: hipLaunchKernelGGL((KERNEL<2, true>), _g, dim3(256), _lds, st, UN, __VA_ARGS__); break; \
+        case 4: hipLaunchKernelGGL((KERNEL<4, true>), _g, dim3(256), _lds, st, UN, __VA_ARGS__); break; \
+        default: hipLaunchKernelGGL((KERNEL<8, true>), _g, dim3(256), _lds, st, UN, __VA_ARGS__); break; \
+      }                                                                                   \
+    } else {                                                                              \
+      switch (LANES) {                                                                    \
+        case 1: hipLaunchKernelGGL((KERNEL<1, false>), _g, dim3(256), _lds, st, UN, __VA_ARGS__); break; \
+        case 2: hipLaunchKernelGGL((KERNEL<2, false>), _g, dim3(256), _lds, st, UN, __VA_ARGS__); break; \
+        case 4: hipLaunchKernelGGL((KERNEL<4, false>), _g, dim3(256), _lds, st, UN, __VA_ARGS__); break; \
+        default: hipLaunchKernelGGL((KERNEL<8, false>), _g, dim3(256), _lds, st, UN, __VA_ARGS__); break; \
+      }                                                                                   \
+    }                                                                                     \
+  } while (0)
+
 hipError_t launch_density(const Uni &u, const Bufs &b, uint32_t n, SweepCfg c, hipStream_t st) {
   if (n == 0) return hipSuccess;
-  FLUID_DISPATCH_L(k_density, c.lanes, n, u, b, n, (uint32_t)c.xcd_tiles);
+  if (c.tiled)
+    FLUID_DISPATCH_TILED(k_density_t, c.lanes, u, b, n);
+  else
+    FLUID_DISPATCH_L(k_density, c.lanes, n, u, b, n, (uint32_t)c.xcd_tiles);
   return hipGetLastError();
 }
 
@@ -959,8 +1130,12 @@ hipError_t launch_solve(const Uni &u, const Bufs &b, uint32_t n, const PredBuf &
                         const PredBuf &out, float4 *ap_out, SweepCfg c, bool first_iter,
                         hipStream_t st) {
   if (n == 0) return hipSuccess;
-  FLUID_DISPATCH_L(k_solve, c.solve_lanes, n, u, b, n, in, out, ap_out,
-                   (uint32_t)c.xcd_tiles, (uint32_t)(first_iter ? 1 : 0));
+  if (c.tiled)
+    FLUID_DISPATCH_TILED(k_solve_t, c.lanes, u, b, n, in, out, ap_out,
+                         (uint32_t)(first_iter ? 1 : 0));
+  else
+    FLUID_DISPATCH_L(k_solve, c.solve_lanes, n, u, b, n, in, out, ap_out,
+                     (uint32_t)c.xcd_tiles, (uint32_t)(first_iter ? 1 : 0));
   return hipGetLastError();
 }
 

@@ -239,6 +239,17 @@ class SphSimulation3DGrid:
         self._chk(self._lib.fluid_get_sweep_config(self._h, arr))
         return tuple(arr)
 
+    def set_tile_config(self, tiled: int, width: int = 0, workgroups: int = 0) -> None:
+        """LDS-staged tiled sweeps on/off, tile width (cells), persistent
+        workgroups (fluid_set_tile_config; 0 keeps a value)."""
+        arr = (C.c_uint32 * 3)(tiled, width, workgroups)
+        self._chk(self._lib.fluid_set_tile_config(self._h, arr))
+
+    def tile_config(self) -> tuple[int, int, int]:
+        arr = (C.c_uint32 * 3)()
+        self._chk(self._lib.fluid_get_tile_config(self._h, arr))
+        return tuple(arr)
+
     def autotune(self, trials: int = 2) -> tuple[int, int, int]:
         arr = (C.c_uint32 * 3)()
         self._chk(self._lib.fluid_autotune(self._h, trials, arr))

@@ -365,10 +365,22 @@ int fluid_time_stages(fluid_sim *sim, uint32_t steps, float *ms_per_stage);
 int fluid_set_sweep_config(fluid_sim *sim, const uint32_t cfg[3]);
 int fluid_get_sweep_config(const fluid_sim *sim, uint32_t out[3]);
 
-/* Autotuner: times every launch configuration on the current state (each
- * trial runs one step from a saved copy of the state, which is restored
- * afterwards, so the simulation does not advance) and keeps the fastest.
- * out[3] receives the chosen configuration (may be null). */
+/* LDS-staged sweeps, cfg[3]:
+ *   cfg[0] 1: density and the PCISPH solve run as persistent workgroups
+ *          over the non-empty cell tiles, each staging its tile's 27-cell
+ *          neighbourhood (nine x-rows) in LDS; 0: global-memory gathers
+ *          (the same sums either way, bit for bit);
+ *   cfg[1] cells per tile along x, 1..16 (0 keeps the current value);
+ *   cfg[2] persistent workgroups (0 keeps the current value). */
+int fluid_set_tile_config(fluid_sim *sim, const uint32_t cfg[3]);
+int fluid_get_tile_config(const fluid_sim *sim, uint32_t out[3]);
+
+/* Autotuner: times every launch configuration (sweep mode: global gathers
+ * with either XCD mapping or LDS tiles of 2, 4, 8 cells; lanes per particle)
+ * on the current state (each trial runs one step from a saved copy of the
+ * state, which is restored afterwards, so the simulation does not advance)
+ * and keeps the fastest.  out[3] receives the chosen sweep configuration
+ * (may be null); fluid_get_tile_config reports the chosen tiling. */
 int fluid_autotune(fluid_sim *sim, uint32_t trials, uint32_t out[3]);
 
 /* Candidate and in-range neighbour pair totals of the current sorted state
