@@ -50,6 +50,7 @@ def stage_launches(stage: str, iters: int) -> int:
 def run_gpu(pkg, scene, warmup: int, steps: int, stage_steps: int):
     sim = pkg.make_sim(scene)
     sim.step(warmup)
+    run_gpu.start_state = sim.read_particles()  # the timed region's starting state
     lanes = sim.autotune(2)  # does not advance the state
     ms = sim.time_steps(0, steps)
     stages = sim.time_stages(stage_steps)
@@ -61,7 +62,23 @@ def run_gpu(pkg, scene, warmup: int, steps: int, stage_steps: int):
     return ms, stages, cand, inr, cells, iters
 
 
-def roofline_for(stages, n, cells, iters, cand, inr):
+TRAFFIC_PROFILE = ROOT / "profiles" / "pmc_traffic_dam_break_50k.md"
+
+
+def profiled_traffic(kernel: str):
+    """HBM bytes per dispatch of `kernel` (e.g. "k_force<8,false>") from the
+    committed FETCH_SIZE / WRITE_SIZE summary (copy-calibrated), or None."""
+    try:
+        for line in TRAFFIC_PROFILE.read_text().splitlines():
+            cols = [c.strip() for c in line.strip().strip("|").split("|")]
+            if len(cols) >= 5 and cols[0] == f"`{kernel}`":
+                return int(float(cols[4]))
+    except OSError:
+        return None
+    return None
+
+
+def roofline_for(stages, n, cells, iters, cand, inr, lanes=None):
     per_stage = {}
     for name, ms in stages.items():
         launches = stage_launches(name, iters)
@@ -79,8 +96,11 @@ def roofline_for(stages, n, cells, iters, cand, inr):
         per_stage[name] = entry
     dom = max(stages, key=lambda k: stages[k] / stage_launches(k, iters))
     d = per_stage[dom]
+    traffic = None
+    if lanes is not None and dom in FLOPS_INRANGE:
+        traffic = profiled_traffic(f"k_{dom}<{lanes[1] if dom == 'solve' else lanes[0]},false>")
     roof = {"bound": "hbm", "kernel": dom, "achieved": d["hbm_gbs"], "peak": HBM_PEAK_GBS,
-            "unit": "GB/s", "frac": d["hbm_frac"], "traffic": None,
+            "unit": "GB/s", "frac": d["hbm_frac"], "traffic": traffic,
             "launch_ms": round(stages[dom] / stage_launches(dom, iters), 5)}
     if "tflops" in d:
         roof["valu"] = {"achieved": d["tflops"], "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
@@ -228,10 +248,95 @@ def run_adjoint(pkg, orc, steps: int = 64, every: int = 16, fd_coords: int = 64,
                          "seconds": round(time.perf_counter() - t0, 1)}}
 
 
-def cpu_baseline(pkg, orc, scene, budget_s: float = 12.0, max_steps: int = 40):
+def run_calibration(pkg, count: int = 64, horizon: int = 2, iters: int = 6):
+    """Batched forward + adjoint over `count` uneven scenes (the first of the
+    batched workload's scenes), calibrating each scene's viscosity: the
+    target is the same scene run with its viscosity scaled by a seeded
+    factor in [0.7, 1.3]; the loss is the mean squared final position
+    error after `horizon` steps; each iteration takes one batched forward
+    rollout, one batched backward (per-scene dL/dviscosity) and a forward
+    difference for the Gauss-Newton curvature.  (Beyond a few steps the
+    positions decorrelate chaotically and viscosity is not identifiable
+    from them.)  Also times one batched forward + backward against the same
+    scenes done one after another on plain handles, cotangents on device."""
+    import numpy as np
+    scs = pkg.scenes.batch_scenes(count)
+    params = [pkg.scenes.make_params(sc, pkg.abi, pkg.derive_sph_params,
+                                     pkg.derive_container_geometry)[0] for sc in scs]
+    v0 = np.array([p.viscosity for p in params], np.float64)
+    vtrue = v0 * (0.7 + 0.6 * pkg.scenes.uniform01(99, count))
+    parts = np.concatenate([sc.particles for sc in scs])
+    b = pkg.make_batch(scs)
+    sl = [b.scene_slice(k) for k in range(count)]
+    nk = np.array([x.stop - x.start for x in sl], np.float64)
+
+    def forward(v, roll=True):
+        for k in range(count):
+            params[k].viscosity = float(v[k])
+            b.update_scene(k, params=params[k])
+        b.write_particles(parts)
+        if roll:
+            b.forward_rollout(horizon, horizon)
+        else:
+            b.step(horizon)
+        return b.read_particles()[:, 0:3].astype(np.float64)
+
+    xt = forward(vtrue, roll=False)
+    v = v0.copy()
+    hist = []
+    for it in range(iters + 1):
+        x = forward(v)
+        r = x - xt
+        err = np.abs(v - vtrue) / vtrue
+        hist.append({"iteration": it, "median_rel_err": float(np.median(err)),
+                     "p90_rel_err": float(np.quantile(err, 0.9)),
+                     "median_loss": float(np.median([np.sum(r[q] ** 2) / m
+                                                     for q, m in zip(sl, nk)]))})
+        if it == iters:
+            break
+        lam = np.concatenate([2 * r[q] / m for q, m in zip(sl, nk)]).astype(np.float32)
+        _, _, gp = b.backward_rollout(lam, None, want_state=False)
+        eps = 1e-3 * v
+        xe = forward(v + eps, roll=False)
+        j2 = np.array([np.sum((xe[q] - x[q]) ** 2) / e ** 2 for q, e in zip(sl, eps)])
+        v = np.clip(v - (gp[:, 0] * nk / 2) / np.maximum(j2, 1e-30), 0.3 * v0, 3 * v0)
+    # timing: one forward + backward, device-resident cotangent
+    lam_d = torch.ones((len(parts), 3), dtype=torch.float32, device="cuda") * 1e-3
+
+    def fb(h, lam_ptr):
+        h.sync()
+        t0 = time.perf_counter()
+        h.forward_rollout(horizon, horizon)
+        h.backward_rollout_device(lam_ptr, 0, 0, 0)
+        return time.perf_counter() - t0
+
+    b.write_particles(parts)
+    fb(b, lam_d.data_ptr())
+    t_batch = min(fb(b, lam_d.data_ptr()) for _ in range(3))
+    b.close()
+    t_seq = 0.0
+    for k, sc in enumerate(scs):
+        h = pkg.make_sim(sc)
+        fb(h, lam_d[sl[k]].data_ptr())
+        t_seq += min(fb(h, lam_d[sl[k]].data_ptr()) for _ in range(2))
+        h.close()
+    return {"scenes": count, "particles": int(nk.sum()), "horizon_steps": horizon,
+            "method": "Gauss-Newton per scene: batched adjoint gradient, forward-difference "
+                      "curvature",
+            "convergence": hist,
+            "fwd_bwd_batched_ms": round(t_batch * 1e3, 3),
+            "fwd_bwd_sequential_ms": round(t_seq * 1e3, 3),
+            "speedup": round(t_seq / t_batch, 3)}
+
+
+def cpu_baseline(pkg, orc, scene, start, warmup: int, budget_s: float = 15.0,
+                 max_steps: int = 400):
+    """The FP32 oracle (OpenMP over the host's cores) stepping the same
+    workload from the state the GPU's timed region starts from (after the
+    warm-up steps), for about `budget_s` seconds."""
     params, cont, g = pkg.scenes.make_params(scene, pkg.abi, pkg.derive_sph_params,
                                              pkg.derive_container_geometry)
-    o = orc.from_scene(scene, pkg.abi, params, cont, g)
+    o = orc.from_scene(scene, pkg.abi, params, cont, g, particles=start)
     o.step(1)  # warm-up (page-in, thread pool)
     t0 = time.perf_counter()
     k = 0
@@ -243,8 +348,13 @@ def cpu_baseline(pkg, orc, scene, budget_s: float = 12.0, max_steps: int = 40):
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     return {"value": round(scene.n * k / dt, 1), "unit": "particle-updates/s", "cores": threads,
             "kind": "port",
-            "sample": f"{scene.name}: {k} f32 oracle steps after 1 warm-up step "
-                      f"({dt:.1f} s, OpenMP {threads} threads)"}
+            "sample": f"{scene.name}: {k} f32 oracle steps from the GPU state after "
+                      f"{warmup} warm-up steps (+1 untimed) ({dt:.1f} s, OpenMP {threads} "
+                      f"threads)"}
+
+
+def progress(msg: str) -> None:
+    print(f"bench: {msg}", file=sys.stderr, flush=True)
 
 
 def main() -> int:
@@ -279,10 +389,12 @@ def main() -> int:
 
     pkg = ge.load_package()
     scene = pkg.scenes.dam_break_50k()
+    progress("dam_break_50k")
     ms, stages, cand, inr, cells, iters = run_gpu(pkg, scene, args.warmup, args.steps,
                                                   args.stage_steps)
+    start50 = run_gpu.start_state
     value = scene.n * args.steps / (ms / 1e3)
-    roof, per_stage = roofline_for(stages, scene.n, cells, iters, cand, inr)
+    roof, per_stage = roofline_for(stages, scene.n, cells, iters, cand, inr, run_gpu.lanes)
     line = {
         "metric": "particle-updates/s at 50k & 1M; batched vs plain speedup; adjoint/fwd cost",
         "value": round(value, 1),
@@ -306,6 +418,7 @@ def main() -> int:
                                "in_range": round(inr / scene.n, 1)},
     }
     if not args.no_1m:
+        progress("block_1m")
         big = pkg.scenes.block_1m()
         k1 = max(10, min(100, args.steps // 10))
         ms1, st1, c1, i1, cells1, it1 = run_gpu(pkg, big, 5, k1, min(args.stage_steps, 10))
@@ -316,11 +429,17 @@ def main() -> int:
                             "stage_ms": {k: round(v, 4) for k, v in st1.items()},
                             "roofline": roof1}
     if not args.no_batch:
+        progress("batched")
         line["batched"] = run_batched(pkg, args.batch_scenes)
     if not args.no_adjoint:
+        progress("adjoint")
         line["adjoint"] = run_adjoint(pkg, ge.load_oracle(), fd_coords=args.fd_coords)
+        if not args.no_batch:
+            progress("calibration")
+            line["calibration"] = run_calibration(pkg)
     if not args.no_cpu:
-        line["cpu_baseline"] = cpu_baseline(pkg, ge.load_oracle(), scene)
+        progress("cpu baseline")
+        line["cpu_baseline"] = cpu_baseline(pkg, ge.load_oracle(), scene, start50, args.warmup)
     print(json.dumps(line), flush=True)
     return 0
 

@@ -53,6 +53,11 @@ hipError_t launch_adj_load(const uint32_t *sid, const float *dl_x3, const float 
 hipError_t launch_adj_store(const uint32_t *sid, const float4 *lx, const float4 *lv,
                             float *out_x3, float *out_v3, uint32_t n, hipStream_t st);
 
+// out[scene * kGradParams + q] = sum of acc[slot * kAccStride + q] over the
+// scene's slots [slot_first[scene], slot_first[scene + 1]).
+hipError_t launch_adj_reduce(const double *acc, const uint32_t *slot_first, uint32_t scenes,
+                             double *out, hipStream_t st);
+
 // The full reverse of one step.  it[k], k = 0..iters-1, are the inputs of
 // PCISPH iteration k; last = the final iteration's output (its pr.y is the
 // final predicted density, its ap the final unclamped acceleration).

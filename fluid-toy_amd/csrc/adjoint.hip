@@ -661,9 +661,41 @@ __global__ __launch_bounds__(256) void k_adj_density(Uni un, AdjBufs a, TapeStep
   a.lout_v[old] = a.lu[s];
 }
 
+// Per-scene sums of the per-slot parameter gradients: one workgroup per
+// scene, a strided pass then a fixed tree, so the order (and the result) is
+// the same on every run.
+__global__ __launch_bounds__(256) void k_adj_reduce(const double *__restrict__ acc,
+                                                    const uint32_t *__restrict__ slot_first,
+                                                    double *__restrict__ out) {
+  __shared__ double sh[kGradParams][256];
+  const uint32_t k = blockIdx.x, t = threadIdx.x;
+  const uint32_t lo = slot_first[k], hi = slot_first[k + 1];
+  double v[kGradParams] = {0.0, 0.0, 0.0, 0.0, 0.0};
+  for (uint32_t i = lo + t; i < hi; i += 256)
+#pragma unroll
+    for (int q = 0; q < kGradParams; ++q) v[q] += acc[(size_t)i * kAccStride + q];
+#pragma unroll
+  for (int q = 0; q < kGradParams; ++q) sh[q][t] = v[q];
+  __syncthreads();
+  for (uint32_t o = 128; o > 0; o >>= 1) {
+    if (t < o)
+#pragma unroll
+      for (int q = 0; q < kGradParams; ++q) sh[q][t] += sh[q][t + o];
+    __syncthreads();
+  }
+  if (t < kGradParams) out[(size_t)k * kGradParams + t] = sh[t][0];
+}
+
 inline uint32_t nblocks(uint64_t threads) { return (uint32_t)((threads + 255) / 256); }
 
 }  // namespace
+
+hipError_t launch_adj_reduce(const double *acc, const uint32_t *slot_first, uint32_t scenes,
+                             double *out, hipStream_t st) {
+  if (scenes == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_adj_reduce, dim3(scenes), dim3(256), 0, st, acc, slot_first, out);
+  return hipGetLastError();
+}
 
 hipError_t launch_adj_load(const uint32_t *sid, const float *dl_x3, const float *dl_v3,
                            float4 *lx, float4 *lv, uint32_t n, hipStream_t st) {

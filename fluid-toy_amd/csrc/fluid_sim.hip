@@ -367,9 +367,11 @@ struct AdjViews {
   double *acc;
   float4 *scratch_v;                  // non-final solve velocities of a replay
   float *in_x, *in_v, *out_x, *out_v;  // canonical (3 floats per particle) staging
+  uint32_t *slot_first;               // scenes + 1 slot offsets
+  double *grads;                      // scenes x kGradParams
 };
 
-AdjViews carve_adj(Carve &c, size_t n) {
+AdjViews carve_adj(Carve &c, size_t n, size_t scenes) {
   AdjViews a{};
   for (int k = 0; k < 2; ++k) {
     a.lam_x[k] = c.take<float4>(n);
@@ -390,6 +392,8 @@ AdjViews carve_adj(Carve &c, size_t n) {
   a.in_v = c.take<float>(3 * n);
   a.out_x = c.take<float>(3 * n);
   a.out_v = c.take<float>(3 * n);
+  a.slot_first = c.take<uint32_t>(scenes + 1);
+  a.grads = c.take<double>(scenes * kGradParams);
   return a;
 }
 
@@ -419,7 +423,7 @@ TapeSlot tape_slot(fluid_sim *s, uint32_t t) {
 
 AdjViews adj_views(fluid_sim *s) {
   Carve c{(char *)s->roll.adj};
-  return carve_adj(c, s->roll.cap_n);
+  return carve_adj(c, s->roll.cap_n, s->sc.size());
 }
 
 // Size the rollout storage for the handle's current n / iterations / grid.
@@ -440,7 +444,7 @@ int ensure_rollout(fluid_sim *s, uint32_t every, uint32_t nckpt) {
   carve_tape(m, n, cells, iters);
   const size_t tp = (m.off + 255) & ~(size_t)255;
   m = Carve{};
-  carve_adj(m, n);
+  carve_adj(m, n, s->sc.size());
   const size_t adj = m.off;
   void *p[4] = {nullptr, nullptr, nullptr, nullptr};
   const size_t bytes[4] = {ck * std::max(nckpt, 1u), fin, tp * std::max(every, 1u), adj};
@@ -1402,7 +1406,12 @@ int fluid_backward_rollout(fluid_sim *s, const float *dl_dpos, const float *dl_d
     }
     HIP_TRY(hipEventRecord(ev[3 + 3 * seg], st));
   }
-  std::vector<double> acc;
+  const size_t S = s->sc.size();
+  std::vector<uint32_t> slot_first(S + 1, 0);
+  for (size_t k = 0; k < S; ++k) slot_first[k] = s->sc[k].first;
+  slot_first[S] = n;
+  if (S == 1) slot_first[0] = 0;
+  std::vector<double> grads(S * kGradParams, 0.0);
   if (n) {
     const uint32_t *sid0 = nseg ? ckpt_view(s, 0).sid : s->b.sid;
     HIP_TRY(launch_adj_store(sid0, A.lam_x[cur], A.lam_v[cur], A.out_x, A.out_v, n, st));
@@ -1411,8 +1420,10 @@ int fluid_backward_rollout(fluid_sim *s, const float *dl_dpos, const float *dl_d
     if (dl_dvel0)
       HIP_TRY(hipMemcpyAsync(dl_dvel0, A.out_v, (size_t)n * 12, hipMemcpyDefault, st));
     if (dl_dparams) {
-      acc.resize((size_t)n * kAccStride);
-      HIP_TRY(hipMemcpyAsync(acc.data(), A.acc, acc.size() * sizeof(double),
+      HIP_TRY(hipMemcpyAsync(A.slot_first, slot_first.data(), (S + 1) * 4,
+                             hipMemcpyHostToDevice, st));
+      HIP_TRY(launch_adj_reduce(A.acc, A.slot_first, (uint32_t)S, A.grads, st));
+      HIP_TRY(hipMemcpyAsync(grads.data(), A.grads, grads.size() * sizeof(double),
                              hipMemcpyDeviceToHost, st));
     }
   }
@@ -1431,15 +1442,7 @@ int fluid_backward_rollout(fluid_sim *s, const float *dl_dpos, const float *dl_d
   R.ms[0] = replay;
   R.ms[1] = reverse;
   R.ms[2] = total;
-  if (dl_dparams) {
-    for (size_t k = 0; k < s->sc.size(); ++k) {
-      const SceneState &c = s->sc[k];
-      double g[kGradParams] = {0, 0, 0, 0, 0};
-      for (uint32_t i = c.first; i < c.first + c.count && !acc.empty(); ++i)
-        for (int q = 0; q < kGradParams; ++q) g[q] += acc[(size_t)i * kAccStride + q];
-      for (int q = 0; q < kGradParams; ++q) dl_dparams[k * kGradParams + q] = g[q];
-    }
-  }
+  if (dl_dparams) std::memcpy(dl_dparams, grads.data(), grads.size() * sizeof(double));
   return check_errors(s);
 }
 
