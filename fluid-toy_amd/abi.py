@@ -176,10 +176,15 @@ PROTOTYPES = {
     "fluid_autotune": (C.c_int, [P, u32, C.POINTER(u32)]),
     "fluid_set_tile_config": (C.c_int, [P, C.POINTER(u32)]),
     "fluid_get_tile_config": (C.c_int, [P, C.POINTER(u32)]),
+    "fluid_set_fusion": (C.c_int, [P, u32]),
+    "fluid_fusion": (u32, [P]),
     "fluid_forward_rollout": (C.c_int, [P, u32, u32]),
     "fluid_backward_rollout": (C.c_int, [P, P, P, P, P, P]),
     "fluid_rollout_timing": (C.c_int, [P, C.POINTER(C.c_float)]),
 }
+
+FLUID_FUSE_XSPH_FORCE = 1
+FLUID_FUSE_SOLVE_LIST = 2
 
 # Gradient slots per scene written by fluid_backward_rollout.
 FLUID_GRAD_PARAMS = 5

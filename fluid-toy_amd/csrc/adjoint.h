@@ -26,6 +26,11 @@ struct AdjBufs {
                                 // predicted-density adjoint), ping-pong
   float *lP;                    // pressure of the iteration being reversed
   double *acc;                  // parameter gradients per slot (kAccStride)
+  // per-lane lists built within one reversed step (lane k of particle s at
+  // [e * n * L + s * L + k]): the solve skin list of the first reversed
+  // PCISPH iteration, and the in-range pairs the force adjoint meets
+  uint32_t *slist, *scnt;       // kSolveListCapacity per particle
+  uint32_t *plist, *pcnt;       // kListCapacity per particle
 };
 
 // What the forward step kept for its reverse (all in the step's slot order).

@@ -258,10 +258,17 @@ __global__ __launch_bounds__(256) void k_adj_integrate(Uni un, AdjBufs a, TapeSt
 // the adjoint of X_k and q_k; the epilogue takes q_k = P_k / max(rho*_{k-1},
 // 1)^2 back to P_k and rho*_{k-1} and reverses iteration k-1's epilogue
 // (k > 0) or predict (k = 0).
+//
+// mode 1 (the first reversed iteration of K >= 2) sweeps the 27 cells and
+// records, per lane, the candidates within the solve skin of the particle at
+// the first forward iteration's predicted positions xq0 (every iteration's
+// in-range pairs lie among them); mode 2 (the other reversed iterations)
+// walks that list; mode 0 (K = 1) only sweeps.
 template <int L, bool B>
 __global__ __launch_bounds__(256) void k_adj_solve(Uni un, AdjBufs a, TapeStep t, uint32_t n,
                                                    TapeIter in, const float4 *__restrict__ lap_in,
-                                                   float4 *__restrict__ lap_out, uint32_t k) {
+                                                   float4 *__restrict__ lap_out, uint32_t k,
+                                                   const float4 *__restrict__ xq0, uint32_t mode) {
   const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
   const uint32_t s = g / L, sub = g % L;
   if (s >= n) return;
@@ -273,8 +280,10 @@ __global__ __launch_bounds__(256) void k_adj_solve(Uni un, AdjBufs a, TapeStep t
   const float h = U->h, h2 = U->h2, mass = U->mass, gs = U->grad_scale;
   const float c_rho = mass * (-6.0f * U->poly6_scale);  // 2 d(m W)/d(r^2) = c_rho d^2
   float lXx = 0.f, lXy = 0.f, lXz = 0.f, lq = 0.f;
-  sweep27<L, false>(U, t.counts, t.starts, in.xq, cx, cy, cz, sub,
-                    [&](uint32_t j, float4 xj, bool) {
+  const size_t stride = (size_t)n * L;
+  uint32_t *__restrict__ lst = a.slist + (size_t)s * L + sub;
+  constexpr uint32_t kCap = kSolveListCapacity / L;
+  auto pair = [&](uint32_t j, float4 xj, bool) {
     const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
     const float r2 = dot3(rx, ry, rz, rx, ry, rz);
     if (!(r2 < h2)) return;
@@ -299,7 +308,33 @@ __global__ __launch_bounds__(256) void k_adj_solve(Uni un, AdjBufs a, TapeStep t
     lXx += cr * rx;
     lXy += cr * ry;
     lXz += cr * rz;
-  });
+  };
+  if (mode == 1) {
+    const float4 x0i = xq0[s];
+    const float skin2 = U->solve_skin2;
+    uint32_t cnt = 0;
+    sweep27<L, false>(U, t.counts, t.starts, in.xq, cx, cy, cz, sub,
+                      [&](uint32_t j, float4 xj, bool v) {
+      const float4 x0j = xq0[j];
+      const float ex = x0i.x - x0j.x, ey = x0i.y - x0j.y, ez = x0i.z - x0j.z;
+      if (dot3(ex, ey, ez, ex, ey, ez) < skin2) {
+        if (cnt < kCap) lst[(size_t)cnt * stride] = j;
+        ++cnt;
+      }
+      pair(j, xj, v);
+    });
+    a.scnt[(size_t)s * L + sub] = cnt <= kCap ? cnt : kSolveListOverflow;
+  } else {
+    const uint32_t cnt = mode == 2 ? a.scnt[(size_t)s * L + sub] : kSolveListOverflow;
+    if (cnt != kSolveListOverflow) {
+      for (uint32_t e = 0; e < cnt; ++e) {
+        const uint32_t j = lst[(size_t)e * stride];
+        pair(j, in.xq[j], true);
+      }
+    } else {
+      sweep27<L, false>(U, t.counts, t.starts, in.xq, cx, cy, cz, sub, pair);
+    }
+  }
   V3 lX = group_sum3<L>(v3(lXx, lXy, lXz));
   lq = group_sum<L>(lq);
   if (sub != 0) return;
@@ -463,11 +498,17 @@ __global__ __launch_bounds__(256) void k_adj_force(Uni un, AdjBufs a, TapeStep t
   const float h2 = U->h2;
   V3 gx = zero3(), gw = zero3(), gu = zero3(), gn = zero3();
   float gd = 0.f, gnd = 0.f, gv = 0.f, gst = 0.f;
+  const size_t stride = (size_t)n * L;
+  uint32_t *__restrict__ lst = a.plist + (size_t)s * L + sub;
+  constexpr uint32_t kCap = kListCapacity / L;
+  uint32_t cnt = 0;
   sweep27<L, false>(U, t.counts, t.starts, t.x, cx, cy, cz, sub,
                     [&](uint32_t j, float4 xj, bool) {
     const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
     const float r2 = dot3(rx, ry, rz, rx, ry, rz);
     if (!(r2 < h2 && r2 > 1e-12f)) return;
+    if (cnt < kCap) lst[(size_t)cnt * stride] = j;
+    ++cnt;
     const float4 *rj = t.rec + 3 * (size_t)j;
     const float4 rj0 = rj[0], rj1 = rj[1], rj2 = rj[2];
     PairGrad G;
@@ -490,6 +531,7 @@ __global__ __launch_bounds__(256) void k_adj_force(Uni un, AdjBufs a, TapeStep t
     gd += G.d_b;
     gnd += G.nd_b;
   });
+  a.pcnt[(size_t)s * L + sub] = cnt <= kCap ? cnt : kListOverflow;
   gx = group_sum3<L>(gx);
   gw = group_sum3<L>(gw);
   gu = group_sum3<L>(gu);
@@ -512,6 +554,40 @@ __global__ __launch_bounds__(256) void k_adj_force(Uni un, AdjBufs a, TapeStep t
   acc[4] += (double)lai.z;
 }
 
+// The in-range pairs of this lane: the list the force adjoint recorded, or
+// (after an overflow) the 27-cell sweep with the same test.  f(j, x_j).
+template <int L, class F>
+__device__ __forceinline__ void for_each_pair(const DevUniforms *__restrict__ U, const AdjBufs &a,
+                                              const TapeStep &t, uint32_t n, uint32_t s,
+                                              uint32_t sub, float4 xi, F &&f) {
+  const uint32_t cnt = a.pcnt[(size_t)s * L + sub];
+  if (cnt != kListOverflow) {
+    const size_t stride = (size_t)n * L;
+    const uint32_t *__restrict__ lst = a.plist + (size_t)s * L + sub;
+    uint32_t e = 0;
+    for (; e + 1 < cnt; e += 2) {
+      const uint32_t j0 = lst[(size_t)e * stride], j1 = lst[(size_t)(e + 1) * stride];
+      const float4 x0 = t.x[j0], x1 = t.x[j1];
+      f(j0, x0);
+      f(j1, x1);
+    }
+    if (e < cnt) {
+      const uint32_t j0 = lst[(size_t)e * stride];
+      f(j0, t.x[j0]);
+    }
+    return;
+  }
+  int cx, cy, cz;
+  cell_of(U, xi, cx, cy, cz);
+  const float h2 = U->h2;
+  sweep27<L, false>(U, t.counts, t.starts, t.x, cx, cy, cz, sub,
+                    [&](uint32_t j, float4 xj, bool) {
+    const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
+    const float r2 = dot3(rx, ry, rz, rx, ry, rz);
+    if (r2 < h2 && r2 > 1e-12f) f(j, xj);
+  });
+}
+
 // ------------------------------------------------------------------------
 // XSPH (xsph.wgsl:81-138): w_i = u_i + eps sum_j m W(r^2) / max(rho_j, 1)
 // (u_j - u_i).  The epilogue reverses the density stage's rho = S / gamma(x)
@@ -532,13 +608,9 @@ __global__ __launch_bounds__(256) void k_adj_xsph(Uni un, AdjBufs a, TapeStep t,
   if (eps > 0.0f) {
     const float h2 = U->h2, mass = U->mass, ps = U->poly6_scale;
     const float idib = 1.0f / fmaxf(ri0.w, 1.0f);
-    int cx, cy, cz;
-    cell_of(U, xi, cx, cy, cz);
-    sweep27<L, false>(U, t.counts, t.starts, t.x, cx, cy, cz, sub,
-                      [&](uint32_t j, float4 xj, bool) {
+    for_each_pair<L>(U, a, t, n, s, sub, xi, [&](uint32_t j, float4 xj) {
       const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
       const float r2 = dot3(rx, ry, rz, rx, ry, rz);
-      if (!(r2 < h2 && r2 > 1e-12f)) return;
       const float4 rj0 = t.rec[3 * (size_t)j], uj = t.rec[3 * (size_t)j + 1];
       const float4 lwj = a.lw[j];
       const float d = h2 - r2;
@@ -625,18 +697,14 @@ __global__ __launch_bounds__(256) void k_adj_density(Uni un, AdjBufs a, TapeStep
   const float4 xi = t.x[s];
   const float2 lsi = a.lrho[s];
   const float4 lni = a.ln[s];
-  int cx, cy, cz;
-  cell_of(U, xi, cx, cy, cz);
   const float h = U->h, h2 = U->h2, mass = U->mass;
   const float c_s = mass * (-6.0f * U->poly6_scale);  // 2 d(m W)/d(r^2) / d^2
   const float c_n = mass * (-3.0f * U->near_scale);   // d(m ns (h-r)^3)/dr / (h-r)^2
   const float cn = U->normal_scale * U->m_over_rho0;
   float gx = 0.f, gy = 0.f, gz = 0.f;
-  sweep27<L, false>(U, t.counts, t.starts, t.x, cx, cy, cz, sub,
-                    [&](uint32_t j, float4 xj, bool) {
+  for_each_pair<L>(U, a, t, n, s, sub, xi, [&](uint32_t j, float4 xj) {
     const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
     const float r2 = dot3(rx, ry, rz, rx, ry, rz);
-    if (!(r2 < h2 && r2 > 1e-12f)) return;
     const float2 lsj = a.lrho[j];
     const float4 lnj = a.ln[j];
     const float d = h2 - r2;
@@ -746,8 +814,9 @@ hipError_t launch_adjoint_step(const Uni &u, AdjBufs a, const TapeStep &t, const
   ADJ_LAUNCH_B(k_adj_integrate, nblocks(n), u, a, t, last, n);
   int cur = 0;
   for (int k = (int)iters - 1; k >= 0; --k) {
+    const uint32_t mode = iters == 1 ? 0u : (k == (int)iters - 1 ? 1u : 2u);
     ADJ_LAUNCH_LB(k_adj_solve, L, n, u, a, t, n, it[k], a.lap[cur], a.lap[1 - cur],
-                  (uint32_t)k);
+                  (uint32_t)k, it[0].xq, mode);
     cur = 1 - cur;
   }
   ADJ_LAUNCH_LB(k_adj_force, L, n, u, a, t, n);

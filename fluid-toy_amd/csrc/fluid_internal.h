@@ -75,6 +75,9 @@ struct DevUniforms {
   float interact_range;  // h * 0.7                                      :303
   float md2_lo, md2_hi;  // (0.1 h)^2 (1 -/+ 1e-6): r < 0.1h is decided on
                          // r^2 outside this band, on a rounded sqrt inside
+  float solve_skin2;     // (h + 2 dt^2 max_accel)^2 (1 + 1e-3): pairs farther
+                         // apart at the first PCISPH iteration stay out of
+                         // range in every later one
   // GpuGridParams (cell_offset: first cell of this grid in a batch's
   // concatenated cell array, 0 for a single scene)
   uint32_t gx, gy, gz, cells, gxy, cell_offset;
@@ -123,6 +126,7 @@ struct Bufs {
   float4 *spos, *svel, *sfrc;
   float4 *srec;
   uint32_t *nbr, *ncnt;
+  uint32_t *slist, *scnt;  // per-lane candidate lists of the later PCISPH iterations
   uint32_t *sid;
   uint32_t *gmap;  // new slot -> previous slot of the last reorder, or null
   uint32_t *counts, *starts;
@@ -145,6 +149,11 @@ constexpr uint32_t kScanTile = kScanThreads * kScanItems;
 
 // In-range neighbour entries stored per particle (split over its lanes).
 constexpr uint32_t kListCapacity = 1024;
+// Solve candidates kept per particle for the later iterations (split over
+// its lanes; a lane that overflows sweeps its cells instead).
+constexpr uint32_t kSolveListCapacity = 512;
+constexpr uint32_t kSolveListOverflow = 0xFFFFFFFFu;
+constexpr uint32_t kListOverflow = 0xFFFFFFFFu;
 
 // Error bits written by kernels into Bufs::err.
 constexpr uint32_t kErrScanSpin = 1u;
@@ -159,6 +168,8 @@ struct SweepCfg {
                     // `lanes` lanes per particle, 256 / lanes particles per
                     // work item (Bufs::tiles.chunk)
   int tile_wgs;     // persistent workgroups of the tiled sweeps
+  int fuse_xf;      // 1: XSPH and force run as one kernel in full-step ranges
+  int solve_list;   // 1: later PCISPH iterations walk the first one's skin list
 };
 
 // Host-side launchers (sph_kernels.hip).  All enqueue on `st`.
@@ -178,6 +189,9 @@ hipError_t launch_xsph(const Uni &u, const Bufs &b, uint32_t n,
 // fuse_predict: also write pcisph_predict's output for each particle.
 hipError_t launch_force(const Uni &u, const Bufs &b, uint32_t n,
                         SweepCfg c, bool fuse_predict, hipStream_t st);
+// XSPH and force as one kernel (the same bits as the two launches).
+hipError_t launch_xsph_force(const Uni &u, const Bufs &b, uint32_t n,
+                             SweepCfg c, bool fuse_predict, hipStream_t st);
 hipError_t launch_predict(const Uni &u, const Bufs &b, uint32_t n,
                           hipStream_t st);
 // first_iter: the input holds predict's zero pressures (density sum only).

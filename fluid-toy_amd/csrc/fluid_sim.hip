@@ -89,6 +89,7 @@ struct fluid_sim {
   uint32_t graph_n = 0, graph_iters = 0;
   int graph_lanes = 0, graph_solve_lanes = 0, graph_xcd = 0, graph_tiled = 0, graph_wgs = 0;
   uint32_t graph_tile_w = 0;
+  int graph_fuse_xf = 0, graph_solve_list = 0;
   uint32_t num_cu = 256;
   uint32_t *d_scene_first = nullptr;  // first tile id of each scene (+ total)
   int cursor = -1;  // last completed stage of a partial step, -1 = none
@@ -118,7 +119,7 @@ SweepCfg pick_cfg(uint32_t) {
   // Default before tuning: 8 lanes per particle, the fastest setting the
   // autotuner found at every size measured (8k to 1M particles, compressed
   // and relaxed states); global-memory sweeps.
-  return SweepCfg{8, 8, 0, 0, 1024};
+  return SweepCfg{8, 8, 0, 0, 1024, 1, 1};
 }
 
 // Per-scene tile counts for tiles of w cells along x; uploads the scene
@@ -143,9 +144,11 @@ int setup_tiles(fluid_sim *s, uint32_t w) {
 
 void update_cfg(fluid_sim *s) {
   if (s->tuned) return;
-  const int wgs = s->cfg.tile_wgs;
+  const int wgs = s->cfg.tile_wgs, fuse = s->cfg.fuse_xf, sl = s->cfg.solve_list;
   s->cfg = pick_cfg(s->n);
   s->cfg.tile_wgs = wgs;
+  s->cfg.fuse_xf = fuse;
+  s->cfg.solve_list = sl;
 }
 
 int push_uniforms(fluid_sim *s) {
@@ -178,6 +181,7 @@ int enqueue_range(fluid_sim *s, uint32_t first, uint32_t last) {
   const Bufs &b = s->b;
   const Uni u = s->uni();
   const bool fuse_fp = first <= FLUID_STAGE_FORCE && last >= FLUID_STAGE_PREDICT;
+  const bool fuse_xf = s->cfg.fuse_xf && first <= FLUID_STAGE_XSPH && last >= FLUID_STAGE_FORCE;
   const bool fuse_fi = first <= FLUID_STAGE_FINALIZE && last >= FLUID_STAGE_INTEGRATE;
   const int fin_buf = (s->iters % 2 == 0) ? 0 : 1;
   for (uint32_t stage = first; stage <= last; ++stage) {
@@ -198,10 +202,13 @@ int enqueue_range(fluid_sim *s, uint32_t first, uint32_t last) {
         HIP_TRY(launch_density(u, b, n, s->cfg, st));
         break;
       case FLUID_STAGE_XSPH:
-        HIP_TRY(launch_xsph(u, b, n, s->cfg, st));
+        if (fuse_xf)
+          HIP_TRY(launch_xsph_force(u, b, n, s->cfg, fuse_fp, st));
+        else
+          HIP_TRY(launch_xsph(u, b, n, s->cfg, st));
         break;
       case FLUID_STAGE_FORCE:
-        HIP_TRY(launch_force(u, b, n, s->cfg, fuse_fp, st));
+        if (!fuse_xf) HIP_TRY(launch_force(u, b, n, s->cfg, fuse_fp, st));
         break;
       case FLUID_STAGE_PREDICT:
         if (!fuse_fp) HIP_TRY(launch_predict(u, b, n, st));
@@ -237,7 +244,8 @@ int ensure_graph(fluid_sim *s) {
   if (s->graph && s->graph_n == s->n && s->graph_iters == s->iters &&
       s->graph_lanes == s->cfg.lanes && s->graph_solve_lanes == s->cfg.solve_lanes &&
       s->graph_xcd == s->cfg.xcd_tiles && s->graph_tiled == s->cfg.tiled &&
-      s->graph_wgs == s->cfg.tile_wgs && s->graph_tile_w == s->b.tiles.w)
+      s->graph_wgs == s->cfg.tile_wgs && s->graph_tile_w == s->b.tiles.w &&
+      s->graph_fuse_xf == s->cfg.fuse_xf && s->graph_solve_list == s->cfg.solve_list)
     return FLUID_OK;
   drop_graph(s);
   hipGraph_t g = nullptr;
@@ -263,6 +271,8 @@ int ensure_graph(fluid_sim *s) {
   s->graph_tiled = s->cfg.tiled;
   s->graph_wgs = s->cfg.tile_wgs;
   s->graph_tile_w = s->b.tiles.w;
+  s->graph_fuse_xf = s->cfg.fuse_xf;
+  s->graph_solve_list = s->cfg.solve_list;
   return FLUID_OK;
 }
 
@@ -369,6 +379,7 @@ struct AdjViews {
   float *in_x, *in_v, *out_x, *out_v;  // canonical (3 floats per particle) staging
   uint32_t *slot_first;               // scenes + 1 slot offsets
   double *grads;                      // scenes x kGradParams
+  uint32_t *pcnt;                     // per-lane in-range pair counts (8 per slot)
 };
 
 AdjViews carve_adj(Carve &c, size_t n, size_t scenes) {
@@ -394,6 +405,7 @@ AdjViews carve_adj(Carve &c, size_t n, size_t scenes) {
   a.out_v = c.take<float>(3 * n);
   a.slot_first = c.take<uint32_t>(scenes + 1);
   a.grads = c.take<double>(scenes * kGradParams);
+  a.pcnt = c.take<uint32_t>(8 * n);
   return a;
 }
 
@@ -508,8 +520,12 @@ int enqueue_tape_step(fluid_sim *s, const TapeSlot &T, float4 *scratch_v) {
   Bufs mb = rb;  // XSPH's output w stays in the tape
   mb.vel = T.w;
   mb.pred[0] = PredBuf{T.xq[0], T.pr[0], scratch_v};
-  HIP_TRY(launch_xsph(u, mb, n, c, st));
-  HIP_TRY(launch_force(u, mb, n, c, true, st));
+  if (c.fuse_xf) {
+    HIP_TRY(launch_xsph_force(u, mb, n, c, true, st));
+  } else {
+    HIP_TRY(launch_xsph(u, mb, n, c, st));
+    HIP_TRY(launch_force(u, mb, n, c, true, st));
+  }
   for (uint32_t k = 0; k < iters; ++k) {
     const PredBuf in{T.xq[k], T.pr[k], scratch_v};
     const PredBuf out{T.xq[k + 1], T.pr[k + 1], k + 1 == iters ? T.vK : scratch_v};
@@ -537,6 +553,10 @@ int enqueue_adjoint_step(fluid_sim *s, const TapeSlot &T, const AdjViews &A, int
   a.lap[1] = A.lap[1];
   a.lP = A.lP;
   a.acc = A.acc;
+  a.slist = s->b.slist;  // forward scratch, free while a step is reversed
+  a.scnt = s->b.scnt;
+  a.plist = s->b.nbr;
+  a.pcnt = A.pcnt;
   const TapeStep t{T.x, T.u, T.w, T.rec, T.vK, T.counts, T.starts, T.gmap};
   std::vector<TapeIter> it(K);
   for (uint32_t k = 0; k < K; ++k)
@@ -626,6 +646,8 @@ static int create_handle(int device, std::vector<SceneState> scenes,
   ALLOC(srec, float4 *, 3 * N);
   ALLOC(nbr, uint32_t *, (size_t)kListCapacity * N);
   ALLOC(ncnt, uint32_t *, N);
+  ALLOC(slist, uint32_t *, (size_t)kSolveListCapacity * N);
+  ALLOC(scnt, uint32_t *, 8 * N);
   ALLOC(sid, uint32_t *, N);
   ALLOC(counts, uint32_t *, C);
   ALLOC(starts, uint32_t *, C);
@@ -1180,6 +1202,23 @@ int fluid_set_tile_config(fluid_sim *s, const uint32_t cfg[3]) {
   return FLUID_OK;
 }
 
+int fluid_set_fusion(fluid_sim *s, uint32_t flags) {
+  if (!s) return fail(FLUID_ERR_INVALID_ARG, "null handle");
+  if (flags & ~(FLUID_FUSE_XSPH_FORCE | FLUID_FUSE_SOLVE_LIST))
+    return fail(FLUID_ERR_INVALID_ARG, "unknown fusion flag");
+  if (s->cursor != -1) return fail(FLUID_ERR_STATE, "a partial step is in progress");
+  s->cfg.fuse_xf = (flags & FLUID_FUSE_XSPH_FORCE) ? 1 : 0;
+  s->cfg.solve_list = (flags & FLUID_FUSE_SOLVE_LIST) ? 1 : 0;
+  s->roll.valid = false;
+  return FLUID_OK;
+}
+
+uint32_t fluid_fusion(const fluid_sim *s) {
+  if (!s) return 0u;
+  return (s->cfg.fuse_xf ? FLUID_FUSE_XSPH_FORCE : 0u) |
+         (s->cfg.solve_list ? FLUID_FUSE_SOLVE_LIST : 0u);
+}
+
 int fluid_get_tile_config(const fluid_sim *s, uint32_t out[3]) {
   if (!s || !out) return fail(FLUID_ERR_INVALID_ARG, "null argument");
   out[0] = (uint32_t)s->cfg.tiled;
@@ -1281,6 +1320,27 @@ int fluid_autotune(fluid_sim *s, uint32_t trials, uint32_t out[3]) {
   if (e == hipSuccess && rc == FLUID_OK && best_w != s->b.tiles.w) {
     e = hipStreamSynchronize(s->stream);
     if (e == hipSuccess) rc = setup_tiles(s, best_w);
+  }
+  // XSPH + force as one kernel or two: timed as whole graph-launched steps
+  // (the fusion only exists in full-step ranges)
+  {
+    float best_f = 1e30f;
+    int fuse = best.fuse_xf;
+    for (int f = 0; f < 2 && e == hipSuccess && rc == FLUID_OK; ++f) {
+      s->cfg = best;
+      s->cfg.fuse_xf = f;
+      float tot = 0.f, ms = 0.f;
+      for (uint32_t t = 0; t < trials && rc == FLUID_OK; ++t) {
+        if ((e = copy(false)) != hipSuccess) break;
+        rc = fluid_time_steps(s, 0, 2, &ms);
+        tot += ms;
+      }
+      if (tot < best_f) {
+        best_f = tot;
+        fuse = f;
+      }
+    }
+    best.fuse_xf = fuse;
   }
   if (e == hipSuccess) e = copy(false);
   if (e == hipSuccess)

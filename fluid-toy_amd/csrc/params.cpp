@@ -131,6 +131,11 @@ void fill_uniforms(DevUniforms &u, const GpuSphParams3D &p,
   const double md2 = (double)u.min_dist * (double)u.min_dist;
   u.md2_lo = (float)(md2 * (1.0 - 1e-6));
   u.md2_hi = (float)(md2 * (1.0 + 1e-6));
+  // Predicted positions of iteration k >= 1 differ from iteration 0's by
+  // dt^2 clamp(a_p), |clamp(a_p)| <= max_accel, for both particles of a pair.
+  const double skin = (double)p.kernel_radius +
+                      2.0 * (double)p.dt * (double)p.dt * (double)u.max_accel;
+  u.solve_skin2 = (float)(skin * skin * (1.0 + 1e-3));
   u.gx = g.grid_size_x;
   u.gy = g.grid_size_y;
   u.gz = g.grid_size_z;

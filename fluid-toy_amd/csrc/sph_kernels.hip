@@ -469,21 +469,15 @@ __device__ __forceinline__ void for_each_neighbour(const DevUniforms *__restrict
 }
 
 // ------------------------------------------------------------------------
-// Stage 5 — xsph.wgsl:81-138.  Writes the canonical velocity; with
-// epsilon <= 0 the reference leaves it untouched, i.e. equal to the sorted
-// copy, which is what is written then.
-template <int L, bool B>
-__global__ __launch_bounds__(256) void k_xsph(Uni un, Bufs b, uint32_t n, uint32_t xcd) {
-  const uint32_t g = xcd_block(xcd) * blockDim.x + threadIdx.x;
-  const uint32_t s = g / L, sub = g % L;
-  if (s >= n) return;
-  const DevUniforms *__restrict__ U = uni_at<B>(un, s);
+// Stage 5 — xsph.wgsl:81-138: the particle's XSPH-corrected velocity w_i
+// (in every lane of its group).  With epsilon <= 0 the reference leaves the
+// velocity untouched, i.e. equal to the sorted copy.
+template <int L>
+__device__ __forceinline__ float4 xsph_particle(const DevUniforms *__restrict__ U,
+                                                const Bufs &b, uint32_t n, uint32_t s,
+                                                uint32_t sub, float4 vi) {
   const float eps = U->eps;
-  const float4 vi = b.svel[s];
-  if (!(eps > 0.0f)) {
-    if (sub == 0) b.vel[s] = vi;
-    return;
-  }
+  if (!(eps > 0.0f)) return vi;
   const float4 pi = b.spos[s];
   const float h2 = U->h2, mass = U->mass, ps = U->poly6_scale;
   float cxv = 0.f, cyv = 0.f, czv = 0.f;
@@ -502,8 +496,17 @@ __global__ __launch_bounds__(256) void k_xsph(Uni un, Bufs b, uint32_t n, uint32
   cxv = group_sum<L>(cxv);
   cyv = group_sum<L>(cyv);
   czv = group_sum<L>(czv);
-  if (sub == 0)
-    b.vel[s] = make_float4(vi.x + eps * cxv, vi.y + eps * cyv, vi.z + eps * czv, 0.f);
+  return make_float4(vi.x + eps * cxv, vi.y + eps * cyv, vi.z + eps * czv, 0.f);
+}
+
+template <int L, bool B>
+__global__ __launch_bounds__(256) void k_xsph(Uni un, Bufs b, uint32_t n, uint32_t xcd) {
+  const uint32_t g = xcd_block(xcd) * blockDim.x + threadIdx.x;
+  const uint32_t s = g / L, sub = g % L;
+  if (s >= n) return;
+  const DevUniforms *__restrict__ U = uni_at<B>(un, s);
+  const float4 w = xsph_particle<L>(U, b, n, s, sub, b.svel[s]);
+  if (sub == 0) b.vel[s] = w;
 }
 
 // ------------------------------------------------------------------------
@@ -512,18 +515,16 @@ __global__ __launch_bounds__(256) void k_xsph(Uni un, Bufs b, uint32_t n, uint32
 // The per-pair quotients are regrouped around one reciprocal each
 // (sp = -m/(2 rho_i) (P_i/rho^n_j + k_near), the viscosity and K_ij share
 // 1/(rho_i + rho_j)); r < 0.1 h, which switches the direction, is decided on
-// the exact bits the oracle uses.
-template <int L, bool B>
-__global__ __launch_bounds__(256) void k_force(Uni un, Bufs b, uint32_t n, uint32_t xcd,
-                                               uint32_t fuse_predict) {
-  const uint32_t g = xcd_block(xcd) * blockDim.x + threadIdx.x;
-  const uint32_t s = g / L, sub = g % L;
-  if (s >= n) return;
-  const DevUniforms *__restrict__ U = uni_at<B>(un, s);
+// the exact bits the oracle uses.  vi is the particle's post-XSPH velocity;
+// neighbours contribute their pre-XSPH velocity (sph_force_3d_grid.wgsl
+// :107-119 reads the sorted copy).
+template <int L>
+__device__ __forceinline__ void force_particle(const DevUniforms *__restrict__ U, const Bufs &b,
+                                               uint32_t n, uint32_t s, uint32_t sub,
+                                               float4 vi, uint32_t fuse_predict) {
   const float4 *rec_i = b.srec + 3 * (size_t)s;
   const float4 pi = rec_i[0];
   const float4 ni = rec_i[2];
-  const float4 vi = b.vel[s];  // post-XSPH self velocity
   const float di = fmaxf(pi.w, 1.0f);
   const float ndi = fmaxf(rec_i[1].w, 1.0f);
   const float h = U->h, mass = U->mass, near_k = U->near_k;
@@ -596,6 +597,33 @@ __global__ __launch_bounds__(256) void k_force(Uni un, Bufs b, uint32_t n, uint3
   }
 }
 
+template <int L, bool B>
+__global__ __launch_bounds__(256) void k_force(Uni un, Bufs b, uint32_t n, uint32_t xcd,
+                                               uint32_t fuse_predict) {
+  const uint32_t g = xcd_block(xcd) * blockDim.x + threadIdx.x;
+  const uint32_t s = g / L, sub = g % L;
+  if (s >= n) return;
+  const DevUniforms *__restrict__ U = uni_at<B>(un, s);
+  force_particle<L>(U, b, n, s, sub, b.vel[s], fuse_predict);
+}
+
+// XSPH and force in one pass per particle: the list is walked for XSPH,
+// the corrected velocity stays in registers (and is stored for the later
+// stages) and the same list, now in cache, is walked for the force.  Force
+// reads only the particle's own corrected velocity, so this gives the bits
+// of the two separate kernels.
+template <int L, bool B>
+__global__ __launch_bounds__(256) void k_xsph_force(Uni un, Bufs b, uint32_t n, uint32_t xcd,
+                                                    uint32_t fuse_predict) {
+  const uint32_t g = xcd_block(xcd) * blockDim.x + threadIdx.x;
+  const uint32_t s = g / L, sub = g % L;
+  if (s >= n) return;
+  const DevUniforms *__restrict__ U = uni_at<B>(un, s);
+  const float4 w = xsph_particle<L>(U, b, n, s, sub, b.svel[s]);
+  if (sub == 0) b.vel[s] = w;
+  force_particle<L>(U, b, n, s, sub, w, fuse_predict);
+}
+
 // ------------------------------------------------------------------------
 // Stage 7 — pcisph_predict.wgsl:50-73, run per sorted slot (the canonical
 // state already lives in slot order).
@@ -618,27 +646,43 @@ __global__ __launch_bounds__(256) void k_predict(Uni un, Bufs b, uint32_t n) {
 // (same operations as the reference evaluates per pair).  sweep(f) walks
 // the candidates of the particle's cell (the grid of the current positions)
 // over the iteration's predicted positions, lane-independently.
+//
+// The first iteration also records, per lane and in its walk order, the
+// candidates within the solve skin of the particle (1.5 h plus a margin at
+// the first iteration's predicted positions); every later iteration's
+// in-range pairs are among them, so the later iterations walk that list and
+// sum the same pairs in the same order per lane (the same bits as a sweep).
 template <int L, class Sweep>
 __device__ __forceinline__ void solve_particle(const DevUniforms *__restrict__ U, const Bufs &b,
-                                               const PredBuf &in, const PredBuf &out,
-                                               float4 *ap_out, uint32_t first_iter, uint32_t s,
-                                               uint32_t sub, float4 xi, float4 p0,
-                                               Sweep &&sweep) {
+                                               uint32_t n, const PredBuf &in, const PredBuf &out,
+                                               float4 *ap_out, uint32_t first_iter,
+                                               uint32_t use_list, uint32_t s, uint32_t sub,
+                                               float4 xi, float4 p0, Sweep &&sweep) {
   const float h = U->h, h2 = U->h2, mass = U->mass, ps = U->poly6_scale;
   const float gs = U->grad_scale;
+  const size_t stride = (size_t)n * L;
+  uint32_t *__restrict__ lst = b.slist + (size_t)s * L + sub;
+  constexpr uint32_t kCap = kSolveListCapacity / L;
   float pd = 0.f, ax = 0.f, ay = 0.f, az = 0.f;
   if (first_iter) {
     // First iteration: every pressure of the input is 0 (pcisph_predict.wgsl
     // :69), so each pressure term is a signed zero and a_total = a_np
     // exactly; only the predicted density needs the sum.
-    sweep([&](uint32_t, float4 xj, bool) {
+    const float skin2 = U->solve_skin2;
+    uint32_t cnt = 0;
+    sweep([&](uint32_t j, float4 xj, bool) {
       const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
       const float r2 = dot3(rx, ry, rz, rx, ry, rz);
       const float d = r2 < h2 ? h2 - r2 : 0.f;
       pd += mass * (ps * d * d * d);
+      if (use_list && r2 < skin2) {
+        if (cnt < kCap) lst[(size_t)cnt * stride] = j;
+        ++cnt;
+      }
     });
+    if (use_list) b.scnt[(size_t)s * L + sub] = cnt <= kCap ? cnt : kSolveListOverflow;
   } else {
-    sweep([&](uint32_t, float4 xj, bool) {
+    auto body = [&](uint32_t, float4 xj, bool) {
       const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
       const float r2 = dot3(rx, ry, rz, rx, ry, rz);
       if (r2 < h2) {
@@ -654,7 +698,27 @@ __device__ __forceinline__ void solve_particle(const DevUniforms *__restrict__ U
           az += sc * rz;
         }
       }
-    });
+    };
+    const uint32_t cnt = use_list ? b.scnt[(size_t)s * L + sub] : kSolveListOverflow;
+    if (cnt != kSolveListOverflow) {
+      const float4 *__restrict__ xq = in.xq;
+      uint32_t k = 0;
+      for (; k + 3 < cnt; k += 4) {
+        const uint32_t j0 = lst[(size_t)k * stride], j1 = lst[(size_t)(k + 1) * stride];
+        const uint32_t j2 = lst[(size_t)(k + 2) * stride], j3 = lst[(size_t)(k + 3) * stride];
+        const float4 x0 = xq[j0], x1 = xq[j1], x2 = xq[j2], x3 = xq[j3];
+        body(j0, x0, true);
+        body(j1, x1, true);
+        body(j2, x2, true);
+        body(j3, x3, true);
+      }
+      for (; k < cnt; ++k) {
+        const uint32_t j0 = lst[(size_t)k * stride];
+        body(j0, xq[j0], true);
+      }
+    } else {
+      sweep(body);
+    }
   }
   pd = group_sum<L>(pd);
   ax = group_sum<L>(ax);
@@ -685,7 +749,7 @@ __device__ __forceinline__ void solve_particle(const DevUniforms *__restrict__ U
 template <int L, bool B>
 __global__ __launch_bounds__(256) void k_solve(Uni un, Bufs b, uint32_t n, PredBuf in,
                                                PredBuf out, float4 *ap_out, uint32_t xcd,
-                                               uint32_t first_iter) {
+                                               uint32_t first_iter, uint32_t use_list) {
   const uint32_t g = xcd_block(xcd) * blockDim.x + threadIdx.x;
   const uint32_t s = g / L, sub = g % L;
   if (s >= n) return;
@@ -695,7 +759,7 @@ __global__ __launch_bounds__(256) void k_solve(Uni un, Bufs b, uint32_t n, PredB
   const int cx = cell_coord(p0.x, U->ox, U->inv_cell);
   const int cy = cell_coord(p0.y, U->oy, U->inv_cell);
   const int cz = cell_coord(p0.z, U->oz, U->inv_cell);
-  solve_particle<L>(U, b, in, out, ap_out, first_iter, s, sub, xi, p0, [&](auto &&f) {
+  solve_particle<L>(U, b, n, in, out, ap_out, first_iter, use_list, s, sub, xi, p0, [&](auto &&f) {
     sweep27<L, false>(U, b.counts, b.starts, in.xq, cx, cy, cz, sub, f);
   });
 }
@@ -705,7 +769,7 @@ __global__ __launch_bounds__(256) void k_solve(Uni un, Bufs b, uint32_t n, PredB
 template <int L, bool B>
 __global__ __launch_bounds__(256) void k_solve_t(Uni un, Bufs b, uint32_t n, PredBuf in,
                                                  PredBuf out, float4 *ap_out,
-                                                 uint32_t first_iter) {
+                                                 uint32_t first_iter, uint32_t use_list) {
   extern __shared__ float4 stage[];
   __shared__ TileShared S;
   const uint32_t count = *b.tiles.count;
@@ -726,11 +790,11 @@ __global__ __launch_bounds__(256) void k_solve_t(Uni un, Bufs b, uint32_t n, Pre
       const int cy = cell_coord(p0.y, U->oy, U->inv_cell);
       const int cz = cell_coord(p0.z, U->oz, U->inv_cell);
       if (staged && tile_owns(S, U, cx, cy, cz)) {
-        solve_particle<L>(U, b, in, out, ap_out, first_iter, s, sub, xi, p0, [&](auto &&f) {
+        solve_particle<L>(U, b, n, in, out, ap_out, first_iter, use_list, s, sub, xi, p0, [&](auto &&f) {
           tile_sweep<L, false>(S, stage, true, cx, sub, f);
         });
       } else {
-        solve_particle<L>(U, b, in, out, ap_out, first_iter, s, sub, xi, p0, [&](auto &&f) {
+        solve_particle<L>(U, b, n, in, out, ap_out, first_iter, use_list, s, sub, xi, p0, [&](auto &&f) {
           sweep27<L, false>(U, b.counts, b.starts, in.xq, cx, cy, cz, sub, f);
         });
       }
@@ -1120,6 +1184,14 @@ hipError_t launch_force(const Uni &u, const Bufs &b, uint32_t n, SweepCfg c, boo
   return hipGetLastError();
 }
 
+hipError_t launch_xsph_force(const Uni &u, const Bufs &b, uint32_t n, SweepCfg c,
+                             bool fuse_predict, hipStream_t st) {
+  if (n == 0) return hipSuccess;
+  FLUID_DISPATCH_L(k_xsph_force, c.lanes, n, u, b, n, (uint32_t)c.xcd_tiles,
+                   (uint32_t)(fuse_predict ? 1 : 0));
+  return hipGetLastError();
+}
+
 hipError_t launch_predict(const Uni &u, const Bufs &b, uint32_t n, hipStream_t st) {
   if (n == 0) return hipSuccess;
   FLUID_LAUNCH_B(k_predict, blocks_for(n, 256), u, b, n);
@@ -1132,10 +1204,11 @@ hipError_t launch_solve(const Uni &u, const Bufs &b, uint32_t n, const PredBuf &
   if (n == 0) return hipSuccess;
   if (c.tiled)
     FLUID_DISPATCH_TILED(k_solve_t, c.lanes, u, b, n, in, out, ap_out,
-                         (uint32_t)(first_iter ? 1 : 0));
+                         (uint32_t)(first_iter ? 1 : 0), (uint32_t)c.solve_list);
   else
     FLUID_DISPATCH_L(k_solve, c.solve_lanes, n, u, b, n, in, out, ap_out,
-                     (uint32_t)c.xcd_tiles, (uint32_t)(first_iter ? 1 : 0));
+                     (uint32_t)c.xcd_tiles, (uint32_t)(first_iter ? 1 : 0),
+                     (uint32_t)c.solve_list);
   return hipGetLastError();
 }
 

@@ -245,6 +245,13 @@ class SphSimulation3DGrid:
         arr = (C.c_uint32 * 3)(tiled, width, workgroups)
         self._chk(self._lib.fluid_set_tile_config(self._h, arr))
 
+    def set_fusion(self, flags: int) -> None:
+        """Kernel fusion flags (fluid_set_fusion): abi.FLUID_FUSE_XSPH_FORCE."""
+        self._chk(self._lib.fluid_set_fusion(self._h, int(flags)))
+
+    def fusion(self) -> int:
+        return int(self._lib.fluid_fusion(self._h))
+
     def tile_config(self) -> tuple[int, int, int]:
         arr = (C.c_uint32 * 3)()
         self._chk(self._lib.fluid_get_tile_config(self._h, arr))

@@ -100,3 +100,47 @@ def test_tiled_stagewise_matches_graph(pkg):
     assert np.array_equal(a.read_particles(), b.read_particles())
     a.close()
     b.close()
+
+
+def test_fused_xsph_force_equals_separate(pkg):
+    """The fused XSPH + force kernel of the captured step gives the bits of
+    the two separate kernels, with global and with tiled sweeps."""
+    scene = pkg.scenes.dam_break_8k()
+    for tiled in (0, 1):
+        a = pkg.make_sim(scene)
+        b = pkg.make_sim(scene)
+        a.set_fusion(0)
+        b.set_fusion(pkg.abi.FLUID_FUSE_XSPH_FORCE)
+        for s in (a, b):
+            s.set_tile_config(tiled, 4)
+        print(f"case: fused xsph+force tiled={tiled}", flush=True)
+        for step in range(4):
+            a.step(1)
+            b.step(1)
+            _same(a, b, f"fused tiled={tiled} step={step}")
+        a.close()
+        b.close()
+
+
+@pytest.mark.parametrize("lanes", [1, 4, 8])
+def test_solve_skin_list_equals_sweep(pkg, lanes):
+    """Later PCISPH iterations over the first iteration's skin list give the
+    bits of the 27-cell sweep (plain and tiled), on the compressed lattice and
+    after it has exploded into fast clusters."""
+    scene = pkg.scenes.dam_break_8k()
+    full = pkg.abi.FLUID_FUSE_XSPH_FORCE | pkg.abi.FLUID_FUSE_SOLVE_LIST
+    for tiled in (0, 1):
+        a = pkg.make_sim(scene)
+        b = pkg.make_sim(scene)
+        for s in (a, b):
+            s.set_sweep_config(lanes, lanes, 0)
+            s.set_tile_config(tiled, 4)
+        a.set_fusion(pkg.abi.FLUID_FUSE_XSPH_FORCE)
+        b.set_fusion(full)
+        print(f"case: solve skin list lanes={lanes} tiled={tiled}", flush=True)
+        for step in range(12):
+            a.step(1)
+            b.step(1)
+            _same(a, b, f"skin lanes={lanes} tiled={tiled} step={step}")
+        a.close()
+        b.close()
