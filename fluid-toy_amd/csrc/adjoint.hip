@@ -327,7 +327,17 @@ __global__ __launch_bounds__(256) void k_adj_solve(Uni un, AdjBufs a, TapeStep t
   } else {
     const uint32_t cnt = mode == 2 ? a.scnt[(size_t)s * L + sub] : kSolveListOverflow;
     if (cnt != kSolveListOverflow) {
-      for (uint32_t e = 0; e < cnt; ++e) {
+      uint32_t e = 0;
+      for (; e + 3 < cnt; e += 4) {
+        const uint32_t j0 = lst[(size_t)e * stride], j1 = lst[(size_t)(e + 1) * stride];
+        const uint32_t j2 = lst[(size_t)(e + 2) * stride], j3 = lst[(size_t)(e + 3) * stride];
+        const float4 x0 = in.xq[j0], x1 = in.xq[j1], x2 = in.xq[j2], x3 = in.xq[j3];
+        pair(j0, x0, true);
+        pair(j1, x1, true);
+        pair(j2, x2, true);
+        pair(j3, x3, true);
+      }
+      for (; e < cnt; ++e) {
         const uint32_t j = lst[(size_t)e * stride];
         pair(j, in.xq[j], true);
       }
