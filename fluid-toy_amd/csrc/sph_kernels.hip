@@ -549,10 +549,11 @@ __device__ __forceinline__ void force_particle(const DevUniforms *__restrict__ U
   const float npi = near_k * ndi;
   const float a_sp = -mass / (2.0f * di);
   const float c_visc = 2.0f * mass * U->visc * h;  // -m * (-visc * h / (avg))
-  const float c_k = 2.0f * U->rho0 * mass;
+  const float st = U->st;
+  const float c_k = 2.0f * U->rho0 * mass * st;    // surface tension folded in
   const float inv_md = 1.0f / md;
-  float apx = 0.f, apy = 0.f, apz = 0.f;
-  float acx = 0.f, acy = 0.f, acz = 0.f;
+  // one accumulator: a = sum (sp + st mk coh) dir - st mk (n_i - n_j)
+  float fx = 0.f, fy = 0.f, fz = 0.f;
   const float4 *__restrict__ srec = b.srec;
   for_each_neighbour<L, 2>(U, b, n, s, sub, pi, [&](uint32_t j, float4 pj) {
     const float rx = pi.x - pj.x, ry = pi.y - pj.y, rz = pi.z - pj.z;
@@ -561,15 +562,8 @@ __device__ __forceinline__ void force_particle(const DevUniforms *__restrict__ U
     const float4 nj = srec[3 * (size_t)j + 2];
     bool close = r2 < md2_lo;
     if (!close && r2 < md2_hi) close = sqrtf(r2) < md;
-    const float r = fast_sqrt(r2);
-    float dx, dy, dz;
-    if (close) {
-      const float3 d3 = close_dir(s, j);
-      dx = d3.x; dy = d3.y; dz = d3.z;
-    } else {
-      const float il = __builtin_amdgcn_rcpf(r);
-      dx = -rx * il; dy = -ry * il; dz = -rz * il;
-    }
+    const float il = __builtin_amdgcn_rsqf(r2);  // 1 / r
+    const float r = r2 * il;
     const float dj = fmaxf(pj.w, 1.0f);
     const float ndj = fmaxf(vj.w, 1.0f);
     const float dr = h - r;
@@ -581,25 +575,32 @@ __device__ __forceinline__ void force_particle(const DevUniforms *__restrict__ U
     const float vdx = dot3(vi.x - vj.x, vi.y - vj.y, vi.z - vj.z, rx, ry, rz);
     const float inv_dd = __builtin_amdgcn_rcpf(di + dj);
     sp += c_visc * fminf(vdx, 0.0f) * inv_dd * __builtin_amdgcn_rcpf(r2 + veps) * grad;
-    // Akinci cohesion + curvature (:206-214)
+    // Akinci cohesion + curvature (:206-214), times surface tension
     const float mk = c_k * inv_dd;
     const float r3 = r * r * r;
     const float dr3 = dr * dr * dr;
     const float coh = (r >= hh) ? aks * dr3 * r3 : aks * (2.0f * dr3 * r3 - h6_64);
-    const float sc = mk * coh;
-    apx += sp * dx;
-    apy += sp * dy;
-    apz += sp * dz;
-    acx += sc * dx - mk * (ni.x - nj.x);
-    acy += sc * dy - mk * (ni.y - nj.y);
-    acz += sc * dz - mk * (ni.z - nj.z);
+    const float coef = __builtin_fmaf(mk, coh, sp);
+    if (close) {
+      const float3 d3 = close_dir(s, j);
+      fx = __builtin_fmaf(coef, d3.x, fx);
+      fy = __builtin_fmaf(coef, d3.y, fy);
+      fz = __builtin_fmaf(coef, d3.z, fz);
+    } else {
+      const float c = -coef * il;  // dir = -r_vec / r
+      fx = __builtin_fmaf(c, rx, fx);
+      fy = __builtin_fmaf(c, ry, fy);
+      fz = __builtin_fmaf(c, rz, fz);
+    }
+    fx = __builtin_fmaf(-mk, ni.x - nj.x, fx);
+    fy = __builtin_fmaf(-mk, ni.y - nj.y, fy);
+    fz = __builtin_fmaf(-mk, ni.z - nj.z, fz);
   });
-  apx = group_sum<L>(apx); apy = group_sum<L>(apy); apz = group_sum<L>(apz);
-  acx = group_sum<L>(acx); acy = group_sum<L>(acy); acz = group_sum<L>(acz);
+  fx = group_sum<L>(fx); fy = group_sum<L>(fy); fz = group_sum<L>(fz);
   if (sub != 0) return;
-  const float st = U->st;
-  const float fx = apx + acx * st + U->grav[0], fy = apy + acy * st + U->grav[1],
-              fz = apz + acz * st + U->grav[2];
+  fx += U->grav[0];
+  fy += U->grav[1];
+  fz += U->grav[2];
   b.frc[s] = make_float4(fx, fy, fz, 0.f);
   if (fuse_predict) {
     // pcisph_predict.wgsl:57-72 for this particle, on the values in hand
