@@ -122,12 +122,13 @@ def run_batched(pkg, count: int, steps: int = 20):
     t_batch = batch.time_steps(0, k) / k
     stages = batch.time_stages(3)
     batch.close()
+    # the same steps (6 .. 5 + k of each scene) on the sequential side
     t_seq = 0.0
     for sc in scenes:
         sim = pkg.make_sim(sc)
         sim.step(5)
         sim.autotune(1)
-        t_seq += sim.time_steps(0, 5) / 5
+        t_seq += sim.time_steps(0, k) / k
         sim.close()
     n = sum(sc.n for sc in scenes)
     return {"scenes": count, "particles": n, "batched_ms_per_step": round(t_batch, 4),
@@ -364,6 +365,7 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--stage-steps", type=int, default=20)
     ap.add_argument("--no-1m", action="store_true", help="skip the 1M-particle block")
+    ap.add_argument("--no-4m", action="store_true", help="skip the 4M-particle block")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-form comparison")
     ap.add_argument("--batch-scenes", type=int, default=256)
@@ -417,17 +419,26 @@ def main() -> int:
         "pairs_per_particle": {"candidates": round(cand / scene.n, 1),
                                "in_range": round(inr / scene.n, 1)},
     }
-    if not args.no_1m:
-        progress("block_1m")
-        big = pkg.scenes.block_1m()
+    # throughput scaling: the 1M block, then the 4M block (100 steps each at
+    # the default --steps)
+    blocks = [] if args.no_1m else [("block_1m", pkg.scenes.block_1m)]
+    if not args.no_4m:
+        blocks.append(("block_4m", pkg.scenes.block_4m))
+    for name, make in blocks:
+        progress(name)
+        big = make()
         k1 = max(10, min(100, args.steps // 10))
         ms1, st1, c1, i1, cells1, it1 = run_gpu(pkg, big, 5, k1, min(args.stage_steps, 10))
         roof1, _ = roofline_for(st1, big.n, cells1, it1, c1, i1)
-        line["block_1m"] = {"particles": big.n, "steps": k1,
-                            "value": round(big.n * k1 / (ms1 / 1e3), 1),
-                            "ms_per_step": round(ms1 / k1, 4),
-                            "stage_ms": {k: round(v, 4) for k, v in st1.items()},
-                            "roofline": roof1}
+        line[name] = {"particles": big.n, "steps": k1, "grid_cells": cells1,
+                      "value": round(big.n * k1 / (ms1 / 1e3), 1),
+                      "ms_per_step": round(ms1 / k1, 4),
+                      "lanes_per_particle": list(run_gpu.lanes),
+                      "pairs_per_particle": {"candidates": round(c1 / big.n, 1),
+                                             "in_range": round(i1 / big.n, 1)},
+                      "stage_ms": {k: round(v, 4) for k, v in st1.items()},
+                      "roofline": roof1}
+        del big
     if not args.no_batch:
         progress("batched")
         line["batched"] = run_batched(pkg, args.batch_scenes)

@@ -402,19 +402,16 @@ __device__ __forceinline__ void force_pair_vjp(const DevUniforms *__restrict__ U
   const float rx = A.x.x - Bn.x.x, ry = A.x.y - Bn.x.y, rz = A.x.z - Bn.x.z;
   const float r2 = dot3(rx, ry, rz, rx, ry, rz);
   const float h = U->h, mass = U->mass, md = U->min_dist;
-  bool close = r2 < U->md2_lo;
-  if (!close && r2 < U->md2_hi) close = sqrtf(r2) < md;
+  const bool close = r2 < U->close_r2;
   const float ir = __builtin_amdgcn_rsqf(r2);
   const float r = r2 * ir;
-  V3 dir;
-  if (close) {
+  V3 dir = v3(-rx * ir, -ry * ir, -rz * ir);
+  if (close) {  // rare: the slot-difference direction
     const float3 c = adj_close_dir(A.slot, Bn.slot);
     dir = v3(c.x, c.y, c.z);
-  } else {
-    dir = v3(-rx * ir, -ry * ir, -rz * ir);
   }
-  const float da = fmaxf(A.d, 1.0f), nda = fmaxf(A.nd, 1.0f);
-  const float db = fmaxf(Bn.d, 1.0f), ndb = fmaxf(Bn.nd, 1.0f);
+  const float da = max1(A.d), nda = max1(A.nd);
+  const float db = max1(Bn.d), ndb = max1(Bn.nd);
   const float kn = U->near_k;
   const float dr = h - r, gs = U->grad_scale;
   const float gk = gs * dr * dr;
@@ -434,7 +431,8 @@ __device__ __forceinline__ void force_pair_vjp(const DevUniforms *__restrict__ U
   const float dr2 = dr * dr, r3 = r2 * r;
   const bool outer = r >= U->half_h;
   const float dr3 = dr2 * dr;
-  const float C = outer ? aks * dr3 * r3 : aks * (2.0f * dr3 * r3 - U->h6_64);
+  const float t3 = dr3 * r3;  // 2 dr3 r3 = 2 t3 exactly
+  const float C = aks * (outer ? t3 : __builtin_fmaf(2.0f, t3, -U->h6_64));
   const float smk = st * mass * K;
   const float coef = S1 * gk + (close ? -200.0f * (1.0f - r / md) : 0.f) + sv + smk * C;
   const float dnx = A.nrm.x - Bn.nrm.x, dny = A.nrm.y - Bn.nrm.y, dnz = A.nrm.z - Bn.nrm.z;
@@ -450,16 +448,12 @@ __device__ __forceinline__ void force_pair_vjp(const DevUniforms *__restrict__ U
   }
   float lg = lA * S1;
   float lr = close ? lA * 200.0f / md : 0.f;
-  float lr2 = 0.f;
-  float ldd = -lK * K * idd;  // d_A and d_B both enter K (and avg)
-  float lvdx = 0.f;
-  if (von) {
-    lg += lA * visc * mu * cv;
-    const float lmu = lA * visc * gk * cv;
-    ldd -= lA * sv * idd;
-    lvdx = lmu * h * iden;
-    lr2 -= lmu * mu * iden;
-  }
+  // the viscosity terms through selects (sv = 0 when !von)
+  const float lmu = von ? lA * visc * gk * cv : 0.f;
+  lg += von ? lA * visc * mu * cv : 0.f;
+  const float ldd = -lK * K * idd - lA * sv * idd;  // d_A, d_B enter K and avg
+  const float lvdx = lmu * h * iden;
+  float lr2 = -(lmu * mu * iden);
   const float lS1 = lA * gk;
   const float lsum = -lS1 * mass * kn * i2dn;  // through the numerator nd_A + nd_B
   if (NEED_A) {
@@ -476,9 +470,11 @@ __device__ __forceinline__ void force_pair_vjp(const DevUniforms *__restrict__ U
   const float dCdr = aks * 3.0f * dr2 * r2 * (dr - r) * (outer ? 1.0f : 2.0f);
   lr += lA * smk * dCdr - lg * 2.0f * gs * dr;
   float gx = lvdx * vx, gy = lvdx * vy, gz = lvdx * vz;
-  if (!close) {
-    // dir = -r_vec / r:  d dir / d r_vec = -(I - rh rh^T) / r
-    const float ldx = coef * lam.x, ldy = coef * lam.y, ldz = coef * lam.z;
+  {
+    // dir = -r_vec / r:  d dir / d r_vec = -(I - rh rh^T) / r; the close
+    // direction does not depend on r_vec (zero scale)
+    const float cd = close ? 0.f : coef;
+    const float ldx = cd * lam.x, ldy = cd * lam.y, ldz = cd * lam.z;
     const float pr = dot3(rx, ry, rz, ldx, ldy, ldz) * ir * ir;
     gx -= (ldx - pr * rx) * ir;
     gy -= (ldy - pr * ry) * ir;
@@ -617,7 +613,7 @@ __global__ __launch_bounds__(256) void k_adj_xsph(Uni un, AdjBufs a, TapeStep t,
   float gxx = 0.f, gxy = 0.f, gxz = 0.f, gux = 0.f, guy = 0.f, guz = 0.f, gd = 0.f;
   if (eps > 0.0f) {
     const float h2 = U->h2, mass = U->mass, ps = U->poly6_scale;
-    const float idib = 1.0f / fmaxf(ri0.w, 1.0f);
+    const float idib = 1.0f / max1(ri0.w);
     for_each_pair<L>(U, a, t, n, s, sub, xi, [&](uint32_t j, float4 xj) {
       const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
       const float r2 = dot3(rx, ry, rz, rx, ry, rz);
@@ -626,7 +622,7 @@ __global__ __launch_bounds__(256) void k_adj_xsph(Uni un, AdjBufs a, TapeStep t,
       const float d = h2 - r2;
       const float mW = mass * ps * d * d * d;
       const float mdW = mass * (-3.0f * ps) * d * d;  // d(m W)/d(r^2)
-      const float idjb = 1.0f / fmaxf(rj0.w, 1.0f);
+      const float idjb = 1.0f / max1(rj0.w);
       const float dux = uj.x - ui.x, duy = uj.y - ui.y, duz = uj.z - ui.z;
       // (i, j): eps m W / d_j (u_j - u_i) in w_i
       const float e1 = eps * dot3(lwi.x, lwi.y, lwi.z, dux, duy, duz);

@@ -73,8 +73,9 @@ struct DevUniforms {
   float max_accel_sq;  // max_accel^2                                    :261
   float boundary_layer;  // h * 0.4                 sph_integrate_3d.wgsl:113
   float interact_range;  // h * 0.7                                      :303
-  float md2_lo, md2_hi;  // (0.1 h)^2 (1 -/+ 1e-6): r < 0.1h is decided on
-                         // r^2 outside this band, on a rounded sqrt inside
+  float close_r2;        // smallest float r2 with sqrtf(r2) >= 0.1 h: the
+                         // correctly rounded test sqrtf(r2) < 0.1 h (:162)
+                         // is exactly r2 < close_r2, sqrtf being monotone
   float solve_skin2;     // (h + 2 dt^2 max_accel)^2 (1 + 1e-3): pairs farther
                          // apart at the first PCISPH iteration stay out of
                          // range in every later one

@@ -128,9 +128,14 @@ void fill_uniforms(DevUniforms &u, const GpuSphParams3D &p,
   u.max_accel_sq = u.max_accel * u.max_accel;
   u.boundary_layer = p.kernel_radius * 0.4f;
   u.interact_range = p.kernel_radius * 0.7f;
-  const double md2 = (double)u.min_dist * (double)u.min_dist;
-  u.md2_lo = (float)(md2 * (1.0 - 1e-6));
-  u.md2_hi = (float)(md2 * (1.0 + 1e-6));
+  {
+    // step from min_dist^2 to the first float whose rounded root reaches
+    // min_dist (a few ulps either way)
+    float t = u.min_dist * u.min_dist;
+    while (t > 0.0f && std::sqrt(std::nextafter(t, 0.0f)) >= u.min_dist) t = std::nextafter(t, 0.0f);
+    while (std::sqrt(t) < u.min_dist) t = std::nextafter(t, INFINITY);
+    u.close_r2 = t;
+  }
   // Predicted positions of iteration k >= 1 differ from iteration 0's by
   // dt^2 clamp(a_p), |clamp(a_p)| <= max_accel, for both particles of a pair.
   const double skin = (double)p.kernel_radius +

@@ -23,6 +23,14 @@ __device__ __forceinline__ float fast_sqrt(float x) {
   return __builtin_amdgcn_sqrtf(x);
 }
 
+// fmaxf(x, 1.0f) for every non-NaN x in one integer max: non-negative floats
+// order as their bit patterns, and negative ones (sign bit set) are negative
+// as int32, below 1.0f's pattern.  fmaxf itself costs two instructions here
+// (its operand is first canonicalized).
+__device__ __forceinline__ float max1(float x) {
+  return __int_as_float(max(__float_as_int(x), 0x3f800000));
+}
+
 // Inclusive wave64 scan on DPP: row_shr 1,2,4,8 within each 16-lane row,
 // then row_bcast:15 into rows 1 and 3 and row_bcast:31 into rows 2 and 3.
 __device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v) {

@@ -502,7 +502,7 @@ __device__ __forceinline__ float4 xsph_particle(const DevUniforms *__restrict__ 
     const float4 vj = srec[3 * (size_t)j + 1];
     const float d = h2 - r2;
     const float w = ps * d * d * d;
-    const float f = mass * __builtin_amdgcn_rcpf(fmaxf(pj.w, 1.0f)) * w;
+    const float f = mass * __builtin_amdgcn_rcpf(max1(pj.w)) * w;
     cxv += f * (vj.x - vi.x);
     cyv += f * (vj.y - vi.y);
     czv += f * (vj.z - vi.z);
@@ -539,13 +539,13 @@ __device__ __forceinline__ void force_particle(const DevUniforms *__restrict__ U
   const float4 *rec_i = b.srec + 3 * (size_t)s;
   const float4 pi = rec_i[0];
   const float4 ni = rec_i[2];
-  const float di = fmaxf(pi.w, 1.0f);
-  const float ndi = fmaxf(rec_i[1].w, 1.0f);
+  const float di = max1(pi.w);
+  const float ndi = max1(rec_i[1].w);
   const float h = U->h, mass = U->mass, near_k = U->near_k;
   const float gs = U->grad_scale, md = U->min_dist;
   const float veps = U->visc_eps, aks = U->akinci_scale;
   const float hh = U->half_h, h6_64 = U->h6_64;
-  const float md2_lo = U->md2_lo, md2_hi = U->md2_hi;
+  const float close_r2 = U->close_r2;
   const float npi = near_k * ndi;
   const float a_sp = -mass / (2.0f * di);
   const float c_visc = 2.0f * mass * U->visc * h;  // -m * (-visc * h / (avg))
@@ -560,12 +560,11 @@ __device__ __forceinline__ void force_particle(const DevUniforms *__restrict__ U
     const float r2 = dot3(rx, ry, rz, rx, ry, rz);
     const float4 vj = srec[3 * (size_t)j + 1];
     const float4 nj = srec[3 * (size_t)j + 2];
-    bool close = r2 < md2_lo;
-    if (!close && r2 < md2_hi) close = sqrtf(r2) < md;
+    const bool close = r2 < close_r2;
     const float il = __builtin_amdgcn_rsqf(r2);  // 1 / r
     const float r = r2 * il;
-    const float dj = fmaxf(pj.w, 1.0f);
-    const float ndj = fmaxf(vj.w, 1.0f);
+    const float dj = max1(pj.w);
+    const float ndj = max1(vj.w);
     const float dr = h - r;
     const float grad = gs * dr * dr;
     // near pressure (:183) and separation (:186-189)
@@ -579,18 +578,21 @@ __device__ __forceinline__ void force_particle(const DevUniforms *__restrict__ U
     const float mk = c_k * inv_dd;
     const float r3 = r * r * r;
     const float dr3 = dr * dr * dr;
-    const float coh = (r >= hh) ? aks * dr3 * r3 : aks * (2.0f * dr3 * r3 - h6_64);
+    // both pieces, then a select (2 dr3 r3 = 2 (dr3 r3) exactly)
+    const float t3 = dr3 * r3;
+    const float coh = aks * ((r >= hh) ? t3 : __builtin_fmaf(2.0f, t3, -h6_64));
     const float coef = __builtin_fmaf(mk, coh, sp);
+    // dir = -r_vec / r, or the slot-difference direction below 0.1 h, which
+    // is rare: the common update runs with a zero scale for close pairs
+    const float c = close ? 0.0f : -coef * il;
+    fx = __builtin_fmaf(c, rx, fx);
+    fy = __builtin_fmaf(c, ry, fy);
+    fz = __builtin_fmaf(c, rz, fz);
     if (close) {
       const float3 d3 = close_dir(s, j);
       fx = __builtin_fmaf(coef, d3.x, fx);
       fy = __builtin_fmaf(coef, d3.y, fy);
       fz = __builtin_fmaf(coef, d3.z, fz);
-    } else {
-      const float c = -coef * il;  // dir = -r_vec / r
-      fx = __builtin_fmaf(c, rx, fx);
-      fy = __builtin_fmaf(c, ry, fy);
-      fz = __builtin_fmaf(c, rz, fz);
     }
     fx = __builtin_fmaf(-mk, ni.x - nj.x, fx);
     fy = __builtin_fmaf(-mk, ni.y - nj.y, fy);
