@@ -55,15 +55,6 @@ __device__ __forceinline__ V3 group_sum3(V3 a) {
   return v3(group_sum<L>(a.x), group_sum<L>(a.y), group_sum<L>(a.z));
 }
 
-// The close-range direction of sph_force_3d_grid.wgsl:163-172 (a function
-// of the two slots only).
-__device__ __noinline__ float3 adj_close_dir(uint32_t s, uint32_t j) {
-  const float idx = (float)s - (float)j;
-  const float a = sinf(idx * 1.0f), c = cosf(idx * 2.0f), e = sinf(idx * 3.0f);
-  const float il = __builtin_amdgcn_rsqf(dot3(a, c, e, a, c, e));
-  return make_float3(a * il, c * il, e * il);
-}
-
 __device__ __forceinline__ void cell_of(const DevUniforms *U, float4 p, int &cx, int &cy,
                                         int &cz) {
   cx = cell_coord(p.x, U->ox, U->inv_cell);
@@ -407,7 +398,7 @@ __device__ __forceinline__ void force_pair_vjp(const DevUniforms *__restrict__ U
   const float r = r2 * ir;
   V3 dir = v3(-rx * ir, -ry * ir, -rz * ir);
   if (close) {  // rare: the slot-difference direction
-    const float3 c = adj_close_dir(A.slot, Bn.slot);
+    const float3 c = close_dir(A.slot, Bn.slot);
     dir = v3(c.x, c.y, c.z);
   }
   const float da = max1(A.d), nda = max1(A.nd);

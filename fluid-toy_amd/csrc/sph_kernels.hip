@@ -406,17 +406,6 @@ __global__ __launch_bounds__(256) void k_density_t(Uni un, Bufs b, uint32_t n) {
   }
 }
 
-// Direction of a pair closer than 0.1 h: normalize(sin(d), cos(2d), sin(3d))
-// of the sorted-slot difference d = s - j (sph_force_3d_grid.wgsl:163-172).
-// Kept out of line: it is rare, and inlined it would hold registers the
-// common path needs.
-__device__ __noinline__ float3 close_dir(uint32_t s, uint32_t j) {
-  const float idx = (float)s - (float)j;
-  const float a = sinf(idx * 1.0f), c = cosf(idx * 2.0f), e = sinf(idx * 3.0f);
-  const float il = __builtin_amdgcn_rsqf(dot3(a, c, e, a, c, e));
-  return make_float3(a * il, c * il, e * il);
-}
-
 // Walk lane g's in-range list, or, if it overflowed, the same candidate
 // subset again with the same test; f(j, p_j) sees every pair once.
 // Walk this lane's share of particle s's in-range list (entries sub, sub+L,
