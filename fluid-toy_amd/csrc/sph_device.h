@@ -25,19 +25,19 @@ __device__ __forceinline__ float fast_sqrt(float x) {
 
 // Direction of a pair closer than 0.1 h: normalize(sin(d), cos(2d), sin(3d))
 // of the sorted-slot difference d = f32(s) - f32(j), an integer
-// (sph_force_3d_grid.wgsl:163-172).  One sincos instead of three large-
-// argument sin/cos: d (|d| < 2^24) is reduced mod 2 pi in double (k < 2^22,
-// the fma leaves ~1e-15 before the 2 pi tail is taken off), then
-// cos 2d = (cos d - sin d)(cos d + sin d) and sin 3d = sin d (3 - 4 sin^2 d),
-// each within a few 1e-7 of the direct functions.  Out of line: the common
-// path does not need its registers.
-__device__ __noinline__ inline float3 close_dir(uint32_t s, uint32_t j) {
+// (sph_force_3d_grid.wgsl:163-172).  d (|d| < 2^24) is reduced to revolutions
+// f = d / (2 pi) - rint(d / (2 pi)) in double (error ~5e-10), and the
+// hardware sin / cos of 2 pi f (the instructions take revolutions; within
+// 1.3e-7 of sin, cos on [-1/2, 1/2], measured on 4M points) give sin d and
+// cos d; then cos 2d = (cos d - sin d)(cos d + sin d) and
+// sin 3d = sin d (3 - 4 sin^2 d).  Each component is within ~1e-6 of the
+// library functions of 2d and 3d, at a few dozen instructions instead of
+// three large-argument reductions.
+__device__ __forceinline__ float3 close_dir(uint32_t s, uint32_t j) {
   const float idx = (float)s - (float)j;
-  const double x = (double)idx;
-  const double k = rint(x * 0.15915494309189535);  // 1 / (2 pi)
-  const double r = fma(-k, 6.283185307179586, x) - k * 2.4492935982947064e-16;
-  float sn, cs;
-  sincosf((float)r, &sn, &cs);
+  const double q = (double)idx * 0.15915494309189535;  // 1 / (2 pi)
+  const float f = (float)(q - rint(q));
+  const float sn = __builtin_amdgcn_sinf(f), cs = __builtin_amdgcn_cosf(f);
   const float a = sn;
   const float c = (cs - sn) * (cs + sn);
   const float e = sn * __builtin_fmaf(-4.0f * sn, sn, 3.0f);
