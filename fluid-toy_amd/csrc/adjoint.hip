@@ -478,87 +478,16 @@ __device__ __forceinline__ void force_pair_vjp(const DevUniforms *__restrict__ U
   G.x_a = v3(gx, gy, gz);
 }
 
-template <int L, bool B>
-__global__ __launch_bounds__(256) void k_adj_force(Uni un, AdjBufs a, TapeStep t, uint32_t n) {
-  const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
-  const uint32_t s = g / L, sub = g % L;
-  if (s >= n) return;
-  const DevUniforms *__restrict__ U = uni_at<B>(un, s);
-  const float4 *ri = t.rec + 3 * (size_t)s;
-  const float4 xi = t.x[s];
-  const float4 rec0 = ri[0], rec1 = ri[1], rec2 = ri[2];
-  const PairSide own{xi, t.w[s], rec2, rec0.w, rec1.w, s};
-  const PairSide asn{xi, rec1, rec2, rec0.w, rec1.w, s};  // i as a neighbour: u_i
-  const V3 lai = v3(a.la[s]);
-  int cx, cy, cz;
-  cell_of(U, xi, cx, cy, cz);
-  const float h2 = U->h2;
-  V3 gx = zero3(), gw = zero3(), gu = zero3(), gn = zero3();
-  float gd = 0.f, gnd = 0.f, gv = 0.f, gst = 0.f;
-  const size_t stride = (size_t)n * L;
-  uint32_t *__restrict__ lst = a.plist + (size_t)s * L + sub;
-  constexpr uint32_t kCap = kListCapacity / L;
-  uint32_t cnt = 0;
-  sweep27<L, false>(U, t.counts, t.starts, t.x, cx, cy, cz, sub,
-                    [&](uint32_t j, float4 xj, bool) {
-    const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
-    const float r2 = dot3(rx, ry, rz, rx, ry, rz);
-    if (!(r2 < h2 && r2 > 1e-12f)) return;
-    if (cnt < kCap) lst[(size_t)cnt * stride] = j;
-    ++cnt;
-    const float4 *rj = t.rec + 3 * (size_t)j;
-    const float4 rj0 = rj[0], rj1 = rj[1], rj2 = rj[2];
-    PairGrad G;
-    // (i, j): the term of a_i
-    const PairSide nb{xj, rj1, rj2, rj0.w, rj1.w, j};
-    force_pair_vjp<true, false>(U, own, nb, lai, G);
-    gx = vadd(gx, G.x_a);
-    gw = vadd(gw, G.v_a);
-    gn = vadd(gn, G.n_a);
-    gd += G.d_a;
-    gnd += G.nd_a;
-    gv += G.visc;
-    gst += G.st;
-    // (j, i): the term of a_j that reads particle i
-    const PairSide oj{xj, t.w[j], rj2, rj0.w, rj1.w, j};
-    force_pair_vjp<false, true>(U, oj, asn, v3(a.la[j]), G);
-    gx = vsub(gx, G.x_a);
-    gu = vadd(gu, G.v_b);
-    gn = vadd(gn, G.n_b);
-    gd += G.d_b;
-    gnd += G.nd_b;
-  });
-  a.pcnt[(size_t)s * L + sub] = cnt <= kCap ? cnt : kListOverflow;
-  gx = group_sum3<L>(gx);
-  gw = group_sum3<L>(gw);
-  gu = group_sum3<L>(gu);
-  gn = group_sum3<L>(gn);
-  gd = group_sum<L>(gd);
-  gnd = group_sum<L>(gnd);
-  gv = group_sum<L>(gv);
-  gst = group_sum<L>(gst);
-  if (sub != 0) return;
-  a.lx[s] = f4(vadd(v3(a.lx[s]), gx));
-  a.lw[s] = f4(vadd(v3(a.lw[s]), gw));
-  a.lu[s] = f4(gu);
-  a.ln[s] = f4(gn);
-  a.lrho[s] = make_float2(gd, gnd);
-  double *acc = a.acc + (size_t)kAccStride * s;
-  acc[0] += (double)gv;
-  acc[1] += (double)gst;
-  acc[2] += (double)lai.x;  // the force adds gravity to every particle
-  acc[3] += (double)lai.y;
-  acc[4] += (double)lai.z;
-}
-
-// The in-range pairs of this lane: the list the force adjoint recorded, or
-// (after an overflow) the 27-cell sweep with the same test.  f(j, x_j).
+// The in-range pairs of this lane: its share (entries sub, sub + L, ...) of
+// the balanced list the force adjoint recorded (total entries: pcnt), or
+// (after an overflow) the lane's candidates of the 27-cell sweep with the
+// same test.  f(j, x_j).
 template <int L, class F>
 __device__ __forceinline__ void for_each_pair(const DevUniforms *__restrict__ U, const AdjBufs &a,
                                               const TapeStep &t, uint32_t n, uint32_t s,
-                                              uint32_t sub, float4 xi, F &&f) {
-  const uint32_t cnt = a.pcnt[(size_t)s * L + sub];
-  if (cnt != kListOverflow) {
+                                              uint32_t sub, float4 xi, uint32_t total, F &&f) {
+  if (total <= kListCapacity) {
+    const uint32_t cnt = total > sub ? (total - 1 - sub) / L + 1 : 0;
     const size_t stride = (size_t)n * L;
     const uint32_t *__restrict__ lst = a.plist + (size_t)s * L + sub;
     uint32_t e = 0;
@@ -585,6 +514,86 @@ __device__ __forceinline__ void for_each_pair(const DevUniforms *__restrict__ U,
   });
 }
 
+template <int L, bool B>
+__global__ __launch_bounds__(256) void k_adj_force(Uni un, AdjBufs a, TapeStep t, uint32_t n) {
+  const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t s = g / L, sub = g % L;
+  if (s >= n) return;
+  const DevUniforms *__restrict__ U = uni_at<B>(un, s);
+  const float4 *ri = t.rec + 3 * (size_t)s;
+  const float4 xi = t.x[s];
+  const float4 rec0 = ri[0], rec1 = ri[1], rec2 = ri[2];
+  const PairSide own{xi, t.w[s], rec2, rec0.w, rec1.w, s};
+  const PairSide asn{xi, rec1, rec2, rec0.w, rec1.w, s};  // i as a neighbour: u_i
+  const V3 lai = v3(a.la[s]);
+  int cx, cy, cz;
+  cell_of(U, xi, cx, cy, cz);
+  const float h2 = U->h2;
+  V3 gx = zero3(), gw = zero3(), gu = zero3(), gn = zero3();
+  float gd = 0.f, gnd = 0.f, gv = 0.f, gst = 0.f;
+  // Pass 1: the in-range pairs into a ballot-balanced list (a cheap
+  // group-synchronous sweep); pass 2 walks it, so the heavy per-pair body
+  // runs with the group's lanes evenly loaded instead of on the sweep's
+  // in-range lanes only.
+  // Every lane of the group ends pass 1 with the group's count in `run`, and
+  // pass 2 uses that, not pcnt (lane 0's store).  Pass 2 reads entries other
+  // lanes of the wave stored in pass 1: the workgroup fence between the
+  // passes waits for those stores and keeps the loads after them.
+  uint32_t run = 0;
+  sweep27<L, true>(U, t.counts, t.starts, t.x, cx, cy, cz, sub,
+                   [&](uint32_t j, float4 xj, bool valid) {
+    const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
+    const float r2 = dot3(rx, ry, rz, rx, ry, rz);
+    group_list_append<L>(a.plist + (size_t)s * L, (size_t)n * L, sub, run,
+                         valid && r2 < h2 && r2 > 1e-12f, j);
+  });
+  if (sub == 0) a.pcnt[s] = run;
+  __threadfence_block();
+  for_each_pair<L>(U, a, t, n, s, sub, xi, run, [&](uint32_t j, float4 xj) {
+    const float4 *rj = t.rec + 3 * (size_t)j;
+    const float4 rj0 = rj[0], rj1 = rj[1], rj2 = rj[2];
+    PairGrad G;
+    // (i, j): the term of a_i
+    const PairSide nb{xj, rj1, rj2, rj0.w, rj1.w, j};
+    force_pair_vjp<true, false>(U, own, nb, lai, G);
+    gx = vadd(gx, G.x_a);
+    gw = vadd(gw, G.v_a);
+    gn = vadd(gn, G.n_a);
+    gd += G.d_a;
+    gnd += G.nd_a;
+    gv += G.visc;
+    gst += G.st;
+    // (j, i): the term of a_j that reads particle i
+    const PairSide oj{xj, t.w[j], rj2, rj0.w, rj1.w, j};
+    force_pair_vjp<false, true>(U, oj, asn, v3(a.la[j]), G);
+    gx = vsub(gx, G.x_a);
+    gu = vadd(gu, G.v_b);
+    gn = vadd(gn, G.n_b);
+    gd += G.d_b;
+    gnd += G.nd_b;
+  });
+  gx = group_sum3<L>(gx);
+  gw = group_sum3<L>(gw);
+  gu = group_sum3<L>(gu);
+  gn = group_sum3<L>(gn);
+  gd = group_sum<L>(gd);
+  gnd = group_sum<L>(gnd);
+  gv = group_sum<L>(gv);
+  gst = group_sum<L>(gst);
+  if (sub != 0) return;
+  a.lx[s] = f4(vadd(v3(a.lx[s]), gx));
+  a.lw[s] = f4(vadd(v3(a.lw[s]), gw));
+  a.lu[s] = f4(gu);
+  a.ln[s] = f4(gn);
+  a.lrho[s] = make_float2(gd, gnd);
+  double *acc = a.acc + (size_t)kAccStride * s;
+  acc[0] += (double)gv;
+  acc[1] += (double)gst;
+  acc[2] += (double)lai.x;  // the force adds gravity to every particle
+  acc[3] += (double)lai.y;
+  acc[4] += (double)lai.z;
+}
+
 // ------------------------------------------------------------------------
 // XSPH (xsph.wgsl:81-138): w_i = u_i + eps sum_j m W(r^2) / max(rho_j, 1)
 // (u_j - u_i).  The epilogue reverses the density stage's rho = S / gamma(x)
@@ -605,7 +614,7 @@ __global__ __launch_bounds__(256) void k_adj_xsph(Uni un, AdjBufs a, TapeStep t,
   if (eps > 0.0f) {
     const float h2 = U->h2, mass = U->mass, ps = U->poly6_scale;
     const float idib = 1.0f / max1(ri0.w);
-    for_each_pair<L>(U, a, t, n, s, sub, xi, [&](uint32_t j, float4 xj) {
+    for_each_pair<L>(U, a, t, n, s, sub, xi, a.pcnt[s], [&](uint32_t j, float4 xj) {
       const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
       const float r2 = dot3(rx, ry, rz, rx, ry, rz);
       const float4 rj0 = t.rec[3 * (size_t)j], uj = t.rec[3 * (size_t)j + 1];
@@ -699,7 +708,7 @@ __global__ __launch_bounds__(256) void k_adj_density(Uni un, AdjBufs a, TapeStep
   const float c_n = mass * (-3.0f * U->near_scale);   // d(m ns (h-r)^3)/dr / (h-r)^2
   const float cn = U->normal_scale * U->m_over_rho0;
   float gx = 0.f, gy = 0.f, gz = 0.f;
-  for_each_pair<L>(U, a, t, n, s, sub, xi, [&](uint32_t j, float4 xj) {
+  for_each_pair<L>(U, a, t, n, s, sub, xi, a.pcnt[s], [&](uint32_t j, float4 xj) {
     const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
     const float r2 = dot3(rx, ry, rz, rx, ry, rz);
     const float2 lsj = a.lrho[j];

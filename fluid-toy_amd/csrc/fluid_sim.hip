@@ -379,7 +379,7 @@ struct AdjViews {
   float *in_x, *in_v, *out_x, *out_v;  // canonical (3 floats per particle) staging
   uint32_t *slot_first;               // scenes + 1 slot offsets
   double *grads;                      // scenes x kGradParams
-  uint32_t *pcnt;                     // per-lane in-range pair counts (8 per slot)
+  uint32_t *pcnt;                     // in-range pair count per slot
 };
 
 AdjViews carve_adj(Carve &c, size_t n, size_t scenes) {
@@ -405,7 +405,7 @@ AdjViews carve_adj(Carve &c, size_t n, size_t scenes) {
   a.out_v = c.take<float>(3 * n);
   a.slot_first = c.take<uint32_t>(scenes + 1);
   a.grads = c.take<double>(scenes * kGradParams);
-  a.pcnt = c.take<uint32_t>(8 * n);
+  a.pcnt = c.take<uint32_t>(n);
   return a;
 }
 

@@ -45,6 +45,26 @@ __device__ __forceinline__ float3 close_dir(uint32_t s, uint32_t j) {
   return make_float3(a * il, c * il, e * il);
 }
 
+// Appends j to a lane group's in-range list when `pair`.  Every lane of the
+// group calls it at the same point of a group-synchronous walk; the group's
+// k-th entry (ranked across its lanes by a ballot, lanes in order) goes to
+// lane k % L's share at depth k / L, i.e. list[(k / L) * stride + k % L]
+// with list = base + slot * L and stride = n * L, so each lane of a later
+// walk over the list takes an equal share.  run counts the group's entries
+// (above kListCapacity the list is incomplete and the count marks it).
+template <int L>
+__device__ __forceinline__ void group_list_append(uint32_t *__restrict__ list, size_t stride,
+                                                  uint32_t sub, uint32_t &run, bool pair,
+                                                  uint32_t j) {
+  const uint32_t gshift = (threadIdx.x & 63u) & ~(uint32_t)(L - 1);
+  const uint32_t gm = (uint32_t)(__ballot(pair) >> gshift) & ((1u << L) - 1u);
+  if (pair) {
+    const uint32_t e = run + (uint32_t)__popc(gm & ((1u << sub) - 1u));
+    if (e < kListCapacity) list[(e / L) * stride + (e % L)] = j;
+  }
+  run += (uint32_t)__popc(gm);
+}
+
 // fmaxf(x, 1.0f) for every non-NaN x in one integer max: non-negative floats
 // order as their bit patterns, and negative ones (sign bit set) are negative
 // as int32, below 1.0f's pattern.  fmaxf itself costs two instructions here

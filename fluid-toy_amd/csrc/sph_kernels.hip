@@ -295,16 +295,10 @@ __device__ __forceinline__ void density_particle(const DevUniforms *__restrict__
   const float ps = U->poly6_scale, ns = U->near_scale, mr = U->m_over_rho0;
   const size_t stride = (size_t)n * L;
   uint32_t *__restrict__ nbr = b.nbr + (size_t)s * L;
-  const uint32_t lane = threadIdx.x & 63u;
-  const uint32_t gshift = lane & ~(uint32_t)(L - 1);
-  const uint32_t below = (1u << sub) - 1u;
   uint32_t run = 0;  // pairs recorded so far by the whole group
   float dens = 0.f, near = 0.f, nx = 0.f, ny = 0.f, nz = 0.f;
   // Branch-free body: a pair outside the support contributes exact zeros
-  // through selects, so the lanes of a wave stay converged.  The group's
-  // k-th recorded pair (ranked across lanes by a ballot, in a fixed order)
-  // goes to lane k % L's sub-list at depth k / L, so every lane of XSPH and
-  // force later walks an equal share.
+  // through selects, so the lanes of a wave stay converged.
   sweep([&](uint32_t j, float4 pj, bool valid) {
     const float rx = pi.x - pj.x, ry = pi.y - pj.y, rz = pi.z - pj.z;
     const float r2 = dot3(rx, ry, rz, rx, ry, rz);
@@ -318,12 +312,7 @@ __device__ __forceinline__ void density_particle(const DevUniforms *__restrict__
     nx += fn * rx;
     ny += fn * ry;
     nz += fn * rz;
-    const uint32_t gm = (uint32_t)(__ballot(pair) >> gshift) & ((1u << L) - 1u);
-    if (pair) {
-      const uint32_t e = run + (uint32_t)__popc(gm & below);
-      if (e < kListCapacity) nbr[(e / L) * stride + (e % L)] = j;
-    }
-    run += (uint32_t)__popc(gm);
+    group_list_append<L>(nbr, stride, sub, run, pair, j);
   });
   if (sub == 0) b.ncnt[s] = run;
   dens = group_sum<L>(dens);
