@@ -271,8 +271,7 @@ __global__ __launch_bounds__(256) void k_adj_solve(Uni un, AdjBufs a, TapeStep t
   const float h = U->h, h2 = U->h2, mass = U->mass, gs = U->grad_scale;
   const float c_rho = mass * (-6.0f * U->poly6_scale);  // 2 d(m W)/d(r^2) = c_rho d^2
   float lXx = 0.f, lXy = 0.f, lXz = 0.f, lq = 0.f;
-  const size_t stride = (size_t)n * L;
-  uint32_t *__restrict__ lst = a.slist + (size_t)s * L + sub;
+  uint32_t *__restrict__ lst = a.slist + lane_block<L>(s, kSolveListCapacity) + sub;
   constexpr uint32_t kCap = kSolveListCapacity / L;
   auto pair = [&](uint32_t j, float4 xj, bool) {
     const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
@@ -309,7 +308,7 @@ __global__ __launch_bounds__(256) void k_adj_solve(Uni un, AdjBufs a, TapeStep t
       const float4 x0j = xq0[j];
       const float ex = x0i.x - x0j.x, ey = x0i.y - x0j.y, ez = x0i.z - x0j.z;
       if (dot3(ex, ey, ez, ex, ey, ez) < skin2) {
-        if (cnt < kCap) lst[(size_t)cnt * stride] = j;
+        if (cnt < kCap) lst[cnt * 64u] = j;
         ++cnt;
       }
       pair(j, xj, v);
@@ -320,8 +319,8 @@ __global__ __launch_bounds__(256) void k_adj_solve(Uni un, AdjBufs a, TapeStep t
     if (cnt != kSolveListOverflow) {
       uint32_t e = 0;
       for (; e + 3 < cnt; e += 4) {
-        const uint32_t j0 = lst[(size_t)e * stride], j1 = lst[(size_t)(e + 1) * stride];
-        const uint32_t j2 = lst[(size_t)(e + 2) * stride], j3 = lst[(size_t)(e + 3) * stride];
+        const uint32_t j0 = lst[e * 64u], j1 = lst[e * 64u + 64u];
+        const uint32_t j2 = lst[e * 64u + 128u], j3 = lst[e * 64u + 192u];
         const float4 x0 = in.xq[j0], x1 = in.xq[j1], x2 = in.xq[j2], x3 = in.xq[j3];
         pair(j0, x0, true);
         pair(j1, x1, true);
@@ -329,7 +328,7 @@ __global__ __launch_bounds__(256) void k_adj_solve(Uni un, AdjBufs a, TapeStep t
         pair(j3, x3, true);
       }
       for (; e < cnt; ++e) {
-        const uint32_t j = lst[(size_t)e * stride];
+        const uint32_t j = lst[e * 64u];
         pair(j, in.xq[j], true);
       }
     } else {
@@ -488,17 +487,16 @@ __device__ __forceinline__ void for_each_pair(const DevUniforms *__restrict__ U,
                                               uint32_t sub, float4 xi, uint32_t total, F &&f) {
   if (total <= kListCapacity) {
     const uint32_t cnt = total > sub ? (total - 1 - sub) / L + 1 : 0;
-    const size_t stride = (size_t)n * L;
-    const uint32_t *__restrict__ lst = a.plist + (size_t)s * L + sub;
+    const uint32_t *__restrict__ lst = a.plist + lane_block<L>(s, kListCapacity) + sub;
     uint32_t e = 0;
     for (; e + 1 < cnt; e += 2) {
-      const uint32_t j0 = lst[(size_t)e * stride], j1 = lst[(size_t)(e + 1) * stride];
+      const uint32_t j0 = lst[e * 64u], j1 = lst[e * 64u + 64u];
       const float4 x0 = t.x[j0], x1 = t.x[j1];
       f(j0, x0);
       f(j1, x1);
     }
     if (e < cnt) {
-      const uint32_t j0 = lst[(size_t)e * stride];
+      const uint32_t j0 = lst[e * 64u];
       f(j0, t.x[j0]);
     }
     return;
@@ -544,7 +542,7 @@ __global__ __launch_bounds__(256) void k_adj_force(Uni un, AdjBufs a, TapeStep t
                    [&](uint32_t j, float4 xj, bool valid) {
     const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
     const float r2 = dot3(rx, ry, rz, rx, ry, rz);
-    group_list_append<L>(a.plist + (size_t)s * L, (size_t)n * L, sub, run,
+    group_list_append<L>(a.plist + lane_block<L>(s, kListCapacity), sub, run,
                          valid && r2 < h2 && r2 > 1e-12f, j);
   });
   if (sub == 0) a.pcnt[s] = run;

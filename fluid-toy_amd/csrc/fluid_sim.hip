@@ -644,9 +644,10 @@ static int create_handle(int device, std::vector<SceneState> scenes,
   ALLOC(svel, float4 *, N);
   ALLOC(sfrc, float4 *, N);
   ALLOC(srec, float4 *, 3 * N);
-  ALLOC(nbr, uint32_t *, (size_t)kListCapacity * N);
+  // per-lane list blocks cover whole waves: round the slots up by 64
+  ALLOC(nbr, uint32_t *, (size_t)kListCapacity * (N + 64));
   ALLOC(ncnt, uint32_t *, N);
-  ALLOC(slist, uint32_t *, (size_t)kSolveListCapacity * N);
+  ALLOC(slist, uint32_t *, (size_t)kSolveListCapacity * (N + 64));
   ALLOC(scnt, uint32_t *, 8 * N);
   ALLOC(sid, uint32_t *, N);
   ALLOC(counts, uint32_t *, C);

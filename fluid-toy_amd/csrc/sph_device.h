@@ -45,22 +45,37 @@ __device__ __forceinline__ float3 close_dir(uint32_t s, uint32_t j) {
   return make_float3(a * il, c * il, e * il);
 }
 
+// Per-lane lists (the in-range lists, the solve's skin lists) live in
+// blocks of the P = 64 / L slots one wave covers: entry k of lane `sub` of
+// slot s is at list[lane_block<L>(s, cap) + k * 64 + sub], cap entries per
+// slot.  A wave's k-th entries are one contiguous 64-word run, and offsets
+// within a block are 32-bit (the block base is computed once per slot).
+template <int L>
+__device__ __forceinline__ size_t lane_block(uint32_t s, uint32_t cap) {
+  constexpr uint32_t P = 64 / L;
+  return (size_t)(s / P) * (P * cap) + (s % P) * L;
+}
+
 // Appends j to a lane group's in-range list when `pair`.  Every lane of the
 // group calls it at the same point of a group-synchronous walk; the group's
 // k-th entry (ranked across its lanes by a ballot, lanes in order) goes to
-// lane k % L's share at depth k / L, i.e. list[(k / L) * stride + k % L]
-// with list = base + slot * L and stride = n * L, so each lane of a later
-// walk over the list takes an equal share.  run counts the group's entries
-// (above kListCapacity the list is incomplete and the count marks it).
+// lane k % L at depth k / L, i.e. blk[(k / L) * 64 + k % L] with
+// blk = list + lane_block<L>(slot, kListCapacity), so each lane of a later
+// walk takes an equal share.  run counts the group's entries (above
+// kListCapacity the list is incomplete and the count marks it).  The group's
+// ballot bits are byte lane / 8 of the wave's ballot (one byte permute),
+// shifted down for L < 8.
 template <int L>
-__device__ __forceinline__ void group_list_append(uint32_t *__restrict__ list, size_t stride,
-                                                  uint32_t sub, uint32_t &run, bool pair,
-                                                  uint32_t j) {
-  const uint32_t gshift = (threadIdx.x & 63u) & ~(uint32_t)(L - 1);
-  const uint32_t gm = (uint32_t)(__ballot(pair) >> gshift) & ((1u << L) - 1u);
+__device__ __forceinline__ void group_list_append(uint32_t *__restrict__ blk, uint32_t sub,
+                                                  uint32_t &run, bool pair, uint32_t j) {
+  const uint32_t lane = threadIdx.x & 63u;
+  const uint64_t bal = __ballot(pair);
+  const uint32_t byte = __builtin_amdgcn_perm((uint32_t)(bal >> 32), (uint32_t)bal,
+                                              0x0c0c0c00u | (lane >> 3));
+  const uint32_t gm = L == 8 ? byte : (byte >> (lane & 7u & ~(uint32_t)(L - 1))) & ((1u << L) - 1u);
   if (pair) {
     const uint32_t e = run + (uint32_t)__popc(gm & ((1u << sub) - 1u));
-    if (e < kListCapacity) list[(e / L) * stride + (e % L)] = j;
+    if (e < kListCapacity) blk[(e / L) * 64u + (e % L)] = j;
   }
   run += (uint32_t)__popc(gm);
 }

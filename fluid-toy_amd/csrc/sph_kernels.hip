@@ -291,10 +291,10 @@ template <int L, class Sweep>
 __device__ __forceinline__ void density_particle(const DevUniforms *__restrict__ U,
                                                  const Bufs &b, uint32_t n, uint32_t s,
                                                  uint32_t sub, float4 pi, Sweep &&sweep) {
-  const float h = U->h, h2 = U->h2, mass = U->mass;
-  const float ps = U->poly6_scale, ns = U->near_scale, mr = U->m_over_rho0;
-  const size_t stride = (size_t)n * L;
-  uint32_t *__restrict__ nbr = b.nbr + (size_t)s * L;
+  const float h = U->h, h2 = U->h2;
+  const float mps = U->mass * U->poly6_scale, mns = U->mass * U->near_scale;
+  const float mr = U->m_over_rho0;
+  uint32_t *__restrict__ nbr = b.nbr + lane_block<L>(s, kListCapacity);
   uint32_t run = 0;  // pairs recorded so far by the whole group
   float dens = 0.f, near = 0.f, nx = 0.f, ny = 0.f, nz = 0.f;
   // Branch-free body: a pair outside the support contributes exact zeros
@@ -305,14 +305,15 @@ __device__ __forceinline__ void density_particle(const DevUniforms *__restrict__
     const bool inr = valid && r2 < h2;
     const bool pair = inr && r2 > 1e-12f;
     const float d = inr ? h2 - r2 : 0.f;
-    dens += mass * (ps * d * d * d);
+    const float d2 = d * d;
+    dens = __builtin_fmaf(mps, d2 * d, dens);
     const float dn = inr ? h - fast_sqrt(r2) : 0.f;
-    near += mass * (ns * dn * dn * dn);
-    const float fn = pair ? mr * d * d : 0.f;
+    near = __builtin_fmaf(mns, dn * dn * dn, near);
+    const float fn = pair ? mr * d2 : 0.f;
     nx += fn * rx;
     ny += fn * ry;
     nz += fn * rz;
-    group_list_append<L>(nbr, stride, sub, run, pair, j);
+    group_list_append<L>(nbr, sub, run, pair, j);
   });
   if (sub == 0) b.ncnt[s] = run;
   dens = group_sum<L>(dens);
@@ -408,13 +409,12 @@ __device__ __forceinline__ void for_each_neighbour(const DevUniforms *__restrict
   const float4 *__restrict__ srec = b.srec;
   if (total <= kListCapacity) {
     const uint32_t cnt = total > sub ? (total - 1 - sub) / L + 1 : 0;
-    const size_t stride = (size_t)n * L;
-    const uint32_t *__restrict__ nbr = b.nbr + (size_t)s * L + sub;
+    const uint32_t *__restrict__ nbr = b.nbr + lane_block<L>(s, kListCapacity) + sub;
     uint32_t k = 0;
     if (UNROLL == 4) {
       for (; k + 3 < cnt; k += 4) {
-        const uint32_t j0 = nbr[0], j1 = nbr[stride], j2 = nbr[2 * stride], j3 = nbr[3 * stride];
-        nbr += 4 * stride;
+        const uint32_t j0 = nbr[0], j1 = nbr[64], j2 = nbr[128], j3 = nbr[192];
+        nbr += 256;
         const float4 p0 = srec[3 * (size_t)j0], p1 = srec[3 * (size_t)j1];
         const float4 p2 = srec[3 * (size_t)j2], p3 = srec[3 * (size_t)j3];
         f(j0, p0);
@@ -425,11 +425,11 @@ __device__ __forceinline__ void for_each_neighbour(const DevUniforms *__restrict
     } else if (cnt >= 2) {
       // two pairs per trip; the next trip's list entries are loaded one trip
       // ahead so only the record gather waits on memory
-      uint32_t j0 = nbr[0], j1 = nbr[stride];
-      nbr += 2 * stride;
+      uint32_t j0 = nbr[0], j1 = nbr[64];
+      nbr += 128;
       for (; k + 3 < cnt; k += 2) {
-        const uint32_t n0 = nbr[0], n1 = nbr[stride];
-        nbr += 2 * stride;
+        const uint32_t n0 = nbr[0], n1 = nbr[64];
+        nbr += 128;
         const float4 p0 = srec[3 * (size_t)j0], p1 = srec[3 * (size_t)j1];
         f(j0, p0);
         f(j1, p1);
@@ -443,7 +443,7 @@ __device__ __forceinline__ void for_each_neighbour(const DevUniforms *__restrict
     }
     for (; k < cnt; ++k) {
       const uint32_t j0 = nbr[0];
-      nbr += stride;
+      nbr += 64;
       f(j0, srec[3 * (size_t)j0]);
     }
     return;
@@ -655,8 +655,7 @@ __device__ __forceinline__ void solve_particle(const DevUniforms *__restrict__ U
                                                float4 xi, float4 p0, Sweep &&sweep) {
   const float h = U->h, h2 = U->h2, mass = U->mass, ps = U->poly6_scale;
   const float gs = U->grad_scale;
-  const size_t stride = (size_t)n * L;
-  uint32_t *__restrict__ lst = b.slist + (size_t)s * L + sub;
+  uint32_t *__restrict__ lst = b.slist + lane_block<L>(s, kSolveListCapacity) + sub;
   constexpr uint32_t kCap = kSolveListCapacity / L;
   float pd = 0.f, ax = 0.f, ay = 0.f, az = 0.f;
   if (first_iter) {
@@ -671,7 +670,7 @@ __device__ __forceinline__ void solve_particle(const DevUniforms *__restrict__ U
       const float d = r2 < h2 ? h2 - r2 : 0.f;
       pd += mass * (ps * d * d * d);
       if (use_list && r2 < skin2) {
-        if (cnt < kCap) lst[(size_t)cnt * stride] = j;
+        if (cnt < kCap) lst[cnt * 64u] = j;
         ++cnt;
       }
     });
@@ -699,8 +698,8 @@ __device__ __forceinline__ void solve_particle(const DevUniforms *__restrict__ U
       const float4 *__restrict__ xq = in.xq;
       uint32_t k = 0;
       for (; k + 3 < cnt; k += 4) {
-        const uint32_t j0 = lst[(size_t)k * stride], j1 = lst[(size_t)(k + 1) * stride];
-        const uint32_t j2 = lst[(size_t)(k + 2) * stride], j3 = lst[(size_t)(k + 3) * stride];
+        const uint32_t j0 = lst[k * 64u], j1 = lst[k * 64u + 64u];
+        const uint32_t j2 = lst[k * 64u + 128u], j3 = lst[k * 64u + 192u];
         const float4 x0 = xq[j0], x1 = xq[j1], x2 = xq[j2], x3 = xq[j3];
         body(j0, x0, true);
         body(j1, x1, true);
@@ -708,7 +707,7 @@ __device__ __forceinline__ void solve_particle(const DevUniforms *__restrict__ U
         body(j3, x3, true);
       }
       for (; k < cnt; ++k) {
-        const uint32_t j0 = lst[(size_t)k * stride];
+        const uint32_t j0 = lst[k * 64u];
         body(j0, xq[j0], true);
       }
     } else {
