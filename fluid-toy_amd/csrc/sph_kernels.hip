@@ -396,13 +396,13 @@ __global__ __launch_bounds__(256) void k_density_t(Uni un, Bufs b, uint32_t n) {
   }
 }
 
-// Walk lane g's in-range list, or, if it overflowed, the same candidate
-// subset again with the same test; f(j, p_j) sees every pair once.
 // Walk this lane's share of particle s's in-range list (entries sub, sub+L,
 // ... of the group's list), or, if the list overflowed, the lane's candidate
-// subset again with the same test; f(j, p_j) sees every pair once.
-template <int L, int UNROLL, class F>
-__device__ __forceinline__ void for_each_neighbour(const DevUniforms *__restrict__ U,
+// subset again with the same test; f(j, p_j) sees every pair once.  Full
+// chunks of U pairs per trip, the next chunk's list entries loaded a trip
+// ahead so only the record gathers wait on memory.
+template <int L, int U, class F>
+__device__ __forceinline__ void for_each_neighbour(const DevUniforms *__restrict__ Un,
                                                    const Bufs &b, uint32_t n, uint32_t s,
                                                    uint32_t sub, float4 pi, F &&f) {
   const uint32_t total = b.ncnt[s];
@@ -411,35 +411,30 @@ __device__ __forceinline__ void for_each_neighbour(const DevUniforms *__restrict
     const uint32_t cnt = total > sub ? (total - 1 - sub) / L + 1 : 0;
     const uint32_t *__restrict__ nbr = b.nbr + lane_block<L>(s, kListCapacity) + sub;
     uint32_t k = 0;
-    if (UNROLL == 4) {
-      for (; k + 3 < cnt; k += 4) {
-        const uint32_t j0 = nbr[0], j1 = nbr[64], j2 = nbr[128], j3 = nbr[192];
-        nbr += 256;
-        const float4 p0 = srec[3 * (size_t)j0], p1 = srec[3 * (size_t)j1];
-        const float4 p2 = srec[3 * (size_t)j2], p3 = srec[3 * (size_t)j3];
-        f(j0, p0);
-        f(j1, p1);
-        f(j2, p2);
-        f(j3, p3);
+    if (cnt >= U) {
+      uint32_t jn[U];
+#pragma unroll
+      for (int q = 0; q < U; ++q) jn[q] = nbr[q * 64];
+      nbr += U * 64;
+      k = U;
+      while (true) {
+        uint32_t jc[U];
+#pragma unroll
+        for (int q = 0; q < U; ++q) jc[q] = jn[q];
+        const bool more = k + U <= cnt;
+        if (more) {
+#pragma unroll
+          for (int q = 0; q < U; ++q) jn[q] = nbr[q * 64];
+          nbr += U * 64;
+          k += U;
+        }
+        float4 p[U];
+#pragma unroll
+        for (int q = 0; q < U; ++q) p[q] = srec[3 * (size_t)jc[q]];
+#pragma unroll
+        for (int q = 0; q < U; ++q) f(jc[q], p[q]);
+        if (!more) break;
       }
-    } else if (cnt >= 2) {
-      // two pairs per trip; the next trip's list entries are loaded one trip
-      // ahead so only the record gather waits on memory
-      uint32_t j0 = nbr[0], j1 = nbr[64];
-      nbr += 128;
-      for (; k + 3 < cnt; k += 2) {
-        const uint32_t n0 = nbr[0], n1 = nbr[64];
-        nbr += 128;
-        const float4 p0 = srec[3 * (size_t)j0], p1 = srec[3 * (size_t)j1];
-        f(j0, p0);
-        f(j1, p1);
-        j0 = n0;
-        j1 = n1;
-      }
-      const float4 p0 = srec[3 * (size_t)j0], p1 = srec[3 * (size_t)j1];
-      f(j0, p0);
-      f(j1, p1);
-      k += 2;
     }
     for (; k < cnt; ++k) {
       const uint32_t j0 = nbr[0];
@@ -448,11 +443,11 @@ __device__ __forceinline__ void for_each_neighbour(const DevUniforms *__restrict
     }
     return;
   }
-  const float h2 = U->h2;
-  const int cx = cell_coord(pi.x, U->ox, U->inv_cell);
-  const int cy = cell_coord(pi.y, U->oy, U->inv_cell);
-  const int cz = cell_coord(pi.z, U->oz, U->inv_cell);
-  sweep27<L, false>(U, b.counts, b.starts, b.spos, cx, cy, cz, sub,
+  const float h2 = Un->h2;
+  const int cx = cell_coord(pi.x, Un->ox, Un->inv_cell);
+  const int cy = cell_coord(pi.y, Un->oy, Un->inv_cell);
+  const int cz = cell_coord(pi.z, Un->oz, Un->inv_cell);
+  sweep27<L, false>(Un, b.counts, b.starts, b.spos, cx, cy, cz, sub,
                     [&](uint32_t j, float4 pj, bool) {
     const float rx = pi.x - pj.x, ry = pi.y - pj.y, rz = pi.z - pj.z;
     const float r2 = dot3(rx, ry, rz, rx, ry, rz);
