@@ -11,7 +11,8 @@
 //     spos, svel, sfrc  the reference's sorted copy (sorted_particle_buffer)
 //     srec            per sorted slot, three float4 written by the density
 //                     pass and gathered by XSPH and force:
-//                       (pos, density), (vel, near_density), (normal, 0)
+//                       (pos, density), (vel, near_density),
+//                       (normal, 1 / max(near_density, 1))
 //     nbr, ncnt       in-range neighbour lists of the density pass: entry e
 //                     of slot s at nbr[(e / lanes) * n * lanes + s * lanes
 //                     + e % lanes], i.e. lane e % lanes's share, so a wave's

@@ -339,7 +339,8 @@ __device__ __forceinline__ void density_particle(const DevUniforms *__restrict__
   float4 *rec = b.srec + 3 * (size_t)s;
   rec[0] = make_float4(pi.x, pi.y, pi.z, dens / gamma);
   rec[1] = make_float4(vi.x, vi.y, vi.z, near);
-  rec[2] = make_float4(nx * sc, ny * sc, nz * sc, 0.f);
+  // .w: 1 / max(near density, 1), the force's per-neighbour reciprocal
+  rec[2] = make_float4(nx * sc, ny * sc, nz * sc, __builtin_amdgcn_rcpf(max1(near)));
 }
 
 template <int L, bool B>
@@ -537,11 +538,10 @@ __device__ __forceinline__ void force_particle(const DevUniforms *__restrict__ U
     const float il = __builtin_amdgcn_rsqf(r2);  // 1 / r
     const float r = r2 * il;
     const float dj = max1(pj.w);
-    const float ndj = max1(vj.w);
     const float dr = h - r;
     const float grad = gs * dr * dr;
     // near pressure (:183) and separation (:186-189)
-    float sp = a_sp * (npi * __builtin_amdgcn_rcpf(ndj) + near_k) * grad;
+    float sp = a_sp * (npi * nj.w + near_k) * grad;  // nj.w = 1 / max(rho_near_j, 1)
     if (close) sp -= 200.0f * (1.0f - r * inv_md);
     // Monaghan viscosity (:193-202): -m * pi_visc * grad, only if v.x < 0
     const float vdx = dot3(vi.x - vj.x, vi.y - vj.y, vi.z - vj.z, rx, ry, rz);
