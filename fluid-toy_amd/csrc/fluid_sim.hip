@@ -89,7 +89,7 @@ struct fluid_sim {
   uint32_t graph_n = 0, graph_iters = 0;
   int graph_lanes = 0, graph_solve_lanes = 0, graph_xcd = 0, graph_tiled = 0, graph_wgs = 0;
   uint32_t graph_tile_w = 0;
-  int graph_fuse_xf = 0, graph_solve_list = 0;
+  int graph_fuse_xf = 0, graph_solve_list = 0, graph_wide = 0;
   uint32_t num_cu = 256;
   uint32_t *d_scene_first = nullptr;  // first tile id of each scene (+ total)
   int cursor = -1;  // last completed stage of a partial step, -1 = none
@@ -119,7 +119,7 @@ SweepCfg pick_cfg(uint32_t) {
   // Default before tuning: 8 lanes per particle, the fastest setting the
   // autotuner found at every size measured (8k to 1M particles, compressed
   // and relaxed states); global-memory sweeps.
-  return SweepCfg{8, 8, 0, 0, 1024, 1, 1};
+  return SweepCfg{8, 8, 0, 0, 1024, 1, 1, 0};
 }
 
 // Per-scene tile counts for tiles of w cells along x; uploads the scene
@@ -145,10 +145,12 @@ int setup_tiles(fluid_sim *s, uint32_t w) {
 void update_cfg(fluid_sim *s) {
   if (s->tuned) return;
   const int wgs = s->cfg.tile_wgs, fuse = s->cfg.fuse_xf, sl = s->cfg.solve_list;
+  const int wide = s->cfg.wide_walk;
   s->cfg = pick_cfg(s->n);
   s->cfg.tile_wgs = wgs;
   s->cfg.fuse_xf = fuse;
   s->cfg.solve_list = sl;
+  s->cfg.wide_walk = wide;
 }
 
 int push_uniforms(fluid_sim *s) {
@@ -245,7 +247,8 @@ int ensure_graph(fluid_sim *s) {
       s->graph_lanes == s->cfg.lanes && s->graph_solve_lanes == s->cfg.solve_lanes &&
       s->graph_xcd == s->cfg.xcd_tiles && s->graph_tiled == s->cfg.tiled &&
       s->graph_wgs == s->cfg.tile_wgs && s->graph_tile_w == s->b.tiles.w &&
-      s->graph_fuse_xf == s->cfg.fuse_xf && s->graph_solve_list == s->cfg.solve_list)
+      s->graph_fuse_xf == s->cfg.fuse_xf && s->graph_solve_list == s->cfg.solve_list &&
+      s->graph_wide == s->cfg.wide_walk)
     return FLUID_OK;
   drop_graph(s);
   hipGraph_t g = nullptr;
@@ -273,6 +276,7 @@ int ensure_graph(fluid_sim *s) {
   s->graph_tile_w = s->b.tiles.w;
   s->graph_fuse_xf = s->cfg.fuse_xf;
   s->graph_solve_list = s->cfg.solve_list;
+  s->graph_wide = s->cfg.wide_walk;
   return FLUID_OK;
 }
 
@@ -1205,11 +1209,12 @@ int fluid_set_tile_config(fluid_sim *s, const uint32_t cfg[3]) {
 
 int fluid_set_fusion(fluid_sim *s, uint32_t flags) {
   if (!s) return fail(FLUID_ERR_INVALID_ARG, "null handle");
-  if (flags & ~(FLUID_FUSE_XSPH_FORCE | FLUID_FUSE_SOLVE_LIST))
+  if (flags & ~(FLUID_FUSE_XSPH_FORCE | FLUID_FUSE_SOLVE_LIST | FLUID_WALK_16))
     return fail(FLUID_ERR_INVALID_ARG, "unknown fusion flag");
   if (s->cursor != -1) return fail(FLUID_ERR_STATE, "a partial step is in progress");
   s->cfg.fuse_xf = (flags & FLUID_FUSE_XSPH_FORCE) ? 1 : 0;
   s->cfg.solve_list = (flags & FLUID_FUSE_SOLVE_LIST) ? 1 : 0;
+  s->cfg.wide_walk = (flags & FLUID_WALK_16) ? 1 : 0;
   s->roll.valid = false;
   return FLUID_OK;
 }
@@ -1217,7 +1222,8 @@ int fluid_set_fusion(fluid_sim *s, uint32_t flags) {
 uint32_t fluid_fusion(const fluid_sim *s) {
   if (!s) return 0u;
   return (s->cfg.fuse_xf ? FLUID_FUSE_XSPH_FORCE : 0u) |
-         (s->cfg.solve_list ? FLUID_FUSE_SOLVE_LIST : 0u);
+         (s->cfg.solve_list ? FLUID_FUSE_SOLVE_LIST : 0u) |
+         (s->cfg.wide_walk ? FLUID_WALK_16 : 0u);
 }
 
 int fluid_get_tile_config(const fluid_sim *s, uint32_t out[3]) {
@@ -1288,6 +1294,7 @@ int fluid_autotune(fluid_sim *s, uint32_t trials, uint32_t out[3]) {
     SweepCfg mc = keep;
     mc.tiled = m.tiled;
     mc.xcd_tiles = m.xcd;
+    mc.wide_walk = 0;  // decided with the fusion below
     for (int li = 0; li < 4 && e == hipSuccess && rc == FLUID_OK; ++li) {
       s->cfg = mc;
       s->cfg.lanes = lanes[li];
@@ -1322,26 +1329,33 @@ int fluid_autotune(fluid_sim *s, uint32_t trials, uint32_t out[3]) {
     e = hipStreamSynchronize(s->stream);
     if (e == hipSuccess) rc = setup_tiles(s, best_w);
   }
-  // XSPH + force as one kernel or two: timed as whole graph-launched steps
-  // (the fusion only exists in full-step ranges)
+  // XSPH + force as one kernel or two, and (at 8 lanes) their 16-lane
+  // walk: timed as whole graph-launched steps (the fusion only exists in
+  // full-step ranges)
   {
     float best_f = 1e30f;
-    int fuse = best.fuse_xf;
+    int fuse = best.fuse_xf, wide = 0;
+    const int nwide = best.lanes == 8 ? 2 : 1;
     for (int f = 0; f < 2 && e == hipSuccess && rc == FLUID_OK; ++f) {
-      s->cfg = best;
-      s->cfg.fuse_xf = f;
-      float tot = 0.f, ms = 0.f;
-      for (uint32_t t = 0; t < trials && rc == FLUID_OK; ++t) {
-        if ((e = copy(false)) != hipSuccess) break;
-        rc = fluid_time_steps(s, 0, 2, &ms);
-        tot += ms;
-      }
-      if (tot < best_f) {
-        best_f = tot;
-        fuse = f;
+      for (int wv = 0; wv < nwide && e == hipSuccess && rc == FLUID_OK; ++wv) {
+        s->cfg = best;
+        s->cfg.fuse_xf = f;
+        s->cfg.wide_walk = wv;
+        float tot = 0.f, ms = 0.f;
+        for (uint32_t t = 0; t < trials && rc == FLUID_OK; ++t) {
+          if ((e = copy(false)) != hipSuccess) break;
+          rc = fluid_time_steps(s, 0, 2, &ms);
+          tot += ms;
+        }
+        if (tot < best_f) {
+          best_f = tot;
+          fuse = f;
+          wide = wv;
+        }
       }
     }
     best.fuse_xf = fuse;
+    best.wide_walk = wide;
   }
   if (e == hipSuccess) e = copy(false);
   if (e == hipSuccess)

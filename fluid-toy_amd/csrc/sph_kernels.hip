@@ -397,26 +397,32 @@ __global__ __launch_bounds__(256) void k_density_t(Uni un, Bufs b, uint32_t n) {
   }
 }
 
-// Walk this lane's share of particle s's in-range list (entries sub, sub+L,
-// ... of the group's list), or, if the list overflowed, the lane's candidate
-// subset again with the same test; f(j, p_j) sees every pair once.  Full
-// chunks of U pairs per trip, the next chunk's list entries loaded a trip
-// ahead so only the record gathers wait on memory.
+// Walk this lane's share of particle s's in-range list, or, if the list
+// overflowed, the lane's candidate subset again with the same test; f(j, p_j)
+// sees every pair once.  The list was written for LW = min(L, 8) lanes (the
+// density pass's); with L = 16 two walkers share each list lane, taking its
+// even and odd depths.  Full chunks of U pairs per trip, the next chunk's list
+// entries loaded a trip ahead so only the record gathers wait on memory.
 template <int L, int U, class F>
 __device__ __forceinline__ void for_each_neighbour(const DevUniforms *__restrict__ Un,
                                                    const Bufs &b, uint32_t n, uint32_t s,
                                                    uint32_t sub, float4 pi, F &&f) {
+  constexpr uint32_t LW = L < 8 ? L : 8, M = L / LW;
   const uint32_t total = b.ncnt[s];
   const float4 *__restrict__ srec = b.srec;
   if (total <= kListCapacity) {
-    const uint32_t cnt = total > sub ? (total - 1 - sub) / L + 1 : 0;
-    const uint32_t *__restrict__ nbr = b.nbr + lane_block<L>(s, kListCapacity) + sub;
+    const uint32_t kl = sub % LW, d0 = sub / LW;
+    const uint32_t nd = total > kl ? (total - 1 - kl) / LW + 1 : 0;  // depths of list lane kl
+    const uint32_t cnt = nd > d0 ? (nd - 1 - d0) / M + 1 : 0;
+    const uint32_t *__restrict__ nbr =
+        b.nbr + lane_block<LW>(s, kListCapacity) + d0 * 64 + kl;
+    constexpr uint32_t kStep = 64 * M;
     uint32_t k = 0;
     if (cnt >= U) {
       uint32_t jn[U];
 #pragma unroll
-      for (int q = 0; q < U; ++q) jn[q] = nbr[q * 64];
-      nbr += U * 64;
+      for (int q = 0; q < U; ++q) jn[q] = nbr[q * kStep];
+      nbr += U * kStep;
       k = U;
       while (true) {
         uint32_t jc[U];
@@ -425,8 +431,8 @@ __device__ __forceinline__ void for_each_neighbour(const DevUniforms *__restrict
         const bool more = k + U <= cnt;
         if (more) {
 #pragma unroll
-          for (int q = 0; q < U; ++q) jn[q] = nbr[q * 64];
-          nbr += U * 64;
+          for (int q = 0; q < U; ++q) jn[q] = nbr[q * kStep];
+          nbr += U * kStep;
           k += U;
         }
         float4 p[U];
@@ -439,7 +445,7 @@ __device__ __forceinline__ void for_each_neighbour(const DevUniforms *__restrict
     }
     for (; k < cnt; ++k) {
       const uint32_t j0 = nbr[0];
-      nbr += 64;
+      nbr += kStep;
       f(j0, srec[3 * (size_t)j0]);
     }
     return;
@@ -1159,25 +1165,39 @@ hipError_t launch_density(const Uni &u, const Bufs &b, uint32_t n, SweepCfg c, h
   return hipGetLastError();
 }
 
+// The list walkers (XSPH, force, both): `lanes`, or 16 over the 8-lane list.
+#define FLUID_DISPATCH_WALK(KERNEL, C, N, UN, ...)                                           \
+  do {                                                                                       \
+    if ((C).wide_walk && (C).lanes == 8) {                                                   \
+      const uint32_t _grid = blocks_for((uint64_t)(N) * 16, 256);                            \
+      if ((UN).scene_of)                                                                     \
+        hipLaunchKernelGGL((KERNEL<16, true>), dim3(_grid), dim3(256), 0, st, UN, __VA_ARGS__); \
+      else                                                                                   \
+        hipLaunchKernelGGL((KERNEL<16, false>), dim3(_grid), dim3(256), 0, st, UN, __VA_ARGS__); \
+    } else {                                                                                 \
+      FLUID_DISPATCH_L(KERNEL, (C).lanes, N, UN, __VA_ARGS__);                               \
+    }                                                                                        \
+  } while (0)
+
 hipError_t launch_xsph(const Uni &u, const Bufs &b, uint32_t n, SweepCfg c, hipStream_t st) {
   if (n == 0) return hipSuccess;
-  FLUID_DISPATCH_L(k_xsph, c.lanes, n, u, b, n, (uint32_t)c.xcd_tiles);
+  FLUID_DISPATCH_WALK(k_xsph, c, n, u, b, n, (uint32_t)c.xcd_tiles);
   return hipGetLastError();
 }
 
 hipError_t launch_force(const Uni &u, const Bufs &b, uint32_t n, SweepCfg c, bool fuse_predict,
                         hipStream_t st) {
   if (n == 0) return hipSuccess;
-  FLUID_DISPATCH_L(k_force, c.lanes, n, u, b, n, (uint32_t)c.xcd_tiles,
-                   (uint32_t)(fuse_predict ? 1 : 0));
+  FLUID_DISPATCH_WALK(k_force, c, n, u, b, n, (uint32_t)c.xcd_tiles,
+                      (uint32_t)(fuse_predict ? 1 : 0));
   return hipGetLastError();
 }
 
 hipError_t launch_xsph_force(const Uni &u, const Bufs &b, uint32_t n, SweepCfg c,
                              bool fuse_predict, hipStream_t st) {
   if (n == 0) return hipSuccess;
-  FLUID_DISPATCH_L(k_xsph_force, c.lanes, n, u, b, n, (uint32_t)c.xcd_tiles,
-                   (uint32_t)(fuse_predict ? 1 : 0));
+  FLUID_DISPATCH_WALK(k_xsph_force, c, n, u, b, n, (uint32_t)c.xcd_tiles,
+                      (uint32_t)(fuse_predict ? 1 : 0));
   return hipGetLastError();
 }
 

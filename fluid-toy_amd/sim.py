@@ -246,7 +246,8 @@ class SphSimulation3DGrid:
         self._chk(self._lib.fluid_set_tile_config(self._h, arr))
 
     def set_fusion(self, flags: int) -> None:
-        """Kernel fusion flags (fluid_set_fusion): abi.FLUID_FUSE_XSPH_FORCE."""
+        """Work-sharing flags (fluid_set_fusion): abi.FLUID_FUSE_XSPH_FORCE,
+        abi.FLUID_FUSE_SOLVE_LIST, abi.FLUID_WALK_16."""
         self._chk(self._lib.fluid_set_fusion(self._h, int(flags)))
 
     def fusion(self) -> int:

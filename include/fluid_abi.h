@@ -374,16 +374,21 @@ int fluid_get_sweep_config(const fluid_sim *sim, uint32_t out[3]);
  *   cfg[2] persistent workgroups (0 keeps the current value). */
 int fluid_set_tile_config(fluid_sim *sim, const uint32_t cfg[3]);
 
-/* Work-sharing switches (flags; default both set), the same bits either way:
+/* Work-sharing switches (flags; default the first two set).  These two give
+ * the same bits either way:
  *   FLUID_FUSE_XSPH_FORCE: in full-step ranges, XSPH and force as one kernel
  *     that walks the neighbour list twice with the corrected velocity kept
  *     in registers;
  *   FLUID_FUSE_SOLVE_LIST: the first PCISPH iteration records, per lane in
  *     its walk order, the candidates within h + 2 dt^2 max_accel of the
  *     particle (no later iteration's in-range pair lies outside), and the
- *     later iterations walk that list instead of the 27 cells. */
+ *     later iterations walk that list instead of the 27 cells.
+ * This one regroups the XSPH and force sums (results within rounding):
+ *   FLUID_WALK_16: with 8 lanes per particle, XSPH and force walk the
+ *     in-range lists with 16 lanes per particle, two per list lane. */
 #define FLUID_FUSE_XSPH_FORCE 1u
 #define FLUID_FUSE_SOLVE_LIST 2u
+#define FLUID_WALK_16 4u
 int fluid_set_fusion(fluid_sim *sim, uint32_t flags);
 uint32_t fluid_fusion(const fluid_sim *sim);
 int fluid_get_tile_config(const fluid_sim *sim, uint32_t out[3]);

@@ -102,18 +102,22 @@ def test_tiled_stagewise_matches_graph(pkg):
     b.close()
 
 
-def test_fused_xsph_force_equals_separate(pkg):
+@pytest.mark.parametrize("wide", [0, 4])
+def test_fused_xsph_force_equals_separate(pkg, wide):
     """The fused XSPH + force kernel of the captured step gives the bits of
-    the two separate kernels, with global and with tiled sweeps."""
+    the two separate kernels, with global and with tiled sweeps, and with
+    the 16-lane walk (FLUID_WALK_16 = 4)."""
     scene = pkg.scenes.dam_break_8k()
     for tiled in (0, 1):
         a = pkg.make_sim(scene)
         b = pkg.make_sim(scene)
-        a.set_fusion(0)
-        b.set_fusion(pkg.abi.FLUID_FUSE_XSPH_FORCE)
+        for s in (a, b):
+            s.set_sweep_config(8, 8, 0)
+        a.set_fusion(wide)
+        b.set_fusion(pkg.abi.FLUID_FUSE_XSPH_FORCE | wide)
         for s in (a, b):
             s.set_tile_config(tiled, 4)
-        print(f"case: fused xsph+force tiled={tiled}", flush=True)
+        print(f"case: fused xsph+force tiled={tiled} wide={wide}", flush=True)
         for step in range(4):
             a.step(1)
             b.step(1)
