@@ -186,6 +186,7 @@ PROTOTYPES = {
 FLUID_FUSE_XSPH_FORCE = 1
 FLUID_FUSE_SOLVE_LIST = 2
 FLUID_WALK_16 = 4
+FLUID_WALK_32 = 8
 
 # Gradient slots per scene written by fluid_backward_rollout.
 FLUID_GRAD_PARAMS = 5

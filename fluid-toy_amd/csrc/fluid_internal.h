@@ -172,9 +172,9 @@ struct SweepCfg {
   int tile_wgs;     // persistent workgroups of the tiled sweeps
   int fuse_xf;      // 1: XSPH and force run as one kernel in full-step ranges
   int solve_list;   // 1: later PCISPH iterations walk the first one's skin list
-  int wide_walk;    // 1 (with lanes = 8): XSPH and force walk the 8-lane
-                    // in-range lists with 16 lanes per particle, two per
-                    // list lane (alternate depths)
+  int wide_walk;    // 1 / 2 (with lanes = 8): XSPH and force walk the
+                    // 8-lane in-range lists with 16 / 32 lanes per
+                    // particle, 2 / 4 per list lane (interleaved depths)
 };
 
 // Host-side launchers (sph_kernels.hip).  All enqueue on `st`.

@@ -1209,12 +1209,14 @@ int fluid_set_tile_config(fluid_sim *s, const uint32_t cfg[3]) {
 
 int fluid_set_fusion(fluid_sim *s, uint32_t flags) {
   if (!s) return fail(FLUID_ERR_INVALID_ARG, "null handle");
-  if (flags & ~(FLUID_FUSE_XSPH_FORCE | FLUID_FUSE_SOLVE_LIST | FLUID_WALK_16))
+  if (flags & ~(FLUID_FUSE_XSPH_FORCE | FLUID_FUSE_SOLVE_LIST | FLUID_WALK_16 | FLUID_WALK_32))
     return fail(FLUID_ERR_INVALID_ARG, "unknown fusion flag");
+  if ((flags & FLUID_WALK_16) && (flags & FLUID_WALK_32))
+    return fail(FLUID_ERR_INVALID_ARG, "FLUID_WALK_16 and FLUID_WALK_32 exclude each other");
   if (s->cursor != -1) return fail(FLUID_ERR_STATE, "a partial step is in progress");
   s->cfg.fuse_xf = (flags & FLUID_FUSE_XSPH_FORCE) ? 1 : 0;
   s->cfg.solve_list = (flags & FLUID_FUSE_SOLVE_LIST) ? 1 : 0;
-  s->cfg.wide_walk = (flags & FLUID_WALK_16) ? 1 : 0;
+  s->cfg.wide_walk = (flags & FLUID_WALK_16) ? 1 : (flags & FLUID_WALK_32) ? 2 : 0;
   s->roll.valid = false;
   return FLUID_OK;
 }
@@ -1223,7 +1225,8 @@ uint32_t fluid_fusion(const fluid_sim *s) {
   if (!s) return 0u;
   return (s->cfg.fuse_xf ? FLUID_FUSE_XSPH_FORCE : 0u) |
          (s->cfg.solve_list ? FLUID_FUSE_SOLVE_LIST : 0u) |
-         (s->cfg.wide_walk ? FLUID_WALK_16 : 0u);
+         (s->cfg.wide_walk == 1 ? FLUID_WALK_16 : 0u) |
+         (s->cfg.wide_walk == 2 ? FLUID_WALK_32 : 0u);
 }
 
 int fluid_get_tile_config(const fluid_sim *s, uint32_t out[3]) {
@@ -1329,13 +1332,13 @@ int fluid_autotune(fluid_sim *s, uint32_t trials, uint32_t out[3]) {
     e = hipStreamSynchronize(s->stream);
     if (e == hipSuccess) rc = setup_tiles(s, best_w);
   }
-  // XSPH + force as one kernel or two, and (at 8 lanes) their 16-lane
+  // XSPH + force as one kernel or two, and (at 8 lanes) their 16- or 32-lane
   // walk: timed as whole graph-launched steps (the fusion only exists in
   // full-step ranges)
   {
     float best_f = 1e30f;
     int fuse = best.fuse_xf, wide = 0;
-    const int nwide = best.lanes == 8 ? 2 : 1;
+    const int nwide = best.lanes == 8 ? 3 : 1;
     for (int f = 0; f < 2 && e == hipSuccess && rc == FLUID_OK; ++f) {
       for (int wv = 0; wv < nwide && e == hipSuccess && rc == FLUID_OK; ++wv) {
         s->cfg = best;

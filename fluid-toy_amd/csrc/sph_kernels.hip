@@ -1165,18 +1165,24 @@ hipError_t launch_density(const Uni &u, const Bufs &b, uint32_t n, SweepCfg c, h
   return hipGetLastError();
 }
 
-// The list walkers (XSPH, force, both): `lanes`, or 16 over the 8-lane list.
-#define FLUID_DISPATCH_WALK(KERNEL, C, N, UN, ...)                                           \
-  do {                                                                                       \
-    if ((C).wide_walk && (C).lanes == 8) {                                                   \
-      const uint32_t _grid = blocks_for((uint64_t)(N) * 16, 256);                            \
-      if ((UN).scene_of)                                                                     \
-        hipLaunchKernelGGL((KERNEL<16, true>), dim3(_grid), dim3(256), 0, st, UN, __VA_ARGS__); \
-      else                                                                                   \
-        hipLaunchKernelGGL((KERNEL<16, false>), dim3(_grid), dim3(256), 0, st, UN, __VA_ARGS__); \
-    } else {                                                                                 \
-      FLUID_DISPATCH_L(KERNEL, (C).lanes, N, UN, __VA_ARGS__);                               \
-    }                                                                                        \
+// The list walkers (XSPH, force, both): `lanes`, or 16 / 32 over the
+// 8-lane list.
+#define FLUID_WALK_CASE(KERNEL, LL, N, UN, ...)                                               \
+  do {                                                                                        \
+    const uint32_t _grid = blocks_for((uint64_t)(N) * (LL), 256);                             \
+    if ((UN).scene_of)                                                                        \
+      hipLaunchKernelGGL((KERNEL<LL, true>), dim3(_grid), dim3(256), 0, st, UN, __VA_ARGS__);  \
+    else                                                                                      \
+      hipLaunchKernelGGL((KERNEL<LL, false>), dim3(_grid), dim3(256), 0, st, UN, __VA_ARGS__); \
+  } while (0)
+#define FLUID_DISPATCH_WALK(KERNEL, C, N, UN, ...)                                            \
+  do {                                                                                        \
+    if ((C).wide_walk == 1 && (C).lanes == 8)                                                 \
+      FLUID_WALK_CASE(KERNEL, 16, N, UN, __VA_ARGS__);                                        \
+    else if ((C).wide_walk == 2 && (C).lanes == 8)                                            \
+      FLUID_WALK_CASE(KERNEL, 32, N, UN, __VA_ARGS__);                                        \
+    else                                                                                      \
+      FLUID_DISPATCH_L(KERNEL, (C).lanes, N, UN, __VA_ARGS__);                                \
   } while (0)
 
 hipError_t launch_xsph(const Uni &u, const Bufs &b, uint32_t n, SweepCfg c, hipStream_t st) {

@@ -247,7 +247,7 @@ class SphSimulation3DGrid:
 
     def set_fusion(self, flags: int) -> None:
         """Work-sharing flags (fluid_set_fusion): abi.FLUID_FUSE_XSPH_FORCE,
-        abi.FLUID_FUSE_SOLVE_LIST, abi.FLUID_WALK_16."""
+        abi.FLUID_FUSE_SOLVE_LIST, abi.FLUID_WALK_16 or abi.FLUID_WALK_32."""
         self._chk(self._lib.fluid_set_fusion(self._h, int(flags)))
 
     def fusion(self) -> int:

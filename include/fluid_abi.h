@@ -383,12 +383,14 @@ int fluid_set_tile_config(fluid_sim *sim, const uint32_t cfg[3]);
  *     its walk order, the candidates within h + 2 dt^2 max_accel of the
  *     particle (no later iteration's in-range pair lies outside), and the
  *     later iterations walk that list instead of the 27 cells.
- * This one regroups the XSPH and force sums (results within rounding):
- *   FLUID_WALK_16: with 8 lanes per particle, XSPH and force walk the
- *     in-range lists with 16 lanes per particle, two per list lane. */
+ * These regroup the XSPH and force sums (results within rounding):
+ *   FLUID_WALK_16, FLUID_WALK_32 (at most one): with 8 lanes per particle,
+ *     XSPH and force walk the in-range lists with 16 or 32 lanes per
+ *     particle, two or four per list lane. */
 #define FLUID_FUSE_XSPH_FORCE 1u
 #define FLUID_FUSE_SOLVE_LIST 2u
 #define FLUID_WALK_16 4u
+#define FLUID_WALK_32 8u
 int fluid_set_fusion(fluid_sim *sim, uint32_t flags);
 uint32_t fluid_fusion(const fluid_sim *sim);
 int fluid_get_tile_config(const fluid_sim *sim, uint32_t out[3]);

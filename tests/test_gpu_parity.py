@@ -91,20 +91,20 @@ def compare_grid(sim, o):
     np.testing.assert_array_equal(sim.read_cell_starts(), o.cell_starts())
 
 
-def walk16(pkg, sim):
-    """XSPH and force walk the 8-lane lists with 16 lanes per particle."""
+def wide_walk(pkg, sim, flag):
+    """XSPH and force walk the 8-lane lists with 16 or 32 lanes per particle."""
     sim.set_sweep_config(8, 8, 0)
-    sim.set_fusion(sim.fusion() | pkg.abi.FLUID_WALK_16)
+    sim.set_fusion(sim.fusion() | flag)
 
 
-@pytest.mark.parametrize("wide", [False, True])
+@pytest.mark.parametrize("wide", [0, 4, 8])
 @pytest.mark.parametrize("scene_name", ["dam_break_8k"])
 def test_stagewise_parity(pkg, orc, scene_name, wide):
     scene = pkg.scenes.SCENES[scene_name]()
-    print(f"case: stagewise {scene_name} n={scene.n} walk16={wide}", flush=True)
+    print(f"case: stagewise {scene_name} n={scene.n} walk flag={wide}", flush=True)
     sim, o = make_pair(pkg, orc, scene)
     if wide:
-        walk16(pkg, sim)
+        wide_walk(pkg, sim, wide)
     S = pkg.abi.STAGE
     sim.step_stages(0, S["scan"]); o.stages(0, S["scan"])
     compare_grid(sim, o)
@@ -147,15 +147,16 @@ def test_stagewise_parity(pkg, orc, scene_name, wide):
     sim.close()
 
 
-@pytest.mark.parametrize("wide", [False, True])
+@pytest.mark.parametrize("wide", [0, 4, 8])
 def test_graph_step_teacher_forced(pkg, orc, wide):
-    """Five graph-launched steps, each started from the oracle's state."""
+    """Five graph-launched steps, each started from the oracle's state (the
+    16- and 32-lane list walks: FLUID_WALK_16 = 4, FLUID_WALK_32 = 8)."""
     scene = pkg.scenes.dam_break_8k()
     sim, o = make_pair(pkg, orc, scene)
     if wide:
-        walk16(pkg, sim)
+        wide_walk(pkg, sim, wide)
     for k in range(5):
-        print(f"case: teacher-forced step {k} walk16={wide}", flush=True)
+        print(f"case: teacher-forced step {k} walk flag={wide}", flush=True)
         sim.write_particles(o.particles())
         o.sync64()
         sim.step()

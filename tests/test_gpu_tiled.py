@@ -102,11 +102,11 @@ def test_tiled_stagewise_matches_graph(pkg):
     b.close()
 
 
-@pytest.mark.parametrize("wide", [0, 4])
+@pytest.mark.parametrize("wide", [0, 4, 8])
 def test_fused_xsph_force_equals_separate(pkg, wide):
     """The fused XSPH + force kernel of the captured step gives the bits of
     the two separate kernels, with global and with tiled sweeps, and with
-    the 16-lane walk (FLUID_WALK_16 = 4)."""
+    the 16- and 32-lane walks (FLUID_WALK_16 = 4, FLUID_WALK_32 = 8)."""
     scene = pkg.scenes.dam_break_8k()
     for tiled in (0, 1):
         a = pkg.make_sim(scene)
@@ -148,3 +148,14 @@ def test_solve_skin_list_equals_sweep(pkg, lanes):
             _same(a, b, f"skin lanes={lanes} tiled={tiled} step={step}")
         a.close()
         b.close()
+
+
+def test_wide_walk_flags(pkg):
+    """The two wide walks exclude each other; the flags read back as set."""
+    sim = pkg.make_sim(pkg.scenes.dam_break_8k())
+    with pytest.raises(pkg.abi.FluidError):
+        sim.set_fusion(pkg.abi.FLUID_WALK_16 | pkg.abi.FLUID_WALK_32)
+    for flags in (pkg.abi.FLUID_WALK_16, pkg.abi.FLUID_WALK_32 | pkg.abi.FLUID_FUSE_XSPH_FORCE):
+        sim.set_fusion(flags)
+        assert sim.fusion() == flags
+    sim.close()
