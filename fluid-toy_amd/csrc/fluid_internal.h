@@ -162,7 +162,8 @@ constexpr uint32_t kErrScanSpin = 1u;
 
 // Launch configuration of the neighbour sweeps.
 struct SweepCfg {
-  int lanes;        // lanes per particle in density / XSPH / force
+  int lanes;        // lanes per particle in density / XSPH / force (1, 2, 4,
+                    // 8, or 16: density only, its list written for 8 walkers)
   int solve_lanes;  // lanes per particle in the PCISPH solve
   int xcd_tiles;    // 1: each XCD takes one contiguous run of tiles;
                     // 0: hardware round-robin order
@@ -172,7 +173,7 @@ struct SweepCfg {
   int tile_wgs;     // persistent workgroups of the tiled sweeps
   int fuse_xf;      // 1: XSPH and force run as one kernel in full-step ranges
   int solve_list;   // 1: later PCISPH iterations walk the first one's skin list
-  int wide_walk;    // 1 / 2 (with lanes = 8): XSPH and force walk the
+  int wide_walk;    // 1 / 2 (with lanes >= 8): XSPH and force walk the
                     // 8-lane in-range lists with 16 / 32 lanes per
                     // particle, 2 / 4 per list lane (interleaved depths)
 };

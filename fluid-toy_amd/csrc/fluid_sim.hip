@@ -177,7 +177,7 @@ int push_uniforms(fluid_sim *s) {
 // writes the predicted state) and finalize + integrate.
 int enqueue_range(fluid_sim *s, uint32_t first, uint32_t last) {
   s->roll.valid = false;
-  s->b.tiles.chunk = 256u / (uint32_t)s->cfg.lanes;
+  s->b.tiles.chunk = 256u / (uint32_t)std::min(s->cfg.lanes, 8);
   const uint32_t n = s->n;
   hipStream_t st = s->stream;
   const Bufs &b = s->b;
@@ -509,7 +509,7 @@ int enqueue_tape_step(fluid_sim *s, const TapeSlot &T, float4 *scratch_v) {
   const Uni u = s->uni();
   hipStream_t st = s->stream;
   Bufs rb = s->b;
-  rb.tiles.chunk = 256u / (uint32_t)c.lanes;
+  rb.tiles.chunk = 256u / (uint32_t)std::min(c.lanes, 8);
   rb.spos = T.x;
   rb.svel = T.u;
   rb.srec = T.rec;
@@ -1177,8 +1177,9 @@ static bool valid_lanes(uint32_t l) { return l == 1 || l == 2 || l == 4 || l == 
 
 int fluid_set_sweep_config(fluid_sim *s, const uint32_t cfg[3]) {
   if (!s || !cfg) return fail(FLUID_ERR_INVALID_ARG, "null argument");
-  if ((cfg[0] && !valid_lanes(cfg[0])) || (cfg[1] && !valid_lanes(cfg[1])))
-    return fail(FLUID_ERR_INVALID_ARG, "lanes must be 1, 2, 4 or 8");
+  if ((cfg[0] && !valid_lanes(cfg[0]) && cfg[0] != 16) || (cfg[1] && !valid_lanes(cfg[1])))
+    return fail(FLUID_ERR_INVALID_ARG,
+                "lanes must be 1, 2, 4, 8 or 16 (density / XSPH / force), 1, 2, 4 or 8 (solve)");
   if (cfg[2] > 1) return fail(FLUID_ERR_INVALID_ARG, "xcd mapping must be 0 or 1");
   if (s->cursor != -1) return fail(FLUID_ERR_STATE, "a partial step is in progress");
   if (cfg[0]) s->cfg.lanes = (int)cfg[0];
@@ -1276,7 +1277,7 @@ int fluid_autotune(fluid_sim *s, uint32_t trials, uint32_t out[3]) {
   HIP_TRY(hipStreamSynchronize(s->stream));
   const SweepCfg keep = s->cfg;
   const uint32_t keep_w = s->b.tiles.w;
-  const int lanes[4] = {1, 2, 4, 8};
+  const int lanes[5] = {1, 2, 4, 8, 16};
   // Sweep modes: global gathers with either workgroup-to-XCD mapping, and
   // LDS-staged tiles of 2, 4 or 8 cells.  Within a mode, lanes per particle
   // for density/XSPH/force and for the solve are timed independently.
@@ -1298,10 +1299,11 @@ int fluid_autotune(fluid_sim *s, uint32_t trials, uint32_t out[3]) {
     mc.tiled = m.tiled;
     mc.xcd_tiles = m.xcd;
     mc.wide_walk = 0;  // decided with the fusion below
-    for (int li = 0; li < 4 && e == hipSuccess && rc == FLUID_OK; ++li) {
+    for (int li = 0; li < 5 && e == hipSuccess && rc == FLUID_OK; ++li) {
+      if (m.tiled && lanes[li] > 8) continue;  // tiles run at most 8 lanes
       s->cfg = mc;
       s->cfg.lanes = lanes[li];
-      s->cfg.solve_lanes = lanes[li];
+      s->cfg.solve_lanes = std::min(lanes[li], 8);
       float ms[FLUID_STAGE_COUNT];
       float tl = 0.f, ts = 0.f;
       for (uint32_t t = 0; t < trials && rc == FLUID_OK; ++t) {
@@ -1319,7 +1321,7 @@ int fluid_autotune(fluid_sim *s, uint32_t trials, uint32_t out[3]) {
         }
       } else {
         if (tl < ml) { ml = tl; mc.lanes = lanes[li]; }
-        if (ts < ms_solve) { ms_solve = ts; mc.solve_lanes = lanes[li]; }
+        if (lanes[li] <= 8 && ts < ms_solve) { ms_solve = ts; mc.solve_lanes = lanes[li]; }
       }
     }
     if (ml + ms_solve < best_total) {
@@ -1338,7 +1340,7 @@ int fluid_autotune(fluid_sim *s, uint32_t trials, uint32_t out[3]) {
   {
     float best_f = 1e30f;
     int fuse = best.fuse_xf, wide = 0;
-    const int nwide = best.lanes == 8 ? 3 : 1;
+    const int nwide = best.lanes >= 8 ? 3 : 1;
     for (int f = 0; f < 2 && e == hipSuccess && rc == FLUID_OK; ++f) {
       for (int wv = 0; wv < nwide && e == hipSuccess && rc == FLUID_OK; ++wv) {
         s->cfg = best;

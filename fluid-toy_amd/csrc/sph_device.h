@@ -57,25 +57,31 @@ __device__ __forceinline__ size_t lane_block(uint32_t s, uint32_t cap) {
 }
 
 // Appends j to a lane group's in-range list when `pair`.  Every lane of the
-// group calls it at the same point of a group-synchronous walk; the group's
-// k-th entry (ranked across its lanes by a ballot, lanes in order) goes to
-// lane k % L at depth k / L, i.e. blk[(k / L) * 64 + k % L] with
-// blk = list + lane_block<L>(slot, kListCapacity), so each lane of a later
-// walk takes an equal share.  run counts the group's entries (above
-// kListCapacity the list is incomplete and the count marks it).  The group's
-// ballot bits are byte lane / 8 of the wave's ballot (one byte permute),
-// shifted down for L < 8.
+// group (L lanes) calls it at the same point of a group-synchronous walk; the
+// group's k-th entry (ranked across its lanes by a ballot, lanes in order)
+// goes to list lane k % LW at depth k / LW, LW = min(L, 8), i.e.
+// blk[(k / LW) * 64 + k % LW] with blk = list + lane_block<LW>(slot,
+// kListCapacity), so each lane of a later walk takes an equal share.  run
+// counts the group's entries (above kListCapacity the list is incomplete and
+// the count marks it).  For L <= 8 the group's ballot bits are byte lane / 8
+// of the wave's ballot (one byte permute), shifted down for L < 8.
 template <int L>
 __device__ __forceinline__ void group_list_append(uint32_t *__restrict__ blk, uint32_t sub,
                                                   uint32_t &run, bool pair, uint32_t j) {
+  constexpr uint32_t LW = L < 8 ? L : 8;
   const uint32_t lane = threadIdx.x & 63u;
   const uint64_t bal = __ballot(pair);
-  const uint32_t byte = __builtin_amdgcn_perm((uint32_t)(bal >> 32), (uint32_t)bal,
-                                              0x0c0c0c00u | (lane >> 3));
-  const uint32_t gm = L == 8 ? byte : (byte >> (lane & 7u & ~(uint32_t)(L - 1))) & ((1u << L) - 1u);
+  uint32_t gm;
+  if constexpr (L > 8) {
+    gm = (uint32_t)(bal >> (lane & ~(uint32_t)(L - 1))) & (uint32_t)((1ull << L) - 1ull);
+  } else {
+    const uint32_t byte = __builtin_amdgcn_perm((uint32_t)(bal >> 32), (uint32_t)bal,
+                                                0x0c0c0c00u | (lane >> 3));
+    gm = L == 8 ? byte : (byte >> (lane & 7u & ~(uint32_t)(L - 1))) & ((1u << L) - 1u);
+  }
   if (pair) {
     const uint32_t e = run + (uint32_t)__popc(gm & ((1u << sub) - 1u));
-    if (e < kListCapacity) blk[(e / L) * 64u + (e % L)] = j;
+    if (e < kListCapacity) blk[(e / LW) * 64u + (e % LW)] = j;
   }
   run += (uint32_t)__popc(gm);
 }

@@ -294,7 +294,8 @@ __device__ __forceinline__ void density_particle(const DevUniforms *__restrict__
   const float h = U->h, h2 = U->h2;
   const float mps = U->mass * U->poly6_scale, mns = U->mass * U->near_scale;
   const float mr = U->m_over_rho0;
-  uint32_t *__restrict__ nbr = b.nbr + lane_block<L>(s, kListCapacity);
+  // the list is written for min(L, 8) walker lanes
+  uint32_t *__restrict__ nbr = b.nbr + lane_block<(L < 8 ? L : 8)>(s, kListCapacity);
   uint32_t run = 0;  // pairs recorded so far by the whole group
   float dens = 0.f, near = 0.f, nx = 0.f, ny = 0.f, nz = 0.f;
   // Branch-free body: a pair outside the support contributes exact zeros
@@ -1156,15 +1157,6 @@ hipError_t launch_reorder(const Uni &, const Bufs &b, uint32_t n, hipStream_t st
     }                                                                                     \
   } while (0)
 
-hipError_t launch_density(const Uni &u, const Bufs &b, uint32_t n, SweepCfg c, hipStream_t st) {
-  if (n == 0) return hipSuccess;
-  if (c.tiled)
-    FLUID_DISPATCH_TILED(k_density_t, c.lanes, u, b, n);
-  else
-    FLUID_DISPATCH_L(k_density, c.lanes, n, u, b, n, (uint32_t)c.xcd_tiles);
-  return hipGetLastError();
-}
-
 // The list walkers (XSPH, force, both): `lanes`, or 16 / 32 over the
 // 8-lane list.
 #define FLUID_WALK_CASE(KERNEL, LL, N, UN, ...)                                               \
@@ -1177,13 +1169,24 @@ hipError_t launch_density(const Uni &u, const Bufs &b, uint32_t n, SweepCfg c, h
   } while (0)
 #define FLUID_DISPATCH_WALK(KERNEL, C, N, UN, ...)                                            \
   do {                                                                                        \
-    if ((C).wide_walk == 1 && (C).lanes == 8)                                                 \
+    if ((C).wide_walk == 1 && (C).lanes >= 8)                                                 \
       FLUID_WALK_CASE(KERNEL, 16, N, UN, __VA_ARGS__);                                        \
-    else if ((C).wide_walk == 2 && (C).lanes == 8)                                            \
+    else if ((C).wide_walk == 2 && (C).lanes >= 8)                                            \
       FLUID_WALK_CASE(KERNEL, 32, N, UN, __VA_ARGS__);                                        \
     else                                                                                      \
-      FLUID_DISPATCH_L(KERNEL, (C).lanes, N, UN, __VA_ARGS__);                                \
+      FLUID_DISPATCH_L(KERNEL, (C).lanes < 8 ? (C).lanes : 8, N, UN, __VA_ARGS__);            \
   } while (0)
+
+hipError_t launch_density(const Uni &u, const Bufs &b, uint32_t n, SweepCfg c, hipStream_t st) {
+  if (n == 0) return hipSuccess;
+  if (c.tiled)
+    FLUID_DISPATCH_TILED(k_density_t, c.lanes, u, b, n);
+  else if (c.lanes == 16)
+    FLUID_WALK_CASE(k_density, 16, n, u, b, n, (uint32_t)c.xcd_tiles);
+  else
+    FLUID_DISPATCH_L(k_density, c.lanes, n, u, b, n, (uint32_t)c.xcd_tiles);
+  return hipGetLastError();
+}
 
 hipError_t launch_xsph(const Uni &u, const Bufs &b, uint32_t n, SweepCfg c, hipStream_t st) {
   if (n == 0) return hipSuccess;

@@ -357,8 +357,11 @@ int fluid_time_stages(fluid_sim *sim, uint32_t steps, float *ms_per_stage);
 
 /* Launch configuration of the neighbour sweeps, cfg[3]:
  *   cfg[0] lanes cooperating on one particle in density/XSPH/force (they
- *          share the neighbour-list layout), cfg[1] the same for the PCISPH
- *          solve: 1, 2, 4 or 8 (0 keeps the current value);
+ *          share the neighbour-list layout): 1, 2, 4, 8, or 16 (density
+ *          sweeps with 16 lanes and writes the 8-lane list; XSPH and force
+ *          then walk it with 8, 16 or 32 lanes: FLUID_WALK_16 / _32);
+ *          cfg[1] the same for the PCISPH solve: 1, 2, 4 or 8 (0 keeps the
+ *          current value);
  *   cfg[2] workgroup-to-XCD mapping: 1 gives each of the 8 XCDs one
  *          contiguous run of tiles (shared L2 between neighbours), 0 keeps
  *          the hardware's round-robin order (spreads dense regions). */
@@ -384,7 +387,7 @@ int fluid_set_tile_config(fluid_sim *sim, const uint32_t cfg[3]);
  *     particle (no later iteration's in-range pair lies outside), and the
  *     later iterations walk that list instead of the 27 cells.
  * These regroup the XSPH and force sums (results within rounding):
- *   FLUID_WALK_16, FLUID_WALK_32 (at most one): with 8 lanes per particle,
+ *   FLUID_WALK_16, FLUID_WALK_32 (at most one): with 8 or 16 lanes,
  *     XSPH and force walk the in-range lists with 16 or 32 lanes per
  *     particle, two or four per list lane. */
 #define FLUID_FUSE_XSPH_FORCE 1u

@@ -159,3 +159,35 @@ def test_wide_walk_flags(pkg):
         sim.set_fusion(flags)
         assert sim.fusion() == flags
     sim.close()
+
+
+@pytest.mark.parametrize("wide", [0, 4, 8])
+def test_density_16_lanes(pkg, wide):
+    """Density with 16 lanes per particle writes the same 8-lane in-range
+    lists (the same pairs in the same order) as with 8: from the same state,
+    one step differs only through the density sums' grouping.  Density and
+    near density agree to float rounding (1e-5); velocity and position to
+    the PCISPH solve's own rounding noise (2e-3, per particle with a floor
+    of 1% of the field RMS, as the oracle parity tests)."""
+    from conftest import rowwise_rel_err
+    scene = pkg.scenes.dam_break_8k()
+    a = pkg.make_sim(scene)
+    b = pkg.make_sim(scene)
+    a.set_sweep_config(8, 8, 0)
+    b.set_sweep_config(16, 8, 0)
+    for s in (a, b):
+        s.set_fusion(s.fusion() | wide)
+    print(f"case: density 16 lanes, walk flag={wide}", flush=True)
+    for step in range(3):
+        a.write_particles(b.read_particles())
+        a.step(1)
+        b.step(1)
+        pa, pb = a.read_sorted_particles(), b.read_sorted_particles()
+        assert np.array_equal(a.read_sorted_to_orig(), b.read_sorted_to_orig()), step
+        for cols, name, tol in ((slice(11, 12), "density", 1e-5), (slice(12, 13), "near", 1e-5),
+                                (slice(4, 7), "velocity", 2e-3), (slice(0, 3), "position", 2e-3)):
+            err = rowwise_rel_err(pb[:, cols], pa[:, cols])
+            print(f"  step {step} {name}: max {err.max():.2e}", flush=True)
+            assert err.max() < tol, (step, name, err.max())
+    a.close()
+    b.close()
