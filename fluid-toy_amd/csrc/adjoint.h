@@ -26,11 +26,10 @@ struct AdjBufs {
                                 // predicted-density adjoint), ping-pong
   float *lP;                    // pressure of the iteration being reversed
   double *acc;                  // parameter gradients per slot (kAccStride)
-  // lists built within one reversed step: the solve skin list of the first
-  // reversed PCISPH iteration (per lane: entry e of lane k of particle s at
-  // [e * n * L + s * L + k], scnt per lane), and the in-range pairs of the
-  // force adjoint (ballot-balanced like the density pass's list: the
-  // group's entry e at [(e / L) * n * L + s * L + e % L], pcnt per particle)
+  // lists built within one reversed step, in the wave blocks of
+  // lane_block<L>: the solve skin list of the first reversed PCISPH
+  // iteration (per lane, scnt per lane), and the in-range pairs of the force
+  // adjoint (ballot-balanced like the density pass's list, pcnt per particle)
   uint32_t *slist, *scnt;       // kSolveListCapacity per particle
   uint32_t *plist, *pcnt;       // kListCapacity per particle
 };

@@ -154,6 +154,8 @@ constexpr uint32_t kListCapacity = 1024;
 // Solve candidates kept per particle for the later iterations (split over
 // its lanes; a lane that overflows sweeps its cells instead).
 constexpr uint32_t kSolveListCapacity = 512;
+// Most lanes per particle any sweep runs with (per-lane counts are sized by it).
+constexpr uint32_t kMaxLanes = 16;
 constexpr uint32_t kSolveListOverflow = 0xFFFFFFFFu;
 constexpr uint32_t kListOverflow = 0xFFFFFFFFu;
 
@@ -164,7 +166,7 @@ constexpr uint32_t kErrScanSpin = 1u;
 struct SweepCfg {
   int lanes;        // lanes per particle in density / XSPH / force (1, 2, 4,
                     // 8, or 16: density only, its list written for 8 walkers)
-  int solve_lanes;  // lanes per particle in the PCISPH solve
+  int solve_lanes;  // lanes per particle in the PCISPH solve (1 .. 16)
   int xcd_tiles;    // 1: each XCD takes one contiguous run of tiles;
                     // 0: hardware round-robin order
   int tiled;        // 1: density and solve stage cell tiles in LDS, both with

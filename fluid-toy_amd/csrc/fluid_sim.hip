@@ -652,7 +652,7 @@ static int create_handle(int device, std::vector<SceneState> scenes,
   ALLOC(nbr, uint32_t *, (size_t)kListCapacity * (N + 64));
   ALLOC(ncnt, uint32_t *, N);
   ALLOC(slist, uint32_t *, (size_t)kSolveListCapacity * (N + 64));
-  ALLOC(scnt, uint32_t *, 8 * N);
+  ALLOC(scnt, uint32_t *, (size_t)kMaxLanes * N);  // one count per solve lane
   ALLOC(sid, uint32_t *, N);
   ALLOC(counts, uint32_t *, C);
   ALLOC(starts, uint32_t *, C);
@@ -1173,13 +1173,12 @@ int fluid_time_stages(fluid_sim *s, uint32_t steps, float *ms_per_stage) {
   return check_errors(s);
 }
 
-static bool valid_lanes(uint32_t l) { return l == 1 || l == 2 || l == 4 || l == 8; }
+static bool valid_lanes(uint32_t l) { return l == 1 || l == 2 || l == 4 || l == 8 || l == 16; }
 
 int fluid_set_sweep_config(fluid_sim *s, const uint32_t cfg[3]) {
   if (!s || !cfg) return fail(FLUID_ERR_INVALID_ARG, "null argument");
-  if ((cfg[0] && !valid_lanes(cfg[0]) && cfg[0] != 16) || (cfg[1] && !valid_lanes(cfg[1])))
-    return fail(FLUID_ERR_INVALID_ARG,
-                "lanes must be 1, 2, 4, 8 or 16 (density / XSPH / force), 1, 2, 4 or 8 (solve)");
+  if ((cfg[0] && !valid_lanes(cfg[0])) || (cfg[1] && !valid_lanes(cfg[1])))
+    return fail(FLUID_ERR_INVALID_ARG, "lanes must be 1, 2, 4, 8 or 16");
   if (cfg[2] > 1) return fail(FLUID_ERR_INVALID_ARG, "xcd mapping must be 0 or 1");
   if (s->cursor != -1) return fail(FLUID_ERR_STATE, "a partial step is in progress");
   if (cfg[0]) s->cfg.lanes = (int)cfg[0];
@@ -1300,10 +1299,10 @@ int fluid_autotune(fluid_sim *s, uint32_t trials, uint32_t out[3]) {
     mc.xcd_tiles = m.xcd;
     mc.wide_walk = 0;  // decided with the fusion below
     for (int li = 0; li < 5 && e == hipSuccess && rc == FLUID_OK; ++li) {
-      if (m.tiled && lanes[li] > 8) continue;  // tiles run at most 8 lanes
+      if (m.tiled && lanes[li] > 8) continue;  // tiled sweeps run at most 8 lanes
       s->cfg = mc;
       s->cfg.lanes = lanes[li];
-      s->cfg.solve_lanes = std::min(lanes[li], 8);
+      s->cfg.solve_lanes = lanes[li];
       float ms[FLUID_STAGE_COUNT];
       float tl = 0.f, ts = 0.f;
       for (uint32_t t = 0; t < trials && rc == FLUID_OK; ++t) {
@@ -1321,7 +1320,7 @@ int fluid_autotune(fluid_sim *s, uint32_t trials, uint32_t out[3]) {
         }
       } else {
         if (tl < ml) { ml = tl; mc.lanes = lanes[li]; }
-        if (lanes[li] <= 8 && ts < ms_solve) { ms_solve = ts; mc.solve_lanes = lanes[li]; }
+        if (ts < ms_solve) { ms_solve = ts; mc.solve_lanes = lanes[li]; }
       }
     }
     if (ml + ms_solve < best_total) {

@@ -1220,9 +1220,14 @@ hipError_t launch_solve(const Uni &u, const Bufs &b, uint32_t n, const PredBuf &
                         const PredBuf &out, float4 *ap_out, SweepCfg c, bool first_iter,
                         hipStream_t st) {
   if (n == 0) return hipSuccess;
+  // per-lane skin-list counts are sized for kMaxLanes lanes per particle
+  if (c.solve_lanes > (int)kMaxLanes || c.lanes > (int)kMaxLanes) return hipErrorInvalidValue;
   if (c.tiled)
     FLUID_DISPATCH_TILED(k_solve_t, c.lanes, u, b, n, in, out, ap_out,
                          (uint32_t)(first_iter ? 1 : 0), (uint32_t)c.solve_list);
+  else if (c.solve_lanes == 16)
+    FLUID_WALK_CASE(k_solve, 16, n, u, b, n, in, out, ap_out, (uint32_t)c.xcd_tiles,
+                    (uint32_t)(first_iter ? 1 : 0), (uint32_t)c.solve_list);
   else
     FLUID_DISPATCH_L(k_solve, c.solve_lanes, n, u, b, n, in, out, ap_out,
                      (uint32_t)c.xcd_tiles, (uint32_t)(first_iter ? 1 : 0),

@@ -360,8 +360,8 @@ int fluid_time_stages(fluid_sim *sim, uint32_t steps, float *ms_per_stage);
  *          share the neighbour-list layout): 1, 2, 4, 8, or 16 (density
  *          sweeps with 16 lanes and writes the 8-lane list; XSPH and force
  *          then walk it with 8, 16 or 32 lanes: FLUID_WALK_16 / _32);
- *          cfg[1] the same for the PCISPH solve: 1, 2, 4 or 8 (0 keeps the
- *          current value);
+ *          cfg[1] the same for the PCISPH solve: 1, 2, 4, 8 or 16 (0 keeps
+ *          the current value; the LDS-tiled sweeps run at most 8);
  *   cfg[2] workgroup-to-XCD mapping: 1 gives each of the 8 XCDs one
  *          contiguous run of tiles (shared L2 between neighbours), 0 keeps
  *          the hardware's round-robin order (spreads dense regions). */

@@ -126,7 +126,7 @@ def test_fused_xsph_force_equals_separate(pkg, wide):
         b.close()
 
 
-@pytest.mark.parametrize("lanes", [1, 4, 8])
+@pytest.mark.parametrize("lanes", [1, 4, 8, 16])
 def test_solve_skin_list_equals_sweep(pkg, lanes):
     """Later PCISPH iterations over the first iteration's skin list give the
     bits of the 27-cell sweep (plain and tiled), on the compressed lattice and
