@@ -353,3 +353,30 @@ def test_block_1m_full_size(pkg, orc):
     report("1m velocity", err)
     assert np.quantile(err, 0.9999) <= 1e-3
     sim.close()
+
+
+def test_block_4m_full_size(pkg, orc):
+    """Full-size 4,194,304-particle block (the second throughput size):
+    exact grid invariants, and one graph step against the float32 oracle."""
+    scene = pkg.scenes.block_4m()
+    print(f"case: block_4m n={scene.n}", flush=True)
+    params, cont, g = pkg.scenes.make_params(scene, pkg.abi, pkg.derive_sph_params,
+                                             pkg.derive_container_geometry)
+    sim = pkg.make_sim(scene)
+    sim.step()
+    counts = sim.read_cell_counts().astype(np.int64)
+    starts = sim.read_cell_starts().astype(np.int64)
+    assert counts.sum() == scene.n
+    np.testing.assert_array_equal(starts, np.cumsum(counts))
+    s2o = sim.read_sorted_to_orig()
+    assert np.array_equal(np.sort(s2o), np.arange(scene.n))
+    o = orc.from_scene(scene, pkg.abi, params, cont, g)
+    o.step()
+    np.testing.assert_array_equal(sim.read_cell_counts(), o.cell_counts())
+    np.testing.assert_array_equal(s2o, o.sorted_to_orig())
+    gp, op = sim.read_particles(), o.particles()
+    check_fp("4m position", gp[:, 0:3], op[:, 0:3])
+    err = rowwise_rel_err(gp[:, 4:7], op[:, 4:7])
+    report("4m velocity", err)
+    assert np.quantile(err, 0.9999) <= 1e-3
+    sim.close()
