@@ -542,7 +542,7 @@ __global__ __launch_bounds__(256) void k_adj_force(Uni un, AdjBufs a, TapeStep t
                    [&](uint32_t j, float4 xj, bool valid) {
     const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
     const float r2 = dot3(rx, ry, rz, rx, ry, rz);
-    group_list_append<L>(a.plist + lane_block<L>(s, kListCapacity), sub, run,
+    group_list_append<L, L>(a.plist + lane_block<L>(s, kListCapacity), sub, run,
                          valid && r2 < h2 && r2 > 1e-12f, j);
   });
   if (sub == 0) a.pcnt[s] = run;

@@ -156,6 +156,9 @@ constexpr uint32_t kListCapacity = 1024;
 constexpr uint32_t kSolveListCapacity = 512;
 // Most lanes per particle any sweep runs with (per-lane counts are sized by it).
 constexpr uint32_t kMaxLanes = 16;
+// Walker lanes the density pass writes the in-range lists for (XSPH and force
+// walk them with 8, 16 or 32 lanes per particle).
+constexpr int kListLanes = 8;
 constexpr uint32_t kSolveListOverflow = 0xFFFFFFFFu;
 constexpr uint32_t kListOverflow = 0xFFFFFFFFu;
 
@@ -164,8 +167,8 @@ constexpr uint32_t kErrScanSpin = 1u;
 
 // Launch configuration of the neighbour sweeps.
 struct SweepCfg {
-  int lanes;        // lanes per particle in density / XSPH / force (1, 2, 4,
-                    // 8, or 16: density only, its list written for 8 walkers)
+  int lanes;        // lanes per particle in the density sweep (1, 2, 4, 8 or
+                    // 16; its list is written for 8 walkers either way)
   int solve_lanes;  // lanes per particle in the PCISPH solve (1 .. 16)
   int xcd_tiles;    // 1: each XCD takes one contiguous run of tiles;
                     // 0: hardware round-robin order
@@ -175,9 +178,9 @@ struct SweepCfg {
   int tile_wgs;     // persistent workgroups of the tiled sweeps
   int fuse_xf;      // 1: XSPH and force run as one kernel in full-step ranges
   int solve_list;   // 1: later PCISPH iterations walk the first one's skin list
-  int wide_walk;    // 1 / 2 (with lanes >= 8): XSPH and force walk the
-                    // 8-lane in-range lists with 16 / 32 lanes per
-                    // particle, 2 / 4 per list lane (interleaved depths)
+  int wide_walk;    // 0 / 1 / 2: XSPH and force walk the 8-lane in-range
+                    // lists with 8 / 16 / 32 lanes per particle (1, 2 or 4
+                    // per list lane, interleaved depths)
 };
 
 // Host-side launchers (sph_kernels.hip).  All enqueue on `st`.

@@ -1333,13 +1333,13 @@ int fluid_autotune(fluid_sim *s, uint32_t trials, uint32_t out[3]) {
     e = hipStreamSynchronize(s->stream);
     if (e == hipSuccess) rc = setup_tiles(s, best_w);
   }
-  // XSPH + force as one kernel or two, and (at 8 lanes) their 16- or 32-lane
+  // XSPH + force as one kernel or two, and their 8-, 16- or 32-lane list
   // walk: timed as whole graph-launched steps (the fusion only exists in
   // full-step ranges)
   {
     float best_f = 1e30f;
     int fuse = best.fuse_xf, wide = 0;
-    const int nwide = best.lanes >= 8 ? 3 : 1;
+    const int nwide = 3;
     for (int f = 0; f < 2 && e == hipSuccess && rc == FLUID_OK; ++f) {
       for (int wv = 0; wv < nwide && e == hipSuccess && rc == FLUID_OK; ++wv) {
         s->cfg = best;

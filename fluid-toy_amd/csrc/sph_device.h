@@ -59,16 +59,15 @@ __device__ __forceinline__ size_t lane_block(uint32_t s, uint32_t cap) {
 // Appends j to a lane group's in-range list when `pair`.  Every lane of the
 // group (L lanes) calls it at the same point of a group-synchronous walk; the
 // group's k-th entry (ranked across its lanes by a ballot, lanes in order)
-// goes to list lane k % LW at depth k / LW, LW = min(L, 8), i.e.
-// blk[(k / LW) * 64 + k % LW] with blk = list + lane_block<LW>(slot,
-// kListCapacity), so each lane of a later walk takes an equal share.  run
-// counts the group's entries (above kListCapacity the list is incomplete and
-// the count marks it).  For L <= 8 the group's ballot bits are byte lane / 8
-// of the wave's ballot (one byte permute), shifted down for L < 8.
-template <int L>
+// goes to list lane k % LW at depth k / LW, i.e. blk[(k / LW) * 64 + k % LW]
+// with blk = list + lane_block<LW>(slot, kListCapacity), so each lane of a
+// later walk takes an equal share.  run counts the group's entries (above
+// kListCapacity the list is incomplete and the count marks it).  For L <= 8
+// the group's ballot bits are byte lane / 8 of the wave's ballot (one byte
+// permute), shifted down for L < 8.
+template <int L, int LW>
 __device__ __forceinline__ void group_list_append(uint32_t *__restrict__ blk, uint32_t sub,
                                                   uint32_t &run, bool pair, uint32_t j) {
-  constexpr uint32_t LW = L < 8 ? L : 8;
   const uint32_t lane = threadIdx.x & 63u;
   const uint64_t bal = __ballot(pair);
   uint32_t gm;

@@ -294,8 +294,8 @@ __device__ __forceinline__ void density_particle(const DevUniforms *__restrict__
   const float h = U->h, h2 = U->h2;
   const float mps = U->mass * U->poly6_scale, mns = U->mass * U->near_scale;
   const float mr = U->m_over_rho0;
-  // the list is written for min(L, 8) walker lanes
-  uint32_t *__restrict__ nbr = b.nbr + lane_block<(L < 8 ? L : 8)>(s, kListCapacity);
+  // the list is written for 8 walker lanes, whatever L
+  uint32_t *__restrict__ nbr = b.nbr + lane_block<kListLanes>(s, kListCapacity);
   uint32_t run = 0;  // pairs recorded so far by the whole group
   float dens = 0.f, near = 0.f, nx = 0.f, ny = 0.f, nz = 0.f;
   // Branch-free body: a pair outside the support contributes exact zeros
@@ -314,7 +314,7 @@ __device__ __forceinline__ void density_particle(const DevUniforms *__restrict__
     nx += fn * rx;
     ny += fn * ry;
     nz += fn * rz;
-    group_list_append<L>(nbr, sub, run, pair, j);
+    group_list_append<L, kListLanes>(nbr, sub, run, pair, j);
   });
   if (sub == 0) b.ncnt[s] = run;
   dens = group_sum<L>(dens);
@@ -400,15 +400,16 @@ __global__ __launch_bounds__(256) void k_density_t(Uni un, Bufs b, uint32_t n) {
 
 // Walk this lane's share of particle s's in-range list, or, if the list
 // overflowed, the lane's candidate subset again with the same test; f(j, p_j)
-// sees every pair once.  The list was written for LW = min(L, 8) lanes (the
-// density pass's); with L = 16 two walkers share each list lane, taking its
-// even and odd depths.  Full chunks of U pairs per trip, the next chunk's list
-// entries loaded a trip ahead so only the record gathers wait on memory.
+// sees every pair once.  The list is written for LW = 8 lanes; with L = 16 or
+// 32 walkers, M = L / 8 of them share each list lane, taking interleaved
+// depths.  Full chunks of U pairs per trip, the next chunk's list entries
+// loaded a trip ahead so only the record gathers wait on memory.
 template <int L, int U, class F>
 __device__ __forceinline__ void for_each_neighbour(const DevUniforms *__restrict__ Un,
                                                    const Bufs &b, uint32_t n, uint32_t s,
                                                    uint32_t sub, float4 pi, F &&f) {
-  constexpr uint32_t LW = L < 8 ? L : 8, M = L / LW;
+  static_assert(L % kListLanes == 0, "walkers are a multiple of the list lanes");
+  constexpr uint32_t LW = kListLanes, M = L / LW;
   const uint32_t total = b.ncnt[s];
   const float4 *__restrict__ srec = b.srec;
   if (total <= kListCapacity) {
@@ -1157,8 +1158,8 @@ hipError_t launch_reorder(const Uni &, const Bufs &b, uint32_t n, hipStream_t st
     }                                                                                     \
   } while (0)
 
-// The list walkers (XSPH, force, both): `lanes`, or 16 / 32 over the
-// 8-lane list.
+// The list walkers (XSPH, force, both): 8, 16 or 32 lanes per particle over
+// the 8-lane list.
 #define FLUID_WALK_CASE(KERNEL, LL, N, UN, ...)                                               \
   do {                                                                                        \
     const uint32_t _grid = blocks_for((uint64_t)(N) * (LL), 256);                             \
@@ -1169,12 +1170,12 @@ hipError_t launch_reorder(const Uni &, const Bufs &b, uint32_t n, hipStream_t st
   } while (0)
 #define FLUID_DISPATCH_WALK(KERNEL, C, N, UN, ...)                                            \
   do {                                                                                        \
-    if ((C).wide_walk == 1 && (C).lanes >= 8)                                                 \
+    if ((C).wide_walk == 1)                                                                   \
       FLUID_WALK_CASE(KERNEL, 16, N, UN, __VA_ARGS__);                                        \
-    else if ((C).wide_walk == 2 && (C).lanes >= 8)                                            \
+    else if ((C).wide_walk == 2)                                                              \
       FLUID_WALK_CASE(KERNEL, 32, N, UN, __VA_ARGS__);                                        \
     else                                                                                      \
-      FLUID_DISPATCH_L(KERNEL, (C).lanes < 8 ? (C).lanes : 8, N, UN, __VA_ARGS__);            \
+      FLUID_WALK_CASE(KERNEL, 8, N, UN, __VA_ARGS__);                                         \
   } while (0)
 
 hipError_t launch_density(const Uni &u, const Bufs &b, uint32_t n, SweepCfg c, hipStream_t st) {

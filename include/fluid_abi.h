@@ -356,10 +356,10 @@ int fluid_time_steps(fluid_sim *sim, uint32_t warmup, uint32_t steps,
 int fluid_time_stages(fluid_sim *sim, uint32_t steps, float *ms_per_stage);
 
 /* Launch configuration of the neighbour sweeps, cfg[3]:
- *   cfg[0] lanes cooperating on one particle in density/XSPH/force (they
- *          share the neighbour-list layout): 1, 2, 4, 8, or 16 (density
- *          sweeps with 16 lanes and writes the 8-lane list; XSPH and force
- *          then walk it with 8, 16 or 32 lanes: FLUID_WALK_16 / _32);
+ *   cfg[0] lanes cooperating on one particle in the density sweep: 1, 2,
+ *          4, 8 or 16 (it writes the in-range list for 8 walker lanes
+ *          either way; XSPH and force walk it with 8, 16 or 32 lanes:
+ *          FLUID_WALK_16 / _32);
  *          cfg[1] the same for the PCISPH solve: 1, 2, 4, 8 or 16 (0 keeps
  *          the current value; the LDS-tiled sweeps run at most 8);
  *   cfg[2] workgroup-to-XCD mapping: 1 gives each of the 8 XCDs one
@@ -387,9 +387,9 @@ int fluid_set_tile_config(fluid_sim *sim, const uint32_t cfg[3]);
  *     particle (no later iteration's in-range pair lies outside), and the
  *     later iterations walk that list instead of the 27 cells.
  * These regroup the XSPH and force sums (results within rounding):
- *   FLUID_WALK_16, FLUID_WALK_32 (at most one): with 8 or 16 lanes,
- *     XSPH and force walk the in-range lists with 16 or 32 lanes per
- *     particle, two or four per list lane. */
+ *   FLUID_WALK_16, FLUID_WALK_32 (at most one): XSPH and force walk the
+ *     in-range lists with 16 or 32 lanes per particle (two or four per list
+ *     lane) instead of 8. */
 #define FLUID_FUSE_XSPH_FORCE 1u
 #define FLUID_FUSE_SOLVE_LIST 2u
 #define FLUID_WALK_16 4u
