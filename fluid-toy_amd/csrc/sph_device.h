@@ -165,10 +165,22 @@ __device__ __forceinline__ void sweep27(const DevUniforms *__restrict__ U,
       const uint32_t c0 = row + (uint32_t)x0, c1 = row + (uint32_t)x1;
       const uint32_t begin = starts[c0] - counts[c0];
       const uint32_t end = starts[c1];
+      // Full trips address through one running pointer (the four loads use
+      // immediate offsets); only a SYNC tail trip clamps its indices.
       if (SYNC) {
         if (begin >= end) continue;
         const uint32_t last = end - 1;
-        for (uint32_t base = begin; base < end; base += 4 * L) {
+        uint32_t base = begin;
+        const float4 *p = pts + begin + sub;
+        for (; base + 4 * L <= end; base += 4 * L, p += 4 * L) {
+          const uint32_t j0 = base + sub;
+          const float4 p0 = p[0], p1 = p[L], p2 = p[2 * L], p3 = p[3 * L];
+          f(j0, p0, true);
+          f(j0 + L, p1, true);
+          f(j0 + 2 * L, p2, true);
+          f(j0 + 3 * L, p3, true);
+        }
+        if (base < end) {
           const uint32_t j0 = base + sub, j1 = j0 + L, j2 = j1 + L, j3 = j2 + L;
           const float4 p0 = pts[min(j0, last)], p1 = pts[min(j1, last)];
           const float4 p2 = pts[min(j2, last)], p3 = pts[min(j3, last)];
@@ -179,14 +191,15 @@ __device__ __forceinline__ void sweep27(const DevUniforms *__restrict__ U,
         }
       } else {
         uint32_t j = begin + sub;
-        for (; j + 3 * L < end; j += 4 * L) {
-          const float4 p0 = pts[j], p1 = pts[j + L], p2 = pts[j + 2 * L], p3 = pts[j + 3 * L];
+        const float4 *p = pts + j;
+        for (; j + 3 * L < end; j += 4 * L, p += 4 * L) {
+          const float4 p0 = p[0], p1 = p[L], p2 = p[2 * L], p3 = p[3 * L];
           f(j, p0, true);
           f(j + L, p1, true);
           f(j + 2 * L, p2, true);
           f(j + 3 * L, p3, true);
         }
-        for (; j < end; j += L) f(j, pts[j], true);
+        for (; j < end; j += L, p += L) f(j, p[0], true);
       }
     }
   }

@@ -665,17 +665,17 @@ __device__ __forceinline__ void solve_particle(const DevUniforms *__restrict__ U
     // First iteration: every pressure of the input is 0 (pcisph_predict.wgsl
     // :69), so each pressure term is a signed zero and a_total = a_np
     // exactly; only the predicted density needs the sum.
-    const float skin2 = U->solve_skin2;
+    const float skin2 = U->solve_skin2, mps = mass * ps;
     uint32_t cnt = 0;
     sweep([&](uint32_t j, float4 xj, bool) {
       const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
       const float r2 = dot3(rx, ry, rz, rx, ry, rz);
-      const float d = r2 < h2 ? h2 - r2 : 0.f;
-      pd += mass * (ps * d * d * d);
-      if (use_list && r2 < skin2) {
-        if (cnt < kCap) lst[cnt * 64u] = j;
-        ++cnt;
-      }
+      // h2 - r2 > 0 exactly when r2 < h2, so the max is the support cut
+      const float d = fmaxf(h2 - r2, 0.f);
+      pd = __builtin_fmaf(mps, d * d * d, pd);
+      const bool keep = use_list && r2 < skin2;
+      if (keep && cnt < kCap) lst[cnt * 64u] = j;
+      cnt += keep ? 1u : 0u;
     });
     if (use_list) b.scnt[(size_t)s * L + sub] = cnt <= kCap ? cnt : kSolveListOverflow;
   } else {
