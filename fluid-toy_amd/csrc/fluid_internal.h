@@ -148,6 +148,9 @@ struct Bufs {
 constexpr uint32_t kScanThreads = 256;
 constexpr uint32_t kScanItems = 16;
 constexpr uint32_t kScanTile = kScanThreads * kScanItems;
+// grids of at least this many cells scan with 4x larger tiles (the tile
+// status array is sized for the smaller tiles, so either fits)
+constexpr uint32_t kScanBigGrid = 1u << 20;
 
 // In-range neighbour entries stored per particle (split over its lanes).
 constexpr uint32_t kListCapacity = 1024;

@@ -87,27 +87,32 @@ __global__ __launch_bounds__(256) void k_build(Uni un, Bufs b, uint32_t n) {
 // as one single-pass inclusive scan with decoupled look-back.  Tiles take
 // ids in launch order from an atomic counter, so a tile only waits on tiles
 // that are already running; every spin is capped (error bit on overflow).
+// ITEMS cells per thread: the look-back costs a roughly fixed time per tile,
+// so large grids use 4x larger tiles (16 -> 64 items: 5M cells 0.16 ->
+// 0.05 ms), small ones the smaller tiles' shorter per-thread scan.
+template <uint32_t ITEMS>
 __global__ __launch_bounds__(256) void k_scan(Bufs b, uint32_t cells) {
+  constexpr uint32_t TILE = kScanThreads * ITEMS;
   __shared__ uint32_t s_tile, s_prefix;
   __shared__ uint32_t s_wave[4];
   if (threadIdx.x == 0) s_tile = atomicAdd(b.tile_counter, 1u);
   __syncthreads();
   const uint32_t tile = s_tile;
-  const uint32_t base = tile * kScanTile + threadIdx.x * kScanItems;
-  uint32_t v[kScanItems];
-  if (base + kScanItems <= cells) {
+  const uint32_t base = tile * TILE + threadIdx.x * ITEMS;
+  uint32_t v[ITEMS];
+  if (base + ITEMS <= cells) {
 #pragma unroll
-    for (uint32_t q = 0; q < kScanItems / 4; ++q) {
+    for (uint32_t q = 0; q < ITEMS / 4; ++q) {
       const uint4 x = *reinterpret_cast<const uint4 *>(b.counts + base + 4 * q);
       v[4 * q] = x.x; v[4 * q + 1] = x.y; v[4 * q + 2] = x.z; v[4 * q + 3] = x.w;
     }
   } else {
 #pragma unroll
-    for (uint32_t q = 0; q < kScanItems; ++q) v[q] = base + q < cells ? b.counts[base + q] : 0u;
+    for (uint32_t q = 0; q < ITEMS; ++q) v[q] = base + q < cells ? b.counts[base + q] : 0u;
   }
 #pragma unroll
-  for (uint32_t q = 1; q < kScanItems; ++q) v[q] += v[q - 1];
-  const uint32_t tsum = v[kScanItems - 1];
+  for (uint32_t q = 1; q < ITEMS; ++q) v[q] += v[q - 1];
+  const uint32_t tsum = v[ITEMS - 1];
   const uint32_t incl = wave_incl_scan(tsum);
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
   if (lane == 63) s_wave[wid] = incl;
@@ -158,14 +163,14 @@ __global__ __launch_bounds__(256) void k_scan(Bufs b, uint32_t cells) {
   }
   __syncthreads();
   const uint32_t off = s_prefix + wave_off + (incl - tsum);
-  if (base + kScanItems <= cells) {
+  if (base + ITEMS <= cells) {
 #pragma unroll
-    for (uint32_t q = 0; q < kScanItems / 4; ++q)
+    for (uint32_t q = 0; q < ITEMS / 4; ++q)
       *reinterpret_cast<uint4 *>(b.starts + base + 4 * q) =
           make_uint4(off + v[4 * q], off + v[4 * q + 1], off + v[4 * q + 2], off + v[4 * q + 3]);
   } else {
 #pragma unroll
-    for (uint32_t q = 0; q < kScanItems; ++q)
+    for (uint32_t q = 0; q < ITEMS; ++q)
       if (base + q < cells) b.starts[base + q] = off + v[q];
   }
 }
@@ -1122,8 +1127,12 @@ hipError_t launch_build(const Uni &u, const Bufs &b, uint32_t n, hipStream_t st)
 
 hipError_t launch_scan(const Uni &u, const Bufs &b, uint32_t cells, SweepCfg c,
                        hipStream_t st) {
-  hipLaunchKernelGGL(k_scan, dim3(blocks_for(cells, kScanTile)), dim3(kScanThreads), 0, st,
-                     b, cells);
+  if (cells >= kScanBigGrid)
+    hipLaunchKernelGGL(k_scan<4 * kScanItems>, dim3(blocks_for(cells, 4 * kScanTile)),
+                       dim3(kScanThreads), 0, st, b, cells);
+  else
+    hipLaunchKernelGGL(k_scan<kScanItems>, dim3(blocks_for(cells, kScanTile)),
+                       dim3(kScanThreads), 0, st, b, cells);
   if (c.tiled && b.tiles.total)
     FLUID_LAUNCH_B(k_tiles, blocks_for(b.tiles.total, 256), u, b);
   return hipGetLastError();

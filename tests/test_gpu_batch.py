@@ -72,3 +72,30 @@ def test_batch_rollout_invariants(pkg):
     with pytest.raises(pkg.abi.FluidError):
         batch.spawn_particles([0, 0, 0], 4, 0.1)
     batch.close()
+
+
+def test_batch_large_grid_scan(pkg):
+    """A batch whose concatenated grid has over 2^20 cells (the scan's large
+    tiles): inclusive starts equal the cumulative counts over the whole
+    concatenation, and scenes stepped by plain handles (small grids, the
+    small tiles) give the same cell counts and particles."""
+    scenes = pkg.scenes.batch_scenes(64, seed=21, n_min=2048, n_max=4096)
+    batch = pkg.make_batch(scenes)
+    cells = sum(batch.scene_info(k)["cells"] for k in range(len(scenes)))
+    print(f"case: 64-scene batch, {cells} cells", flush=True)
+    assert cells >= 1 << 20
+    batch.step(2)
+    counts = batch.read_cell_counts().astype(np.int64)
+    starts = batch.read_cell_starts().astype(np.int64)
+    np.testing.assert_array_equal(starts, np.cumsum(counts))
+    for k in (0, 31, 63):
+        info = batch.scene_info(k)
+        plain = pkg.make_sim(scenes[k])
+        plain.step(2)
+        cl = slice(info["first_cell"], info["first_cell"] + info["cells"])
+        np.testing.assert_array_equal(counts[cl], plain.read_cell_counts())
+        got = batch.read_particles()[info["first"]:info["first"] + info["count"]]
+        err = rowwise_rel_err(got[:, 0:7], plain.read_particles()[:, 0:7])
+        assert err.max() <= 1e-5, (k, err.max())
+        plain.close()
+    batch.close()
