@@ -13,11 +13,16 @@
 //                     pass and gathered by XSPH and force:
 //                       (pos, density), (vel, near_density),
 //                       (normal, 1 / max(near_density, 1))
-//     nbr, ncnt       in-range neighbour lists of the density pass: entry e
-//                     of slot s at nbr[(e / lanes) * n * lanes + s * lanes
-//                     + e % lanes], i.e. lane e % lanes's share, so a wave's
-//                     k-th reads are one contiguous run; ncnt[s] is the
-//                     entry count, > kListCapacity marking an overflow
+//     nbr, ncnt       in-range neighbour lists of the density pass, written
+//                     for kListLanes = 8 walker lanes: entry e of slot s is
+//                     list lane e % 8's entry at depth e / 8, at
+//                     nbr[lane_block<8>(s, kListCapacity) + (e / 8) * 64 +
+//                     e % 8] (blocks of the 8 slots a wave of 8-lane walkers
+//                     covers, so a wave's depth-d reads are one contiguous
+//                     run); ncnt[s] is the entry count, > kListCapacity
+//                     marking an overflow
+//     slist, scnt     the solve's per-lane skin lists (same block layout
+//                     for the solve's lane count) and per-lane counts
 //     pred[2]         sorted_predicted_{a,b} split into
 //                       xq = (x*, q) with q = p / max(rho*,1)^2,
 //                       pr = (p, rho*), v = (v*, 0)

@@ -286,10 +286,10 @@ __global__ __launch_bounds__(256) void k_rank_gather(Bufs b, uint32_t n) {
 
 // ------------------------------------------------------------------------
 // Stage 4 — sph_density_3d_grid.wgsl:111-194 (+ boundary_gamma :86-109).
-// Besides density, near density and the Akinci normal, each lane records the
-// in-range neighbours it met (0 < r^2 < h^2, the pairs XSPH and force sum
-// over) in its list, and the particle's neighbour record is written for the
-// later gathers.
+// Besides density, near density and the Akinci normal, the particle's L
+// lanes record the in-range neighbours they meet (0 < r^2 < h^2, the pairs
+// XSPH and force sum over) in one ballot-balanced list for 8 walker lanes,
+// and the particle's neighbour record is written for the later gathers.
 // Per-particle body of the density stage; sweep(f) walks the particle's
 // candidates group-synchronously (sweep27 or tile_sweep with SYNC).
 template <int L, class Sweep>
