@@ -187,6 +187,8 @@ FLUID_FUSE_XSPH_FORCE = 1
 FLUID_FUSE_SOLVE_LIST = 2
 FLUID_WALK_16 = 4
 FLUID_WALK_32 = 8
+FLUID_SOLVE_WALK_X2 = 16
+FLUID_SOLVE_WALK_X4 = 32
 
 # Gradient slots per scene written by fluid_backward_rollout.
 FLUID_GRAD_PARAMS = 5

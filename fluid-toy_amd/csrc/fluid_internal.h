@@ -189,6 +189,9 @@ struct SweepCfg {
   int wide_walk;    // 0 / 1 / 2: XSPH and force walk the 8-lane in-range
                     // lists with 8 / 16 / 32 lanes per particle (1, 2 or 4
                     // per list lane, interleaved depths)
+  int solve_walk;   // 0 / 1 / 2 (global sweeps, skin lists on, 2 <= solve
+                    // lanes <= 8): the later PCISPH iterations walk the skin
+                    // lists with 1 / 2 / 4 walkers per list lane
 };
 
 // Host-side launchers (sph_kernels.hip).  All enqueue on `st`.

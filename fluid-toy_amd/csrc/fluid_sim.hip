@@ -89,7 +89,7 @@ struct fluid_sim {
   uint32_t graph_n = 0, graph_iters = 0;
   int graph_lanes = 0, graph_solve_lanes = 0, graph_xcd = 0, graph_tiled = 0, graph_wgs = 0;
   uint32_t graph_tile_w = 0;
-  int graph_fuse_xf = 0, graph_solve_list = 0, graph_wide = 0;
+  int graph_fuse_xf = 0, graph_solve_list = 0, graph_wide = 0, graph_solve_walk = 0;
   uint32_t num_cu = 256;
   uint32_t *d_scene_first = nullptr;  // first tile id of each scene (+ total)
   int cursor = -1;  // last completed stage of a partial step, -1 = none
@@ -119,7 +119,7 @@ SweepCfg pick_cfg(uint32_t) {
   // Default before tuning: 8 lanes per particle, the fastest setting the
   // autotuner found at every size measured (8k to 1M particles, compressed
   // and relaxed states); global-memory sweeps.
-  return SweepCfg{8, 8, 0, 0, 1024, 1, 1, 0};
+  return SweepCfg{8, 8, 0, 0, 1024, 1, 1, 0, 0};
 }
 
 // Per-scene tile counts for tiles of w cells along x; uploads the scene
@@ -145,12 +145,13 @@ int setup_tiles(fluid_sim *s, uint32_t w) {
 void update_cfg(fluid_sim *s) {
   if (s->tuned) return;
   const int wgs = s->cfg.tile_wgs, fuse = s->cfg.fuse_xf, sl = s->cfg.solve_list;
-  const int wide = s->cfg.wide_walk;
+  const int wide = s->cfg.wide_walk, sw = s->cfg.solve_walk;
   s->cfg = pick_cfg(s->n);
   s->cfg.tile_wgs = wgs;
   s->cfg.fuse_xf = fuse;
   s->cfg.solve_list = sl;
   s->cfg.wide_walk = wide;
+  s->cfg.solve_walk = sw;
 }
 
 int push_uniforms(fluid_sim *s) {
@@ -248,7 +249,7 @@ int ensure_graph(fluid_sim *s) {
       s->graph_xcd == s->cfg.xcd_tiles && s->graph_tiled == s->cfg.tiled &&
       s->graph_wgs == s->cfg.tile_wgs && s->graph_tile_w == s->b.tiles.w &&
       s->graph_fuse_xf == s->cfg.fuse_xf && s->graph_solve_list == s->cfg.solve_list &&
-      s->graph_wide == s->cfg.wide_walk)
+      s->graph_wide == s->cfg.wide_walk && s->graph_solve_walk == s->cfg.solve_walk)
     return FLUID_OK;
   drop_graph(s);
   hipGraph_t g = nullptr;
@@ -277,6 +278,7 @@ int ensure_graph(fluid_sim *s) {
   s->graph_fuse_xf = s->cfg.fuse_xf;
   s->graph_solve_list = s->cfg.solve_list;
   s->graph_wide = s->cfg.wide_walk;
+  s->graph_solve_walk = s->cfg.solve_walk;
   return FLUID_OK;
 }
 
@@ -1209,14 +1211,19 @@ int fluid_set_tile_config(fluid_sim *s, const uint32_t cfg[3]) {
 
 int fluid_set_fusion(fluid_sim *s, uint32_t flags) {
   if (!s) return fail(FLUID_ERR_INVALID_ARG, "null handle");
-  if (flags & ~(FLUID_FUSE_XSPH_FORCE | FLUID_FUSE_SOLVE_LIST | FLUID_WALK_16 | FLUID_WALK_32))
+  if (flags & ~(FLUID_FUSE_XSPH_FORCE | FLUID_FUSE_SOLVE_LIST | FLUID_WALK_16 | FLUID_WALK_32 |
+                FLUID_SOLVE_WALK_X2 | FLUID_SOLVE_WALK_X4))
     return fail(FLUID_ERR_INVALID_ARG, "unknown fusion flag");
   if ((flags & FLUID_WALK_16) && (flags & FLUID_WALK_32))
     return fail(FLUID_ERR_INVALID_ARG, "FLUID_WALK_16 and FLUID_WALK_32 exclude each other");
+  if ((flags & FLUID_SOLVE_WALK_X2) && (flags & FLUID_SOLVE_WALK_X4))
+    return fail(FLUID_ERR_INVALID_ARG,
+                "FLUID_SOLVE_WALK_X2 and FLUID_SOLVE_WALK_X4 exclude each other");
   if (s->cursor != -1) return fail(FLUID_ERR_STATE, "a partial step is in progress");
   s->cfg.fuse_xf = (flags & FLUID_FUSE_XSPH_FORCE) ? 1 : 0;
   s->cfg.solve_list = (flags & FLUID_FUSE_SOLVE_LIST) ? 1 : 0;
   s->cfg.wide_walk = (flags & FLUID_WALK_16) ? 1 : (flags & FLUID_WALK_32) ? 2 : 0;
+  s->cfg.solve_walk = (flags & FLUID_SOLVE_WALK_X2) ? 1 : (flags & FLUID_SOLVE_WALK_X4) ? 2 : 0;
   s->roll.valid = false;
   return FLUID_OK;
 }
@@ -1226,7 +1233,9 @@ uint32_t fluid_fusion(const fluid_sim *s) {
   return (s->cfg.fuse_xf ? FLUID_FUSE_XSPH_FORCE : 0u) |
          (s->cfg.solve_list ? FLUID_FUSE_SOLVE_LIST : 0u) |
          (s->cfg.wide_walk == 1 ? FLUID_WALK_16 : 0u) |
-         (s->cfg.wide_walk == 2 ? FLUID_WALK_32 : 0u);
+         (s->cfg.wide_walk == 2 ? FLUID_WALK_32 : 0u) |
+         (s->cfg.solve_walk == 1 ? FLUID_SOLVE_WALK_X2 : 0u) |
+         (s->cfg.solve_walk == 2 ? FLUID_SOLVE_WALK_X4 : 0u);
 }
 
 int fluid_get_tile_config(const fluid_sim *s, uint32_t out[3]) {
@@ -1298,6 +1307,7 @@ int fluid_autotune(fluid_sim *s, uint32_t trials, uint32_t out[3]) {
     mc.tiled = m.tiled;
     mc.xcd_tiles = m.xcd;
     mc.wide_walk = 0;  // decided with the fusion below
+    mc.solve_walk = 0;
     for (int li = 0; li < 5 && e == hipSuccess && rc == FLUID_OK; ++li) {
       if (m.tiled && lanes[li] > 8) continue;  // tiled sweeps run at most 8 lanes
       s->cfg = mc;
@@ -1360,6 +1370,28 @@ int fluid_autotune(fluid_sim *s, uint32_t trials, uint32_t out[3]) {
     }
     best.fuse_xf = fuse;
     best.wide_walk = wide;
+  }
+  // the later PCISPH iterations' walkers per skin-list lane, as whole steps
+  if (best.solve_list && !best.tiled && best.solve_lanes >= 2 && best.solve_lanes <= 8) {
+    float best_f = 1e30f;
+    int sw = 0;
+    for (int v = 0; v < 3 && e == hipSuccess && rc == FLUID_OK; ++v) {
+      s->cfg = best;
+      s->cfg.solve_walk = v;
+      float tot = 0.f, ms = 0.f;
+      for (uint32_t t = 0; t < trials && rc == FLUID_OK; ++t) {
+        if ((e = copy(false)) != hipSuccess) break;
+        rc = fluid_time_steps(s, 0, 2, &ms);
+        tot += ms;
+      }
+      // one walker per list lane sums in the sweep's order; the wider walks
+      // regroup the sums, so they must win by 2% to be taken
+      if (v == 0 || tot < 0.98f * best_f) {
+        best_f = tot;
+        sw = v;
+      }
+    }
+    best.solve_walk = sw;
   }
   if (e == hipSuccess) e = copy(false);
   if (e == hipSuccess)

@@ -655,18 +655,24 @@ __global__ __launch_bounds__(256) void k_predict(Uni un, Bufs b, uint32_t n) {
 // the first iteration's predicted positions); every later iteration's
 // in-range pairs are among them, so the later iterations walk that list and
 // sum the same pairs in the same order per lane (the same bits as a sweep).
-template <int L, class Sweep>
+// With LS < L (later iterations only), L / LS walkers share each of the LS
+// list lanes, taking interleaved depths (the same pairs, grouped otherwise;
+// an overflowed list lane's walkers sweep its candidates between them).
+template <int L, class Sweep, int LS = L>
 __device__ __forceinline__ void solve_particle(const DevUniforms *__restrict__ U, const Bufs &b,
                                                uint32_t n, const PredBuf &in, const PredBuf &out,
                                                float4 *ap_out, uint32_t first_iter,
                                                uint32_t use_list, uint32_t s, uint32_t sub,
                                                float4 xi, float4 p0, Sweep &&sweep) {
+  static_assert(L % LS == 0, "walkers are a multiple of the list lanes");
+  constexpr uint32_t M = L / LS;
   const float h = U->h, h2 = U->h2, mass = U->mass, ps = U->poly6_scale;
   const float gs = U->grad_scale;
-  uint32_t *__restrict__ lst = b.slist + lane_block<L>(s, kSolveListCapacity) + sub;
-  constexpr uint32_t kCap = kSolveListCapacity / L;
+  const uint32_t kl = sub % LS, d0 = sub / LS;
+  uint32_t *__restrict__ lst = b.slist + lane_block<LS>(s, kSolveListCapacity) + kl;
+  constexpr uint32_t kCap = kSolveListCapacity / LS;
   float pd = 0.f, ax = 0.f, ay = 0.f, az = 0.f;
-  if (first_iter) {
+  if (M == 1 && first_iter) {
     // First iteration: every pressure of the input is 0 (pcisph_predict.wgsl
     // :69), so each pressure term is a signed zero and a_total = a_np
     // exactly; only the predicted density needs the sum.
@@ -682,7 +688,7 @@ __device__ __forceinline__ void solve_particle(const DevUniforms *__restrict__ U
       if (keep && cnt < kCap) lst[cnt * 64u] = j;
       cnt += keep ? 1u : 0u;
     });
-    if (use_list) b.scnt[(size_t)s * L + sub] = cnt <= kCap ? cnt : kSolveListOverflow;
+    if (use_list) b.scnt[(size_t)s * LS + kl] = cnt <= kCap ? cnt : kSolveListOverflow;
   } else {
     auto body = [&](uint32_t, float4 xj, bool) {
       const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
@@ -701,21 +707,24 @@ __device__ __forceinline__ void solve_particle(const DevUniforms *__restrict__ U
         }
       }
     };
-    const uint32_t cnt = use_list ? b.scnt[(size_t)s * L + sub] : kSolveListOverflow;
-    if (cnt != kSolveListOverflow) {
+    const uint32_t cntk = use_list ? b.scnt[(size_t)s * LS + kl] : kSolveListOverflow;
+    if (cntk != kSolveListOverflow) {
+      // this walker's depths d0, d0 + M, ... of list lane kl
+      const uint32_t cnt = cntk > d0 ? (cntk - 1 - d0) / M + 1 : 0;
+      const uint32_t *__restrict__ p = lst + d0 * 64u;
+      constexpr uint32_t kStep = 64u * M;
       const float4 *__restrict__ xq = in.xq;
       uint32_t k = 0;
-      for (; k + 3 < cnt; k += 4) {
-        const uint32_t j0 = lst[k * 64u], j1 = lst[k * 64u + 64u];
-        const uint32_t j2 = lst[k * 64u + 128u], j3 = lst[k * 64u + 192u];
+      for (; k + 3 < cnt; k += 4, p += 4 * kStep) {
+        const uint32_t j0 = p[0], j1 = p[kStep], j2 = p[2 * kStep], j3 = p[3 * kStep];
         const float4 x0 = xq[j0], x1 = xq[j1], x2 = xq[j2], x3 = xq[j3];
         body(j0, x0, true);
         body(j1, x1, true);
         body(j2, x2, true);
         body(j3, x3, true);
       }
-      for (; k < cnt; ++k) {
-        const uint32_t j0 = lst[k * 64u];
+      for (; k < cnt; ++k, p += kStep) {
+        const uint32_t j0 = p[0];
         body(j0, xq[j0], true);
       }
     } else {
@@ -764,6 +773,27 @@ __global__ __launch_bounds__(256) void k_solve(Uni un, Bufs b, uint32_t n, PredB
   solve_particle<L>(U, b, n, in, out, ap_out, first_iter, use_list, s, sub, xi, p0, [&](auto &&f) {
     sweep27<L, false>(U, b.counts, b.starts, in.xq, cx, cy, cz, sub, f);
   });
+}
+
+// Later PCISPH iterations with L = LS * M walkers over the LS-lane skin lists
+// the first iteration recorded (no recording here).
+template <int L, int LS, bool B>
+__global__ __launch_bounds__(256) void k_solve_w(Uni un, Bufs b, uint32_t n, PredBuf in,
+                                                 PredBuf out, float4 *ap_out, uint32_t xcd) {
+  const uint32_t g = xcd_block(xcd) * blockDim.x + threadIdx.x;
+  const uint32_t s = g / L, sub = g % L;
+  if (s >= n) return;
+  const DevUniforms *__restrict__ U = uni_at<B>(un, s);
+  const float4 xi = in.xq[s];
+  const float4 p0 = b.spos[s];
+  const int cx = cell_coord(p0.x, U->ox, U->inv_cell);
+  const int cy = cell_coord(p0.y, U->oy, U->inv_cell);
+  const int cz = cell_coord(p0.z, U->oz, U->inv_cell);
+  auto sweep = [&](auto &&f) {
+    sweep27<L, false>(U, b.counts, b.starts, in.xq, cx, cy, cz, sub, f);
+  };
+  solve_particle<L, decltype(sweep) &, LS>(U, b, n, in, out, ap_out, 0u, 1u, s, sub, xi, p0,
+                                           sweep);
 }
 
 // Tiled form of k_solve (see k_density_t): the tile's candidate rows of the
@@ -1235,7 +1265,26 @@ hipError_t launch_solve(const Uni &u, const Bufs &b, uint32_t n, const PredBuf &
   if (c.tiled)
     FLUID_DISPATCH_TILED(k_solve_t, c.lanes, u, b, n, in, out, ap_out,
                          (uint32_t)(first_iter ? 1 : 0), (uint32_t)c.solve_list);
-  else if (c.solve_lanes == 16)
+  else if (!first_iter && c.solve_list && c.solve_walk && c.solve_lanes >= 2 &&
+           c.solve_lanes <= 8) {
+    // later iterations: 2 or 4 walkers per skin-list lane
+    const int M = 1 << c.solve_walk, LS = c.solve_lanes;
+    const uint32_t grid = blocks_for((uint64_t)n * (uint64_t)(LS * M), 256);
+#define FLUID_SOLVE_W(LL, LSS)                                                                  \
+  if (u.scene_of)                                                                               \
+    hipLaunchKernelGGL((k_solve_w<LL, LSS, true>), dim3(grid), dim3(256), 0, st, u, b, n, in,   \
+                       out, ap_out, (uint32_t)c.xcd_tiles);                                      \
+  else                                                                                          \
+    hipLaunchKernelGGL((k_solve_w<LL, LSS, false>), dim3(grid), dim3(256), 0, st, u, b, n, in,  \
+                       out, ap_out, (uint32_t)c.xcd_tiles);
+    if (LS == 2 && M == 2) { FLUID_SOLVE_W(4, 2) }
+    else if (LS == 2) { FLUID_SOLVE_W(8, 2) }
+    else if (LS == 4 && M == 2) { FLUID_SOLVE_W(8, 4) }
+    else if (LS == 4) { FLUID_SOLVE_W(16, 4) }
+    else if (M == 2) { FLUID_SOLVE_W(16, 8) }
+    else { FLUID_SOLVE_W(32, 8) }
+#undef FLUID_SOLVE_W
+  } else if (c.solve_lanes == 16)
     FLUID_WALK_CASE(k_solve, 16, n, u, b, n, in, out, ap_out, (uint32_t)c.xcd_tiles,
                     (uint32_t)(first_iter ? 1 : 0), (uint32_t)c.solve_list);
   else

@@ -389,11 +389,16 @@ int fluid_set_tile_config(fluid_sim *sim, const uint32_t cfg[3]);
  * These regroup the XSPH and force sums (results within rounding):
  *   FLUID_WALK_16, FLUID_WALK_32 (at most one): XSPH and force walk the
  *     in-range lists with 16 or 32 lanes per particle (two or four per list
- *     lane) instead of 8. */
+ *     lane) instead of 8;
+ *   FLUID_SOLVE_WALK_X2, FLUID_SOLVE_WALK_X4 (at most one; with skin lists,
+ *     global sweeps and 2 to 8 solve lanes): the later PCISPH iterations walk
+ *     the skin lists with two or four walkers per list lane. */
 #define FLUID_FUSE_XSPH_FORCE 1u
 #define FLUID_FUSE_SOLVE_LIST 2u
 #define FLUID_WALK_16 4u
 #define FLUID_WALK_32 8u
+#define FLUID_SOLVE_WALK_X2 16u
+#define FLUID_SOLVE_WALK_X4 32u
 int fluid_set_fusion(fluid_sim *sim, uint32_t flags);
 uint32_t fluid_fusion(const fluid_sim *sim);
 int fluid_get_tile_config(const fluid_sim *sim, uint32_t out[3]);

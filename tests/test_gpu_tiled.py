@@ -191,3 +191,56 @@ def test_density_16_lanes(pkg, wide):
             assert err.max() < tol, (step, name, err.max())
     a.close()
     b.close()
+
+
+@pytest.mark.parametrize("solve_lanes", [2, 4, 8])
+@pytest.mark.parametrize("walk", ["x2", "x4"])
+def test_solve_wide_walk(pkg, solve_lanes, walk):
+    """The later PCISPH iterations with two or four walkers per skin-list
+    lane sum the same pairs as one walker, grouped otherwise: from the same
+    state, one step agrees to the solve's rounding noise (predicted density
+    1e-5; velocity and position 2e-3, per particle with a floor of 1% of the
+    field RMS, as the oracle parity tests; pressure 2e-3 likewise)."""
+    from conftest import rowwise_rel_err
+    flag = pkg.abi.FLUID_SOLVE_WALK_X2 if walk == "x2" else pkg.abi.FLUID_SOLVE_WALK_X4
+    base = pkg.abi.FLUID_FUSE_XSPH_FORCE | pkg.abi.FLUID_FUSE_SOLVE_LIST
+    scene = pkg.scenes.dam_break_8k()
+    a = pkg.make_sim(scene)
+    b = pkg.make_sim(scene)
+    for s in (a, b):
+        s.set_sweep_config(8, solve_lanes, 0)
+        s.set_tile_config(0)
+    a.set_fusion(base)
+    b.set_fusion(base | flag)
+    assert b.fusion() == base | flag
+    print(f"case: solve walk {walk} solve_lanes={solve_lanes}", flush=True)
+    for step in range(4):
+        a.write_particles(b.read_particles())
+        a.step(1)
+        b.step(1)
+        pa, pb = a.read_sorted_particles(), b.read_sorted_particles()
+        assert np.array_equal(a.read_sorted_to_orig(), b.read_sorted_to_orig()), step
+        rows = [(pb[:, sl], pa[:, sl], name, tol)
+                for sl, name, tol in ((slice(4, 7), "velocity", 2e-3),
+                                      (slice(0, 3), "position", 2e-3))]
+        for k in (0, 1):
+            qa, qb = a.read_predicted(k), b.read_predicted(k)
+            rows += [(qb[:, 7:8], qa[:, 7:8], f"predicted density {k}", 1e-5),
+                     (qb[:, 3:4], qa[:, 3:4], f"pressure {k}", 2e-3)]
+        for got, want, name, tol in rows:
+            err = rowwise_rel_err(got, want)
+            print(f"  step {step} {name}: max {err.max():.2e}", flush=True)
+            assert err.max() < tol, (step, name, err.max())
+    a.close()
+    b.close()
+
+
+def test_solve_walk_flags(pkg):
+    """The two solve walks exclude each other; the flags read back as set."""
+    sim = pkg.make_sim(pkg.scenes.dam_break_8k())
+    with pytest.raises(pkg.abi.FluidError):
+        sim.set_fusion(pkg.abi.FLUID_SOLVE_WALK_X2 | pkg.abi.FLUID_SOLVE_WALK_X4)
+    flags = pkg.abi.FLUID_SOLVE_WALK_X4 | pkg.abi.FLUID_FUSE_SOLVE_LIST
+    sim.set_fusion(flags)
+    assert sim.fusion() == flags
+    sim.close()
