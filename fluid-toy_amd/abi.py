@@ -108,6 +108,58 @@ class FluidSceneDesc(C.Structure):
                 ("gravity", GpuGravity), ("bounds_extent", C.c_float)]
 
 
+class GpuSprayParams(C.Structure):
+    """src/state/interaction.rs:131-153 (80 B)."""
+    _fields_ = [("min_speed", C.c_float), ("emission_rate", C.c_float),
+                ("lifetime", C.c_float), ("lifetime_variation", C.c_float),
+                ("drag", C.c_float), ("speed_multiplier", C.c_float),
+                ("velocity_jitter", C.c_float), ("dt", C.c_float),
+                ("max_particles", C.c_uint32), ("num_sph_particles", C.c_uint32),
+                ("frame_count", C.c_uint32), ("gravity_y", C.c_float),
+                ("k_trapped_air", C.c_float), ("k_wave_crest", C.c_float),
+                ("ta_limit", C.c_float), ("bubble_buoyancy", C.c_float),
+                ("bubble_drag", C.c_float), ("wc_limit", C.c_float),
+                ("_pad", C.c_float * 2)]
+
+
+class GpuSprayParticle(C.Structure):
+    """src/state/interaction.rs:160-174 (48 B)."""
+    _fields_ = [("pos_x", C.c_float), ("pos_y", C.c_float), ("pos_z", C.c_float),
+                ("lifetime", C.c_float), ("vel_x", C.c_float), ("vel_y", C.c_float),
+                ("vel_z", C.c_float), ("max_lifetime", C.c_float), ("kind", C.c_uint32),
+                ("age", C.c_float), ("_pad", C.c_float * 2)]
+
+
+class FluidSprayState(C.Structure):
+    _fields_ = [("write_head", C.c_uint32), ("ta_limit", C.c_float), ("wc_limit", C.c_float),
+                ("live", C.c_uint32 * 4), ("emitted", C.c_uint32)]
+
+
+# GpuSprayParticle (src/state/interaction.rs:160-174): 12 words per slot;
+# word 8 (kind) is a u32, the rest f32.
+SPRAY_POS = slice(0, 3)
+SPRAY_LIFETIME = 3
+SPRAY_VEL = slice(4, 7)
+SPRAY_MAX_LIFETIME = 7
+SPRAY_KIND = 8
+SPRAY_AGE = 9
+SPRAY_KIND_NAMES = ("spray", "foam", "bubble")
+
+
+def default_spray_params(dt: float, gravity: float = 9.81, max_particles: int = 100_000,
+                         num_sph_particles: int = 0, frame_count: int = 0) -> GpuSprayParams:
+    """SprayConfig::default (src/state/interaction.rs:67-92) as the app packs
+    it (app.rs:1048-1079): dt is the frame time, gravity_y = -gravity, and the
+    ceilings at their initial guesses (spray.rs:30-31)."""
+    p = GpuSprayParams()
+    p.min_speed, p.emission_rate, p.lifetime, p.lifetime_variation = 0.8, 30.0, 2.8, 0.5
+    p.drag, p.speed_multiplier, p.velocity_jitter, p.dt = 1.5, 0.85, 0.9, dt
+    p.max_particles, p.num_sph_particles, p.frame_count = max_particles, num_sph_particles, frame_count
+    p.gravity_y, p.k_trapped_air, p.k_wave_crest = -gravity, 1.25, 1.4
+    p.ta_limit, p.bubble_buoyancy, p.bubble_drag, p.wc_limit = 3.0, 2.5, 0.6, 1.0
+    return p
+
+
 def default_mouse_force() -> GpuMouseForce:
     """GpuMouseForce::default (src/state/interaction.rs:113-126)."""
     m = GpuMouseForce()
@@ -181,6 +233,20 @@ PROTOTYPES = {
     "fluid_forward_rollout": (C.c_int, [P, u32, u32]),
     "fluid_backward_rollout": (C.c_int, [P, P, P, P, P, P]),
     "fluid_rollout_timing": (C.c_int, [P, C.POINTER(C.c_float)]),
+    "fluid_spray_create": (C.c_int, [P, u32, C.POINTER(GpuSprayParams), C.POINTER(P)]),
+    "fluid_spray_destroy": (C.c_int, [P]),
+    "fluid_spray_update_params": (C.c_int, [P, C.POINTER(GpuSprayParams)]),
+    "fluid_spray_step": (C.c_int, [P, u32]),
+    "fluid_spray_emit": (C.c_int, [P, u32]),
+    "fluid_spray_simulate": (C.c_int, [P]),
+    "fluid_spray_auto_limits": (C.c_int, [P, C.POINTER(C.c_float)]),
+    "fluid_spray_read_stats": (C.c_int, [P, C.POINTER(u32)]),
+    "fluid_spray_read_state": (C.c_int, [P, C.POINTER(FluidSprayState)]),
+    "fluid_spray_read_particles": (C.c_int, [P, P, u32]),
+    "fluid_spray_write_particles": (C.c_int, [P, P, u32]),
+    "fluid_spray_read_emit": (C.c_int, [P, P, P, u32]),
+    "fluid_spray_reset": (C.c_int, [P]),
+    "fluid_spray_time_steps": (C.c_int, [P, u32, u32, u32, C.POINTER(C.c_float)]),
 }
 
 FLUID_FUSE_XSPH_FORCE = 1

@@ -53,6 +53,10 @@ static_assert(sizeof(GpuRigidBody) == 96, "GpuRigidBody is 96 B");
 static_assert(offsetof(GpuRigidBody, rot_row0) == 48, "rot @48");
 static_assert(sizeof(GpuRigidBodyAccum) == 32, "accum is 32 B");
 static_assert(sizeof(PredictedState) == 32, "PredictedState is 32 B");
+static_assert(sizeof(GpuSprayParams) == 80, "GpuSprayParams is 80 B");
+static_assert(offsetof(GpuSprayParams, wc_limit) == 68, "wc_limit @68");
+static_assert(sizeof(GpuSprayParticle) == 48, "GpuSprayParticle is 48 B");
+static_assert(offsetof(GpuSprayParticle, kind) == 32, "kind @32");
 
 namespace fluid {
 
@@ -229,6 +233,38 @@ hipError_t launch_integrate(const Uni &u, const Bufs &b, uint32_t n,
                             const float4 *fin_v, hipStream_t st);
 hipError_t launch_count_pairs(const Uni &u, const Bufs &b, uint32_t n,
                               unsigned long long *out4, hipStream_t st);
+
+// Device state of a spray handle.  The reference keeps the write head in a
+// storage buffer and the potential ceilings on the host (spray.rs:85-100);
+// here both live on the device.  `emitted` is the last emit pass's spawn
+// count, added to `head` by the next emit pass (its spawn kernel reads the
+// head while the pass runs).
+struct SprayDev {
+  uint32_t head;
+  float ta_limit, wc_limit;
+  uint32_t live[4];        // total, spray, foam, bubble after the last simulate
+  uint32_t emitted;
+  uint32_t frame_max[2];   // f32 bits of the potential maxima since the last simulate
+  uint32_t pad[6];
+};
+
+struct SprayBufs {
+  SprayDev *d;
+  float4 *ring;            // GpuSprayParticle as three float4 per slot
+  uint32_t cap;            // ring slots
+  uint32_t *emit_n;        // spawns per sorted particle
+  uint32_t *emit_incl;     // their inclusive scan
+  float4 *pot;             // trapped air, crest, divergence, neighbours
+  unsigned long long *tile_status;  // scan of emit_n
+  uint32_t *tile_counter;
+};
+
+// Whitewater passes (spray.hip): emit = potentials, slot scan and spawns
+// over the first n sorted particles; simulate = every ring slot.
+hipError_t launch_spray_emit(const Uni &u, const Bufs &b, const SprayBufs &sb, uint32_t n,
+                             const GpuSprayParams &p, hipStream_t st);
+hipError_t launch_spray_simulate(const Uni &u, const Bufs &b, const SprayBufs &sb,
+                                 const GpuSprayParams &p, hipStream_t st);
 
 // Host-side parameter helpers (params.cpp).
 void fill_uniforms(DevUniforms &u, const GpuSphParams3D &p,

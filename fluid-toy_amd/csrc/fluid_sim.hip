@@ -10,6 +10,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <memory>
 #include <string>
 #include <vector>
 
@@ -98,6 +99,7 @@ struct fluid_sim {
   bool tuned = false;  // cfg chosen by fluid_set_sweep_lanes / fluid_autotune
   uint32_t ntiles = 0;
   RolloutState roll;
+  uint32_t sprays = 0;  // spray handles bound to this simulation
 
   Uni uni() const { return Uni{d_u, d_scene_of}; }
   bool batched() const { return sc.size() > 1; }
@@ -815,6 +817,7 @@ int fluid_batch_update_scene(fluid_sim *s, uint32_t scene, const GpuSphParams3D 
 
 int fluid_destroy(fluid_sim *s) {
   if (!s) return FLUID_OK;
+  if (s->sprays) return fail(FLUID_ERR_STATE, "destroy the simulation's spray handles first");
   (void)hipSetDevice(s->device);
   if (s->stream) (void)hipStreamSynchronize(s->stream);
   drop_graph(s);
@@ -1561,6 +1564,266 @@ int fluid_rollout_timing(const fluid_sim *s, float ms[3]) {
   if (!s || !ms) return fail(FLUID_ERR_INVALID_ARG, "null argument");
   for (int k = 0; k < 3; ++k) ms[k] = s->roll.ms[k];
   return FLUID_OK;
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------------------
+// Whitewater (SpraySystem, src/simulation/spray.rs).  The handle borrows its
+// simulation's sorted particles, grid, uniforms and stream; its own device
+// memory is the ring, the per-particle emit outputs with their scan, and the
+// SprayDev state block.
+struct fluid_spray {
+  fluid_sim *sim = nullptr;
+  GpuSprayParams params{};
+  SprayBufs sb{};
+  uint32_t last_n = 0;  // particles of the last emit pass
+  std::vector<void *> allocs;
+};
+
+namespace {
+
+constexpr float kAutoTaInit = 3.0f;  // spray.rs:30
+constexpr float kAutoWcInit = 1.0f;  // spray.rs:31
+
+int spray_check(fluid_spray *sp) {
+  if (!sp || !sp->sim) return fail(FLUID_ERR_INVALID_ARG, "null spray handle");
+  if (sp->sim->cursor != -1)
+    return fail(FLUID_ERR_STATE, "a partial step of the simulation is in progress");
+  HIP_TRY(hipSetDevice(sp->sim->device));
+  return FLUID_OK;
+}
+
+int spray_params_ok(const fluid_spray *sp, const GpuSprayParams *p) {
+  if (!p) return fail(FLUID_ERR_INVALID_ARG, "null spray params");
+  if (p->max_particles != sp->sb.cap)
+    return fail(FLUID_ERR_INVALID_ARG, "params.max_particles differs from the ring size");
+  return FLUID_OK;
+}
+
+// Initial state: head 0, no pending spawns, the ceilings' initial guesses.
+hipError_t spray_init_state(fluid_spray *sp) {
+  SprayDev d{};
+  d.ta_limit = kAutoTaInit;
+  d.wc_limit = kAutoWcInit;
+  hipError_t e = hipMemcpyAsync(sp->sb.d, &d, sizeof d, hipMemcpyHostToDevice,
+                                sp->sim->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(sp->sim->stream);
+  return e;
+}
+
+uint32_t spray_emitters(const fluid_spray *sp, uint32_t num_sph) {
+  return std::min(std::min(num_sph, sp->params.num_sph_particles), sp->sim->n);
+}
+
+int spray_emit(fluid_spray *sp, uint32_t num_sph) {
+  fluid_sim *s = sp->sim;
+  const uint32_t n = spray_emitters(sp, num_sph);
+  HIP_TRY(launch_spray_emit(s->uni(), s->b, sp->sb, n, sp->params, s->stream));
+  sp->last_n = n;
+  return FLUID_OK;
+}
+
+int spray_simulate(fluid_spray *sp) {
+  fluid_sim *s = sp->sim;
+  HIP_TRY(launch_spray_simulate(s->uni(), s->b, sp->sb, sp->params, s->stream));
+  return FLUID_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int fluid_spray_create(fluid_sim *s, uint32_t max_spray, const GpuSprayParams *params,
+                       fluid_spray **out) {
+  if (!s || !out) return fail(FLUID_ERR_INVALID_ARG, "null argument");
+  *out = nullptr;
+  if (s->batched()) return fail(FLUID_ERR_INVALID_ARG, "spray needs a plain (not batched) simulation");
+  if (max_spray == 0) return fail(FLUID_ERR_INVALID_ARG, "max_spray must be positive");
+  auto sp = std::make_unique<fluid_spray>();
+  sp->sim = s;
+  sp->sb.cap = max_spray;
+  int rc = spray_params_ok(sp.get(), params);
+  if (rc) return rc;
+  sp->params = *params;
+  HIP_TRY(hipSetDevice(s->device));
+  auto alloc = [&](size_t bytes, hipError_t *e) {
+    void *p = nullptr;
+    *e = hipMalloc(&p, bytes ? bytes : 16);
+    if (*e == hipSuccess) {
+      sp->allocs.push_back(p);
+      *e = hipMemset(p, 0, bytes ? bytes : 16);
+    }
+    return p;
+  };
+  const size_t nmax = std::max<uint32_t>(s->max_n, 1);
+  hipError_t e = hipSuccess;
+  sp->sb.d = (SprayDev *)alloc(sizeof(SprayDev), &e);
+  if (e == hipSuccess) sp->sb.ring = (float4 *)alloc((size_t)max_spray * sizeof(GpuSprayParticle), &e);
+  if (e == hipSuccess) sp->sb.emit_n = (uint32_t *)alloc(nmax * 4, &e);
+  if (e == hipSuccess) sp->sb.emit_incl = (uint32_t *)alloc(nmax * 4, &e);
+  if (e == hipSuccess) sp->sb.pot = (float4 *)alloc(nmax * sizeof(float4), &e);
+  if (e == hipSuccess)
+    sp->sb.tile_status =
+        (unsigned long long *)alloc((nmax / kScanTile + 2) * sizeof(unsigned long long), &e);
+  if (e == hipSuccess) sp->sb.tile_counter = (uint32_t *)alloc(4, &e);
+  if (e == hipSuccess) e = spray_init_state(sp.get());
+  if (e != hipSuccess) {
+    for (void *p : sp->allocs) (void)hipFree(p);
+    return fail(FLUID_ERR_HIP, std::string("spray allocation: ") + hipGetErrorString(e));
+  }
+  ++s->sprays;
+  *out = sp.release();
+  return FLUID_OK;
+}
+
+int fluid_spray_destroy(fluid_spray *sp) {
+  if (!sp) return FLUID_OK;
+  if (sp->sim) {
+    (void)hipSetDevice(sp->sim->device);
+    (void)hipStreamSynchronize(sp->sim->stream);
+    --sp->sim->sprays;
+  }
+  for (void *p : sp->allocs) (void)hipFree(p);
+  delete sp;
+  return FLUID_OK;
+}
+
+int fluid_spray_update_params(fluid_spray *sp, const GpuSprayParams *params) {
+  if (!sp) return fail(FLUID_ERR_INVALID_ARG, "null spray handle");
+  int rc = spray_params_ok(sp, params);
+  if (rc) return rc;
+  sp->params = *params;  // kernel arguments: every later launch sees them
+  return FLUID_OK;
+}
+
+int fluid_spray_emit(fluid_spray *sp, uint32_t num_sph_particles) {
+  int rc = spray_check(sp);
+  if (rc) return rc;
+  return spray_emit(sp, num_sph_particles);
+}
+
+int fluid_spray_simulate(fluid_spray *sp) {
+  int rc = spray_check(sp);
+  if (rc) return rc;
+  return spray_simulate(sp);
+}
+
+int fluid_spray_step(fluid_spray *sp, uint32_t num_sph_particles) {
+  int rc = spray_check(sp);
+  if (rc) return rc;
+  rc = spray_emit(sp, num_sph_particles);
+  if (rc) return rc;
+  return spray_simulate(sp);
+}
+
+int fluid_spray_read_state(fluid_spray *sp, fluid_spray_state *out) {
+  int rc = spray_check(sp);
+  if (rc) return rc;
+  if (!out) return fail(FLUID_ERR_INVALID_ARG, "null argument");
+  SprayDev d{};
+  HIP_TRY(hipMemcpyAsync(&d, sp->sb.d, sizeof d, hipMemcpyDeviceToHost, sp->sim->stream));
+  HIP_TRY(hipStreamSynchronize(sp->sim->stream));
+  out->write_head = d.head + d.emitted;
+  out->ta_limit = d.ta_limit;
+  out->wc_limit = d.wc_limit;
+  for (int k = 0; k < 4; ++k) out->live[k] = d.live[k];
+  out->emitted = d.emitted;
+  return check_errors(sp->sim);
+}
+
+int fluid_spray_auto_limits(fluid_spray *sp, float out[2]) {
+  if (!out) return fail(FLUID_ERR_INVALID_ARG, "null argument");
+  fluid_spray_state st{};
+  int rc = fluid_spray_read_state(sp, &st);
+  if (rc) return rc;
+  out[0] = st.ta_limit;
+  out[1] = st.wc_limit;
+  return FLUID_OK;
+}
+
+int fluid_spray_read_stats(fluid_spray *sp, uint32_t out[4]) {
+  if (!out) return fail(FLUID_ERR_INVALID_ARG, "null argument");
+  fluid_spray_state st{};
+  int rc = fluid_spray_read_state(sp, &st);
+  if (rc) return rc;
+  for (int k = 0; k < 4; ++k) out[k] = st.live[k];
+  return FLUID_OK;
+}
+
+int fluid_spray_read_particles(fluid_spray *sp, GpuSprayParticle *out, uint32_t count) {
+  int rc = spray_check(sp);
+  if (rc) return rc;
+  if (!out) return fail(FLUID_ERR_INVALID_ARG, "null argument");
+  if (count < sp->sb.cap) return fail(FLUID_ERR_CAPACITY, "output smaller than the ring");
+  HIP_TRY(hipMemcpyAsync(out, sp->sb.ring, (size_t)sp->sb.cap * sizeof(GpuSprayParticle),
+                         hipMemcpyDeviceToHost, sp->sim->stream));
+  HIP_TRY(hipStreamSynchronize(sp->sim->stream));
+  return check_errors(sp->sim);
+}
+
+int fluid_spray_write_particles(fluid_spray *sp, const GpuSprayParticle *in, uint32_t count) {
+  int rc = spray_check(sp);
+  if (rc) return rc;
+  if (!in) return fail(FLUID_ERR_INVALID_ARG, "null argument");
+  if (count != sp->sb.cap) return fail(FLUID_ERR_INVALID_ARG, "count must equal the ring size");
+  HIP_TRY(hipMemcpyAsync(sp->sb.ring, in, (size_t)sp->sb.cap * sizeof(GpuSprayParticle),
+                         hipMemcpyHostToDevice, sp->sim->stream));
+  HIP_TRY(hipStreamSynchronize(sp->sim->stream));
+  return FLUID_OK;
+}
+
+int fluid_spray_read_emit(fluid_spray *sp, float *potentials, uint32_t *spawns,
+                          uint32_t count) {
+  int rc = spray_check(sp);
+  if (rc) return rc;
+  const uint32_t n = sp->last_n;
+  if ((!potentials && !spawns) || count < n)
+    return fail(count < n ? FLUID_ERR_CAPACITY : FLUID_ERR_INVALID_ARG,
+                "outputs must hold the last emit pass's particles");
+  hipStream_t st = sp->sim->stream;
+  if (potentials && n)
+    HIP_TRY(hipMemcpyAsync(potentials, sp->sb.pot, (size_t)n * sizeof(float4),
+                           hipMemcpyDeviceToHost, st));
+  if (spawns && n)
+    HIP_TRY(hipMemcpyAsync(spawns, sp->sb.emit_n, (size_t)n * 4, hipMemcpyDeviceToHost, st));
+  HIP_TRY(hipStreamSynchronize(st));
+  return check_errors(sp->sim);
+}
+
+int fluid_spray_reset(fluid_spray *sp) {
+  int rc = spray_check(sp);
+  if (rc) return rc;
+  HIP_TRY(hipMemsetAsync(sp->sb.ring, 0, (size_t)sp->sb.cap * sizeof(GpuSprayParticle),
+                         sp->sim->stream));
+  HIP_TRY(spray_init_state(sp));
+  return FLUID_OK;
+}
+
+int fluid_spray_time_steps(fluid_spray *sp, uint32_t num_sph_particles, uint32_t warmup,
+                           uint32_t frames, float *ms) {
+  int rc = spray_check(sp);
+  if (rc) return rc;
+  if (!ms) return fail(FLUID_ERR_INVALID_ARG, "null argument");
+  for (uint32_t k = 0; k < warmup; ++k) {
+    if ((rc = spray_emit(sp, num_sph_particles)) || (rc = spray_simulate(sp))) return rc;
+  }
+  hipStream_t st = sp->sim->stream;
+  hipEvent_t e0, e1;
+  HIP_TRY(hipEventCreate(&e0));
+  HIP_TRY(hipEventCreate(&e1));
+  HIP_TRY(hipStreamSynchronize(st));
+  HIP_TRY(hipEventRecord(e0, st));
+  for (uint32_t k = 0; k < frames; ++k) {
+    if ((rc = spray_emit(sp, num_sph_particles)) || (rc = spray_simulate(sp))) break;
+  }
+  HIP_TRY(hipEventRecord(e1, st));
+  HIP_TRY(hipEventSynchronize(e1));
+  HIP_TRY(hipEventElapsedTime(ms, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (rc) return rc;
+  return check_errors(sp->sim);
 }
 
 }  // extern "C"

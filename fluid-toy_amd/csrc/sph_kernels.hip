@@ -403,72 +403,6 @@ __global__ __launch_bounds__(256) void k_density_t(Uni un, Bufs b, uint32_t n) {
   }
 }
 
-// Walk this lane's share of particle s's in-range list, or, if the list
-// overflowed, the lane's candidate subset again with the same test; f(j, p_j)
-// sees every pair once.  The list is written for LW = 8 lanes; with L = 16 or
-// 32 walkers, M = L / 8 of them share each list lane, taking interleaved
-// depths.  Full chunks of U pairs per trip, the next chunk's list entries
-// loaded a trip ahead so only the record gathers wait on memory.
-template <int L, int U, class F>
-__device__ __forceinline__ void for_each_neighbour(const DevUniforms *__restrict__ Un,
-                                                   const Bufs &b, uint32_t n, uint32_t s,
-                                                   uint32_t sub, float4 pi, F &&f) {
-  static_assert(L % kListLanes == 0, "walkers are a multiple of the list lanes");
-  constexpr uint32_t LW = kListLanes, M = L / LW;
-  const uint32_t total = b.ncnt[s];
-  const float4 *__restrict__ srec = b.srec;
-  if (total <= kListCapacity) {
-    const uint32_t kl = sub % LW, d0 = sub / LW;
-    const uint32_t nd = total > kl ? (total - 1 - kl) / LW + 1 : 0;  // depths of list lane kl
-    const uint32_t cnt = nd > d0 ? (nd - 1 - d0) / M + 1 : 0;
-    const uint32_t *__restrict__ nbr =
-        b.nbr + lane_block<LW>(s, kListCapacity) + d0 * 64 + kl;
-    constexpr uint32_t kStep = 64 * M;
-    uint32_t k = 0;
-    if (cnt >= U) {
-      uint32_t jn[U];
-#pragma unroll
-      for (int q = 0; q < U; ++q) jn[q] = nbr[q * kStep];
-      nbr += U * kStep;
-      k = U;
-      while (true) {
-        uint32_t jc[U];
-#pragma unroll
-        for (int q = 0; q < U; ++q) jc[q] = jn[q];
-        const bool more = k + U <= cnt;
-        if (more) {
-#pragma unroll
-          for (int q = 0; q < U; ++q) jn[q] = nbr[q * kStep];
-          nbr += U * kStep;
-          k += U;
-        }
-        float4 p[U];
-#pragma unroll
-        for (int q = 0; q < U; ++q) p[q] = srec[3 * (size_t)jc[q]];
-#pragma unroll
-        for (int q = 0; q < U; ++q) f(jc[q], p[q]);
-        if (!more) break;
-      }
-    }
-    for (; k < cnt; ++k) {
-      const uint32_t j0 = nbr[0];
-      nbr += kStep;
-      f(j0, srec[3 * (size_t)j0]);
-    }
-    return;
-  }
-  const float h2 = Un->h2;
-  const int cx = cell_coord(pi.x, Un->ox, Un->inv_cell);
-  const int cy = cell_coord(pi.y, Un->oy, Un->inv_cell);
-  const int cz = cell_coord(pi.z, Un->oz, Un->inv_cell);
-  sweep27<L, false>(Un, b.counts, b.starts, b.spos, cx, cy, cz, sub,
-                    [&](uint32_t j, float4 pj, bool) {
-    const float rx = pi.x - pj.x, ry = pi.y - pj.y, rz = pi.z - pj.z;
-    const float r2 = dot3(rx, ry, rz, rx, ry, rz);
-    if (r2 < h2 && r2 > 1e-12f) f(j, srec[3 * (size_t)j]);
-  });
-}
-
 // ------------------------------------------------------------------------
 // Stage 5 — xsph.wgsl:81-138: the particle's XSPH-corrected velocity w_i
 // (in every lane of its group).  With epsilon <= 0 the reference leaves the

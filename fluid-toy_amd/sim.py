@@ -82,6 +82,8 @@ class SphSimulation3DGrid:
 
     # -- lifetime ---------------------------------------------------------
     def close(self) -> None:
+        for sp in list(getattr(self, "_sprays", ())):
+            sp.close()
         if getattr(self, "_h", None):
             self._lib.fluid_destroy(self._h)
             self._h = None
@@ -378,3 +380,110 @@ class SphBatch(SphSimulation3DGrid):
     def read_cell_starts(self) -> np.ndarray:
         total = sum(self.scene_info(k)["cells"] for k in range(self.num_scenes()))
         return self._read_u32(self._lib.fluid_read_cell_starts, total)
+
+
+class SpraySystem:
+    """Mirror of the reference's SpraySystem (src/simulation/spray.rs:82-491)
+    over fluid_spray_*: whitewater particles emitted from and advected by the
+    simulation's sorted particles of its last step.  Ring readbacks are
+    (max, 12) uint32 arrays of the GpuSprayParticle words (abi.SPRAY_*); use
+    `.view(np.float32)` for the float fields."""
+
+    def __init__(self, sim: SphSimulation3DGrid, max_spray: int,
+                 params: abi.GpuSprayParams | None = None):
+        self._lib = sim._lib
+        self._sim = sim
+        self.max_particles = int(max_spray)
+        if params is None:
+            params = abi.default_spray_params(dt=1.0 / 60.0, max_particles=max_spray,
+                                              num_sph_particles=sim.num_particles())
+        self.params = params
+        handle = C.c_void_p()
+        abi.check(self._lib, self._lib.fluid_spray_create(sim._h, max_spray, C.byref(params),
+                                                          C.byref(handle)))
+        self._h = handle
+        if not hasattr(sim, "_sprays"):
+            sim._sprays = []
+        sim._sprays.append(self)
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            self._lib.fluid_spray_destroy(self._h)
+            self._h = None
+            if self in getattr(self._sim, "_sprays", []):
+                self._sim._sprays.remove(self)
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _chk(self, rc: int) -> None:
+        abi.check(self._lib, rc)
+
+    def update_params(self, params: abi.GpuSprayParams) -> None:
+        """update_params (spray.rs:345-347)."""
+        self._chk(self._lib.fluid_spray_update_params(self._h, C.byref(params)))
+        self.params = params
+
+    def step(self, num_sph_particles: int | None = None) -> None:
+        """step (spray.rs:349-428): emit, then simulate."""
+        n = self._sim.num_particles() if num_sph_particles is None else num_sph_particles
+        self._chk(self._lib.fluid_spray_step(self._h, n))
+
+    def emit(self, num_sph_particles: int | None = None) -> None:
+        n = self._sim.num_particles() if num_sph_particles is None else num_sph_particles
+        self._chk(self._lib.fluid_spray_emit(self._h, n))
+
+    def simulate(self) -> None:
+        self._chk(self._lib.fluid_spray_simulate(self._h))
+
+    def auto_limits(self) -> tuple[float, float]:
+        """auto_limits (spray.rs:458-460): (trapped air, wave crest)."""
+        out = (C.c_float * 2)()
+        self._chk(self._lib.fluid_spray_auto_limits(self._h, out))
+        return float(out[0]), float(out[1])
+
+    def read_stats(self) -> np.ndarray:
+        """read_stats (spray.rs:464-475): total, spray, foam, bubble."""
+        out = (C.c_uint32 * 4)()
+        self._chk(self._lib.fluid_spray_read_stats(self._h, out))
+        return np.array(out, dtype=np.uint32)
+
+    def state(self) -> dict:
+        st = abi.FluidSprayState()
+        self._chk(self._lib.fluid_spray_read_state(self._h, C.byref(st)))
+        return {"write_head": st.write_head, "ta_limit": st.ta_limit, "wc_limit": st.wc_limit,
+                "live": list(st.live), "emitted": st.emitted}
+
+    def read_particles(self) -> np.ndarray:
+        """spray_buffer (spray.rs:477-479) as a (max, 12) uint32 array."""
+        out = np.zeros((self.max_particles, 12), np.uint32)
+        self._chk(self._lib.fluid_spray_read_particles(self._h, out.ctypes.data,
+                                                       self.max_particles))
+        return out
+
+    def write_particles(self, ring: np.ndarray) -> None:
+        ring = np.ascontiguousarray(ring, dtype=np.uint32).reshape(self.max_particles, 12)
+        self._chk(self._lib.fluid_spray_write_particles(self._h, ring.ctypes.data,
+                                                        self.max_particles))
+
+    def read_emit(self, n: int) -> tuple[np.ndarray, np.ndarray]:
+        """The last emit pass: potentials (n, 4) float32 (trapped air, crest,
+        divergence, neighbours) and spawns (n,) uint32."""
+        pot = np.zeros((n, 4), np.float32)
+        ne = np.zeros(n, np.uint32)
+        self._chk(self._lib.fluid_spray_read_emit(self._h, pot.ctypes.data, ne.ctypes.data, n))
+        return pot, ne
+
+    def reset(self) -> None:
+        """reset (spray.rs:482-490)."""
+        self._chk(self._lib.fluid_spray_reset(self._h))
+
+    def time_steps(self, warmup: int, frames: int, num_sph_particles: int | None = None) -> float:
+        n = self._sim.num_particles() if num_sph_particles is None else num_sph_particles
+        ms = C.c_float(0.0)
+        self._chk(self._lib.fluid_spray_time_steps(self._h, n, warmup, frames, C.byref(ms)))
+        return ms.value
+

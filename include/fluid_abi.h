@@ -22,7 +22,7 @@
  *  - Plain pointers and sizes only; no C++ or torch types cross the boundary.
  *
  * Every POD struct below is byte-identical to its Rust/WGSL counterpart
- * (static_asserts in fluid-toy_amd/csrc/fluid_types.h).
+ * (static_asserts in fluid-toy_amd/csrc/fluid_internal.h).
  */
 #ifndef FLUID_ABI_H
 #define FLUID_ABI_H
@@ -460,6 +460,114 @@ int fluid_backward_rollout(fluid_sim *sim, const float *dl_dpos,
 /* Timing of the last fluid_backward_rollout on the handle's stream (HIP
  * events): ms[0] replays, ms[1] reverse kernels, ms[2] whole call. */
 int fluid_rollout_timing(const fluid_sim *sim, float ms[3]);
+
+/* ---- Whitewater: spray, foam and bubble particles ---------------------
+ * The reference's SpraySystem (src/simulation/spray.rs:82-491), the step's
+ * first consumer: an emit pass over the grid-sorted particles of the last
+ * step (spray_emit.wgsl) and a simulate pass over a ring of diffuse
+ * particles (spray_simulate.wgsl), both reading the simulation's sorted
+ * particles and grid.  A spray handle is bound to one plain (not batched)
+ * simulation handle, shares its stream, and must be destroyed before it.
+ *
+ * Differences from the reference, both removing races:
+ *  - spawn slots are handed out in sorted-particle order (particle i's
+ *    spawns follow those of particles < i), the order of one sequential
+ *    run of the reference's racing atomicAdd (spray_emit.wgsl:335); when
+ *    one frame spawns more than the ring holds, the later spawns win;
+ *  - the potential ceilings' auto-calibration (spray.rs:40-51, 431-455) runs
+ *    on the device after every emit pass, so frame f's emit sees the maxima
+ *    of every frame before f (the reference harvests them asynchronously,
+ *    one or more frames late, and skips frames while a readback is in
+ *    flight).  The ta_limit / wc_limit fields of GpuSprayParams are
+ *    therefore not read: the app fills them from auto_limits() anyway
+ *    (app.rs:1051-1057). */
+
+/* src/state/interaction.rs:131-153 (80 B) */
+typedef struct GpuSprayParams {
+  float min_speed;
+  float emission_rate;
+  float lifetime;
+  float lifetime_variation;
+  float drag;
+  float speed_multiplier;
+  float velocity_jitter;
+  float dt;
+  uint32_t max_particles;
+  uint32_t num_sph_particles;
+  uint32_t frame_count;
+  float gravity_y;
+  float k_trapped_air;
+  float k_wave_crest;
+  float ta_limit;
+  float bubble_buoyancy;
+  float bubble_drag;
+  float wc_limit;
+  float _pad[2];
+} GpuSprayParams;
+
+/* src/state/interaction.rs:160-174 (48 B); kind 0 spray, 1 foam, 2 bubble */
+typedef struct GpuSprayParticle {
+  float pos_x, pos_y, pos_z;
+  float lifetime;
+  float vel_x, vel_y, vel_z;
+  float max_lifetime;
+  uint32_t kind;
+  float age;
+  float _pad[2];
+} GpuSprayParticle;
+
+/* Device-resident state of a spray handle. */
+typedef struct fluid_spray_state {
+  uint32_t write_head;  /* spawns so far (mod 2^32); slot = head % max    */
+  float ta_limit;       /* auto-calibrated trapped-air ceiling            */
+  float wc_limit;       /* auto-calibrated wave-crest ceiling             */
+  uint32_t live[4];     /* last simulate pass: total, spray, foam, bubble */
+  uint32_t emitted;     /* spawns of the last emit pass                   */
+} fluid_spray_state;
+
+typedef struct fluid_spray fluid_spray;
+
+/* SpraySystem::new (spray.rs:105-343): a zeroed (all dead) ring of
+ * max_spray particles, write head 0, ceilings at their initial guesses
+ * (AUTO_TA_INIT, AUTO_WC_INIT, spray.rs:30-31).  params->max_particles must
+ * equal max_spray. */
+int fluid_spray_create(fluid_sim *sim, uint32_t max_spray,
+                       const GpuSprayParams *params, fluid_spray **out);
+int fluid_spray_destroy(fluid_spray *spray);
+/* update_params (spray.rs:345-347) */
+int fluid_spray_update_params(fluid_spray *spray, const GpuSprayParams *params);
+/* step (spray.rs:349-428): emit over the first
+ * min(num_sph_particles, params.num_sph_particles, particles of the
+ * simulation) sorted particles, then simulate every ring slot. */
+int fluid_spray_step(fluid_spray *spray, uint32_t num_sph_particles);
+/* The two passes of a step on their own (spray.rs:368-399): emit (with the
+ * slot assignment and spawn writes) and simulate (with the ceilings'
+ * update from the emit pass's maxima). */
+int fluid_spray_emit(fluid_spray *spray, uint32_t num_sph_particles);
+int fluid_spray_simulate(fluid_spray *spray);
+/* auto_limits (spray.rs:458-460): out = {trapped air, wave crest} */
+int fluid_spray_auto_limits(fluid_spray *spray, float out[2]);
+/* read_stats (spray.rs:464-475): live counts of the last simulate pass */
+int fluid_spray_read_stats(fluid_spray *spray, uint32_t out[4]);
+int fluid_spray_read_state(fluid_spray *spray, fluid_spray_state *out);
+/* spray_buffer (spray.rs:477-479) as a readback; the write restores a ring
+ * (count must be the ring size). */
+int fluid_spray_read_particles(fluid_spray *spray, GpuSprayParticle *out,
+                               uint32_t count);
+int fluid_spray_write_particles(fluid_spray *spray, const GpuSprayParticle *in,
+                                uint32_t count);
+/* The last emit pass per sorted particle: potentials[4 * i ..] = trapped
+ * air, wave crest (after its velocity gate), divergence, in-range neighbour
+ * count; spawns[i] = particles spawned.  Zero for particles below
+ * min_speed. */
+int fluid_spray_read_emit(fluid_spray *spray, float *potentials,
+                          uint32_t *spawns, uint32_t count);
+/* reset (spray.rs:482-490) */
+int fluid_spray_reset(fluid_spray *spray);
+/* Wall time of `frames` spray steps after `warmup` untimed ones (HIP
+ * events on the stream), over the simulation's current sorted state. */
+int fluid_spray_time_steps(fluid_spray *spray, uint32_t num_sph_particles,
+                           uint32_t warmup, uint32_t frames, float *ms);
 
 #ifdef __cplusplus
 }

@@ -20,6 +20,10 @@
 // FP64 variant promotes their values.
 //
 // Parity status of each function is recorded next to it ("pinned by ...").
+// The file also restates the whitewater passes that consume the step's sorted
+// state (spray_emit.wgsl, spray_simulate.wgsl, src/simulation/spray.rs), in
+// f32 only.
+//
 // What only running the real wgpu pipeline could pin — naga/driver rounding
 // of the sums, sin/cos of the close-range fallback direction, the racing
 // reorder order — is not pinned here (no Rust/Vulkan toolchain exists here).
@@ -823,9 +827,339 @@ float powi_f32(float x, int n) {
   return res;
 }
 
+// ---------------------------------------------------------------------------
+// Whitewater: spray_emit.wgsl and spray_simulate.wgsl over a sorted particle
+// array and its grid (f32 only), plus the ceilings' update of spray.rs:40-51.
+// Slots are handed out in sorted order, one spawn at a time: one sequential
+// run of the reference's atomicAdd (spray_emit.wgsl:335).
+namespace spray {
+
+constexpr float kTwoPi = 6.28318530718f;  // spray_emit.wgsl:114-154
+constexpr float kAutoLimitMinFrac = 0.1f;
+constexpr uint32_t kEmitMinNeighbors = 10u;
+constexpr float kKineticMaxSpeed = 2.5f;
+constexpr float kCrestDensityFrac = 0.9f;
+constexpr float kCrestVelNormalMin = 0.6f;
+constexpr float kCrestDivergenceMin = 0.0f;
+constexpr float kCrestDivergenceMax = 0.5f;
+constexpr uint32_t kMaxEmitPerFrame = 16u;
+constexpr uint32_t kSprayMaxNeighbors = 6u;  // spray_simulate.wgsl:123-135
+constexpr uint32_t kBubbleMinNeighbors = 20u;
+constexpr float kNewbornSprayTime = 0.15f;
+constexpr float kPi = 3.14159265359f;
+
+// spray_emit.wgsl:156-164
+uint32_t hash(uint32_t seed) {
+  uint32_t x = seed;
+  x = x ^ (x >> 16u);
+  x = x * 0x45d9f3bu;
+  x = x ^ (x >> 16u);
+  x = x * 0x45d9f3bu;
+  x = x ^ (x >> 16u);
+  return x;
+}
+
+// :166-168
+float hash_float(uint32_t seed) { return (float)(hash(seed) & 0xFFFFu) / 65535.0f; }
+
+// :185-187
+float remap_clamp(float x, float lo, float hi) {
+  return clampv((x - lo) / (hi - lo), 0.0f, 1.0f);
+}
+
+// WGSL u32(f32): saturating
+uint32_t to_u32(float v) {
+  if (!(v > 0.0f)) return 0u;
+  if (v >= 4294967296.0f) return 0xFFFFFFFFu;
+  return (uint32_t)v;
+}
+
+struct Grid {
+  const GpuGridParams &g;
+  // position_to_cell (spray_emit.wgsl:171-174)
+  void cell(V3<float> pos, int32_t c[3]) const {
+    c[0] = to_i32(std::floor((pos.x - g.grid_origin_x) * g.inv_cell_size));
+    c[1] = to_i32(std::floor((pos.y - g.grid_origin_y) * g.inv_cell_size));
+    c[2] = to_i32(std::floor((pos.z - g.grid_origin_z) * g.inv_cell_size));
+  }
+  // is_valid_cell (:180-183)
+  bool valid(const int32_t c[3]) const {
+    return c[0] >= 0 && c[0] < (int32_t)g.grid_size_x && c[1] >= 0 &&
+           c[1] < (int32_t)g.grid_size_y && c[2] >= 0 && c[2] < (int32_t)g.grid_size_z;
+  }
+  // cell_to_index (:176-178), u32 arithmetic
+  uint32_t index(const int32_t c[3]) const {
+    return (uint32_t)c[0] + (uint32_t)c[1] * g.grid_size_x +
+           (uint32_t)c[2] * g.grid_size_x * g.grid_size_y;
+  }
+  // the 27-cell loop (dz, dy, dx, then the cell's particles in order)
+  template <class F>
+  void visit(V3<float> pos, const uint32_t *counts, const uint32_t *starts, F &&f) const {
+    int32_t ci[3];
+    cell(pos, ci);
+    for (int dz = -1; dz <= 1; ++dz)
+      for (int dy = -1; dy <= 1; ++dy)
+        for (int dx = -1; dx <= 1; ++dx) {
+          const int32_t nc[3] = {ci[0] + dx, ci[1] + dy, ci[2] + dz};
+          if (!valid(nc)) continue;
+          const uint32_t idx = index(nc);
+          const uint32_t count = counts[idx];
+          const uint32_t start = starts[idx] - count;
+          for (uint32_t k = 0; k < count; ++k) f(start + k);
+        }
+  }
+};
+
+// world_to_local (container_common.wgsl:28-35)
+V3<float> world_to_local(const GpuContainerGeometry &c, V3<float> pos) {
+  const V3<float> p = mk(pos.x, pos.y - c.center_y, pos.z);
+  return mk(dot(P3(c.inverse_row0), p), dot(P3(c.inverse_row1), p), dot(P3(c.inverse_row2), p));
+}
+
+// spray_simulate.wgsl:142-145
+float cap_volume_fraction(float t) {
+  const float tc = clampv(t, 0.0f, 1.0f);
+  return tc * tc * (3.0f - tc) * 0.25f;
+}
+
+}  // namespace spray
+
 }  // namespace
 
 extern "C" {
+
+// spray_emit.wgsl:190-320 for sorted particles 0..n-1: potentials[4i..] =
+// trapped air, crest (after its velocity gate), divergence, neighbour count
+// (zeros below min_speed); n_emit[i] = spawns; frame_max = the maxima the
+// pass reports (f32, 0 when none).  ta_limit / wc_limit are the ceilings.
+void oracle_spray_potentials(const void *sorted_particles, uint32_t n, const uint32_t *counts,
+                             const uint32_t *starts, const GpuGridParams *grid,
+                             const GpuSphParams3D *sph, const GpuSprayParams *prm,
+                             float ta_limit, float wc_limit, float *potentials,
+                             uint32_t *n_emit, float *frame_max) {
+  using namespace spray;
+  const Part<float> *sp = (const Part<float> *)sorted_particles;
+  const Grid G{*grid};
+  frame_max[0] = frame_max[1] = 0.0f;
+  const float h = sph->kernel_radius, h_sq = sph->kernel_radius_sq;
+  const float poly6_norm = 315.0f / (64.0f * kPi * sph->kernel_radius_pow9);
+  for (uint32_t i = 0; i < n; ++i) {
+    float *pot = potentials + 4 * (size_t)i;
+    pot[0] = pot[1] = pot[2] = pot[3] = 0.0f;
+    n_emit[i] = 0u;
+    const Part<float> &p = sp[i];
+    const V3<float> pos = P3(p.pos), vel = P3(p.vel);
+    const float speed = length(vel);
+    if (speed < prm->min_speed) continue;
+    const V3<float> normal_i = P3(p.normal);
+    const float n_len_i = length(normal_i);
+    const bool crest_eligible =
+        n_len_i > 1e-4f && p.density < sph->rest_density * kCrestDensityFrac;
+    V3<float> n_hat_i = mk(0.0f, 1.0f, 0.0f);
+    if (crest_eligible) n_hat_i = divs(normal_i, n_len_i);
+    float trapped_air = 0.0f, crest = 0.0f, divergence = 0.0f;
+    uint32_t n_neighbors = 0u;
+    G.visit(pos, counts, starts, [&](uint32_t j) {
+      const Part<float> &pj = sp[j];
+      const V3<float> r_vec = sub(pos, P3(pj.pos));
+      const float r_sq = dot(r_vec, r_vec);
+      if (r_sq >= h_sq || r_sq < 1e-12f) return;
+      const float r = std::sqrt(r_sq);
+      const float w = 1.0f - r / h;
+      const V3<float> x_hat = divs(r_vec, r);
+      n_neighbors += 1u;
+      const float d2 = h_sq - r_sq;
+      const float w_vol = (sph->mass / std::max(pj.density, 1e-6f)) * poly6_norm * d2 * d2 * d2;
+      const V3<float> v_ij = sub(vel, P3(pj.vel));
+      const float v_ij_len = length(v_ij);
+      if (v_ij_len > 1e-6f)
+        trapped_air += v_ij_len * (1.0f - dot(divs(v_ij, v_ij_len), x_hat)) * w_vol;
+      divergence += dot(v_ij, x_hat) * w;
+      if (crest_eligible && dot(mk(-x_hat.x, -x_hat.y, -x_hat.z), n_hat_i) < 0.0f) {
+        const V3<float> normal_j = P3(pj.normal);
+        const float n_len_j = length(normal_j);
+        if (n_len_j > 1e-4f)
+          crest += (1.0f - dot(n_hat_i, divs(normal_j, n_len_j))) * w_vol;
+      }
+    });
+    pot[0] = trapped_air;
+    pot[1] = crest;
+    pot[2] = divergence;
+    pot[3] = (float)n_neighbors;
+    if (n_neighbors < kEmitMinNeighbors) continue;
+    if (dot(divs(vel, speed), n_hat_i) < kCrestVelNormalMin) crest = 0.0f;
+    pot[1] = crest;
+    if (trapped_air > 0.0f) frame_max[0] = std::max(frame_max[0], trapped_air);
+    if (crest > 0.0f) frame_max[1] = std::max(frame_max[1], crest);
+    const float phi_div = remap_clamp(divergence, kCrestDivergenceMin, kCrestDivergenceMax);
+    const float ta_hi = std::max(ta_limit, 1e-4f);
+    const float wc_hi = std::max(wc_limit, 1e-4f);
+    const float phi_ta = remap_clamp(trapped_air, kAutoLimitMinFrac * ta_hi, ta_hi);
+    const float phi_wc = remap_clamp(crest, kAutoLimitMinFrac * wc_hi, wc_hi) * phi_div;
+    const float phi_k = remap_clamp(speed, prm->min_speed, kKineticMaxSpeed);
+    const float rate =
+        phi_k * (prm->k_trapped_air * phi_ta + prm->k_wave_crest * phi_wc) * prm->emission_rate;
+    const float expected = rate * prm->dt;
+    const uint32_t base_seed = i + prm->frame_count * 7919u;
+    uint32_t ne = to_u32(std::floor(expected));
+    if (hash_float(base_seed * 13u + 5u) < expected - std::floor(expected)) ne = ne + 1u;
+    n_emit[i] = std::min(ne, kMaxEmitPerFrame);
+  }
+}
+
+// spray_emit.wgsl:322-372: the spawns of sorted particles 0..n-1 (n_emit[i]
+// each), written to ring slots (head + k) % max_particles in sorted order;
+// returns the new head.
+uint32_t oracle_spray_spawn(const void *sorted_particles, uint32_t n,
+                            const GpuSphParams3D *sph, const GpuSprayParams *prm,
+                            const uint32_t *n_emit, uint32_t head, GpuSprayParticle *ring) {
+  using namespace spray;
+  const Part<float> *sp = (const Part<float> *)sorted_particles;
+  const float h = sph->kernel_radius;
+  for (uint32_t i = 0; i < n; ++i) {
+    if (n_emit[i] == 0u) continue;
+    const Part<float> &p = sp[i];
+    const V3<float> pos0 = P3(p.pos), vel = P3(p.vel);
+    const float speed = length(vel);
+    const float phi_k = remap_clamp(speed, prm->min_speed, kKineticMaxSpeed);
+    const uint32_t base_seed = i + prm->frame_count * 7919u;
+    const V3<float> v_dir = divs(vel, speed);
+    V3<float> ref_axis = mk(0.0f, 1.0f, 0.0f);
+    if (std::fabs(v_dir.y) > 0.99f) ref_axis = mk(1.0f, 0.0f, 0.0f);
+    const V3<float> e1 = normalize(cross(v_dir, ref_axis));
+    const V3<float> e2 = cross(v_dir, e1);
+    for (uint32_t e = 0; e < n_emit[i]; ++e) {
+      const uint32_t slot = head % prm->max_particles;
+      head = head + 1u;
+      const uint32_t seed = base_seed + e * 3571u;
+      const float r0 = hash_float(seed), r1 = hash_float(seed + 1u);
+      const float r2 = hash_float(seed + 2u), r3 = hash_float(seed + 3u);
+      const float r4 = hash_float(seed + 4u) * 2.0f - 1.0f;
+      const float r5 = hash_float(seed + 5u) * 2.0f - 1.0f;
+      const float r6 = hash_float(seed + 6u) * 2.0f - 1.0f;
+      const float disc_r = h * 0.5f * std::sqrt(r0);
+      const float theta = kTwoPi * r1;
+      const float along = r2 * prm->dt * speed;
+      const V3<float> pos = add(add(add(pos0, muls(e1, disc_r * std::cos(theta))),
+                                    muls(e2, disc_r * std::sin(theta))),
+                                muls(v_dir, along));
+      const V3<float> jitter = muls(mk(r4, r5, r6), prm->velocity_jitter);
+      const V3<float> spawn_vel = add(muls(vel, prm->speed_multiplier), jitter);
+      const float lt_min = prm->lifetime * (1.0f - prm->lifetime_variation);
+      const float lt_span = 2.0f * prm->lifetime * prm->lifetime_variation;
+      const float lt = lt_min + phi_k * r3 * lt_span;
+      GpuSprayParticle &o = ring[slot];
+      o.pos_x = pos.x;
+      o.pos_y = pos.y;
+      o.pos_z = pos.z;
+      o.lifetime = lt;
+      o.vel_x = spawn_vel.x;
+      o.vel_y = spawn_vel.y;
+      o.vel_z = spawn_vel.z;
+      o.max_lifetime = lt;
+      o.kind = 0u;
+      o.age = 0.0f;
+      o._pad[0] = o._pad[1] = 0.0f;
+    }
+  }
+  return head;
+}
+
+// spray_simulate.wgsl:147-281 over ring[0..max_particles); live = total,
+// spray, foam, bubble counts after the pass.
+void oracle_spray_simulate(GpuSprayParticle *ring, const void *sorted_particles,
+                           const uint32_t *counts, const uint32_t *starts,
+                           const GpuGridParams *grid, const GpuSphParams3D *sph,
+                           const GpuContainerGeometry *cont, const GpuSprayParams *prm,
+                           uint32_t live[4]) {
+  using namespace spray;
+  const Part<float> *sp = (const Part<float> *)sorted_particles;
+  const Grid G{*grid};
+  live[0] = live[1] = live[2] = live[3] = 0u;
+  const float dt = prm->dt, h = sph->kernel_radius, h_sq = sph->kernel_radius_sq;
+  for (uint32_t idx = 0; idx < prm->max_particles; ++idx) {
+    GpuSprayParticle &p = ring[idx];
+    if (p.lifetime <= 0.0f) continue;
+    const V3<float> pos = mk(p.pos_x, p.pos_y, p.pos_z);
+    V3<float> vel = mk(p.vel_x, p.vel_y, p.vel_z);
+    uint32_t neighbor_count = 0u;
+    float w_sum = 0.0f;
+    V3<float> v_avg = mk(0.0f, 0.0f, 0.0f);
+    G.visit(pos, counts, starts, [&](uint32_t j) {
+      const V3<float> r_vec = sub(pos, P3(sp[j].pos));
+      const float r_sq = dot(r_vec, r_vec);
+      if (r_sq < h_sq) {
+        const float w = 1.0f - std::sqrt(r_sq) / h;
+        neighbor_count = neighbor_count + 1u;
+        w_sum = w_sum + w;
+        v_avg = add(v_avg, muls(P3(sp[j].vel), w));
+      }
+    });
+    if (w_sum > 1e-6f) v_avg = divs(v_avg, w_sum);
+    uint32_t kind = 1u;
+    if (neighbor_count <= kSprayMaxNeighbors)
+      kind = 0u;
+    else if (neighbor_count >= kBubbleMinNeighbors)
+      kind = 2u;
+    if (kind == 1u) {
+      const V3<float> lp = world_to_local(*cont, pos);
+      const float half[3] = {cont->half_width, cont->half_height, cont->half_depth};
+      const float l[3] = {lp.x, lp.y, lp.z};
+      float missing = 0.0f;
+      for (int a = 0; a < 3; ++a) {
+        const float dist_lo = half[a] + l[a], dist_hi = half[a] - l[a];
+        if (dist_lo < h) missing += cap_volume_fraction(1.0f - dist_lo / h);
+        if (dist_hi < h) missing += cap_volume_fraction(1.0f - dist_hi / h);
+      }
+      const float visible = std::max(1.0f - missing, 0.25f);
+      if (to_u32(std::nearbyint((float)neighbor_count / visible)) >= kBubbleMinNeighbors)
+        kind = 2u;
+    }
+    if (kind == 1u && p.age < kNewbornSprayTime) kind = 0u;
+    const V3<float> gravity = mk(0.0f, prm->gravity_y, 0.0f);
+    float lifetime_decay = 0.0f;
+    if (kind == 0u) {
+      vel = add(vel, smul(dt, sub(gravity, smul(prm->drag, vel))));
+    } else if (kind == 1u) {
+      vel = v_avg;
+      lifetime_decay = 1.0f;
+    } else {
+      vel = add(vel, smul(dt, smul(-prm->bubble_buoyancy, gravity)));
+      const float t = clampv(prm->bubble_drag, 0.0f, 1.0f);
+      vel = add(muls(vel, 1.0f - t), muls(v_avg, t));
+    }
+    const V3<float> new_pos = add(pos, smul(dt, vel));
+    p.pos_x = new_pos.x;
+    p.pos_y = new_pos.y;
+    p.pos_z = new_pos.z;
+    p.vel_x = vel.x;
+    p.vel_y = vel.y;
+    p.vel_z = vel.z;
+    p.kind = kind;
+    p.age = p.age + dt;
+    p.lifetime = p.lifetime - dt * lifetime_decay;
+    const V3<float> lp = world_to_local(*cont, new_pos);
+    const bool inside = lp.x >= -cont->half_width && lp.x <= cont->half_width &&
+                        lp.y >= -cont->half_height && lp.y <= cont->half_height &&
+                        lp.z >= -cont->half_depth && lp.z <= cont->half_depth;
+    if (!inside) p.lifetime = 0.0f;
+    if (p.lifetime > 0.0f) {
+      live[0] += 1u;
+      live[1u + kind] += 1u;
+    }
+  }
+}
+
+// update_auto_limit (src/simulation/spray.rs:40-51)
+float oracle_spray_update_limit(float current, float frame_max, float floor_v) {
+  if (frame_max <= 1e-6f) return current;
+  const float alpha = frame_max > current ? 0.05f : 0.01f;
+  return std::max(current + alpha * (frame_max - current), floor_v);
+}
+
+// hash_float (spray_emit.wgsl:166-168)
+float oracle_spray_hash_float(uint32_t seed) { return spray::hash_float(seed); }
+uint32_t oracle_spray_hash(uint32_t seed) { return spray::hash(seed); }
 
 // compute_rest_density (src/state/simulation.rs:402-425), f32.
 float oracle_rest_density_f32(float h, float mass) {

@@ -43,6 +43,17 @@ def load():
         lib.oracle_pcisph_delta_f32.argtypes = [C.c_float] * 4
         lib.oracle_powi_f32.restype = C.c_float
         lib.oracle_powi_f32.argtypes = [C.c_float, C.c_int]
+        lib.oracle_spray_potentials.argtypes = [P, C.c_uint32, P, P, P, P, P, C.c_float,
+                                                C.c_float, P, P, P]
+        lib.oracle_spray_spawn.restype = C.c_uint32
+        lib.oracle_spray_spawn.argtypes = [P, C.c_uint32, P, P, P, C.c_uint32, P]
+        lib.oracle_spray_simulate.argtypes = [P, P, P, P, P, P, P, P, P]
+        lib.oracle_spray_update_limit.restype = C.c_float
+        lib.oracle_spray_update_limit.argtypes = [C.c_float] * 3
+        lib.oracle_spray_hash_float.restype = C.c_float
+        lib.oracle_spray_hash_float.argtypes = [C.c_uint32]
+        lib.oracle_spray_hash.restype = C.c_uint32
+        lib.oracle_spray_hash.argtypes = [C.c_uint32]
         _LIB = lib
     return _LIB
 
@@ -184,3 +195,60 @@ def from_scene(scene, abi_mod, params, container, gravity, fp64: bool = False,
     parts = scene.particles if particles is None else particles
     return Oracle(parts, params, g, container, gravity, mouse, rigid_body,
                   scene.pcisph_iterations, fp64=fp64, sdf=sdf)
+
+
+# ---------------------------------------------------------------------------
+# Whitewater passes (spray_emit.wgsl, spray_simulate.wgsl) over a sorted
+# particle array (N, 16) float32 and its grid.  Rings are (max, 12) uint32
+# arrays holding the GpuSprayParticle words (kind is word 8).
+
+def _ptr(a: np.ndarray):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def spray_potentials(sorted_particles: np.ndarray, counts: np.ndarray, starts: np.ndarray,
+                     grid, sph, spray_params, ta_limit: float, wc_limit: float):
+    """(potentials (N, 4) float32, n_emit (N,) uint32, frame_max (2,) float32)."""
+    lib = load()
+    sp = np.ascontiguousarray(sorted_particles, dtype=np.float32).reshape(-1, 16)
+    cnt = np.ascontiguousarray(counts, dtype=np.uint32)
+    st = np.ascontiguousarray(starts, dtype=np.uint32)
+    n = len(sp)
+    pot = np.zeros((n, 4), np.float32)
+    ne = np.zeros(n, np.uint32)
+    fm = np.zeros(2, np.float32)
+    lib.oracle_spray_potentials(_ptr(sp), n, _ptr(cnt), _ptr(st), C.byref(grid), C.byref(sph),
+                                C.byref(spray_params), ta_limit, wc_limit, _ptr(pot), _ptr(ne),
+                                _ptr(fm))
+    return pot, ne, fm
+
+
+def spray_spawn(sorted_particles: np.ndarray, sph, spray_params, n_emit: np.ndarray, head: int,
+                ring: np.ndarray) -> int:
+    """Writes the spawns into `ring` (modified in place); returns the new head."""
+    lib = load()
+    sp = np.ascontiguousarray(sorted_particles, dtype=np.float32).reshape(-1, 16)
+    ne = np.ascontiguousarray(n_emit, dtype=np.uint32)
+    assert ring.dtype == np.uint32 and ring.flags.c_contiguous and ring.shape[1] == 12
+    return int(lib.oracle_spray_spawn(_ptr(sp), len(sp), C.byref(sph), C.byref(spray_params),
+                                      _ptr(ne), head & 0xFFFFFFFF, _ptr(ring)))
+
+
+def spray_simulate(ring: np.ndarray, sorted_particles: np.ndarray, counts: np.ndarray,
+                   starts: np.ndarray, grid, sph, container, spray_params) -> np.ndarray:
+    """Advances `ring` in place; returns the live counts (total, spray, foam, bubble)."""
+    lib = load()
+    sp = np.ascontiguousarray(sorted_particles, dtype=np.float32).reshape(-1, 16)
+    cnt = np.ascontiguousarray(counts, dtype=np.uint32)
+    st = np.ascontiguousarray(starts, dtype=np.uint32)
+    assert ring.dtype == np.uint32 and ring.flags.c_contiguous and ring.shape[1] == 12
+    live = np.zeros(4, np.uint32)
+    lib.oracle_spray_simulate(_ptr(ring), _ptr(sp), _ptr(cnt), _ptr(st), C.byref(grid),
+                              C.byref(sph), C.byref(container), C.byref(spray_params),
+                              _ptr(live))
+    return live
+
+
+def spray_update_limit(current: float, frame_max: float, floor: float) -> float:
+    return float(load().oracle_spray_update_limit(current, frame_max, floor))
+

@@ -59,7 +59,8 @@ def c_layout(expr_list):
 @pytest.mark.parametrize("name,size", [
     ("SphParticle3D", 64), ("GpuSphParams3D", 64), ("GpuGridParams", 48),
     ("GpuContainerGeometry", 128), ("GpuGravity", 16), ("GpuMouseForce", 48),
-    ("GpuRigidBody", 96), ("GpuRigidBodyAccum", 32), ("PredictedState", 32)])
+    ("GpuRigidBody", 96), ("GpuRigidBodyAccum", 32), ("PredictedState", 32),
+    ("GpuSprayParams", 80), ("GpuSprayParticle", 48)])
 def test_struct_sizes_match_reference_layouts(pkg, name, size):
     cls = getattr(pkg.abi, name)
     assert C.sizeof(cls) == size
@@ -75,6 +76,25 @@ def test_create_info_layout(pkg):
                     ["sizeof(fluid_create_info)"])
     assert offs[:-1] == [getattr(pkg.abi.FluidCreateInfo, f).offset for f in fields]
     assert offs[-1] == C.sizeof(pkg.abi.FluidCreateInfo)
+
+
+def test_spray_state_layout(pkg):
+    fields = [f[0] for f in pkg.abi.FluidSprayState._fields_]
+    offs = c_layout([f"offsetof(fluid_spray_state, {f})" for f in fields] +
+                    ["sizeof(fluid_spray_state)"])
+    assert offs[:-1] == [getattr(pkg.abi.FluidSprayState, f).offset for f in fields]
+    assert offs[-1] == C.sizeof(pkg.abi.FluidSprayState)
+
+
+def test_spray_error_paths_without_device(pkg):
+    lib = pkg.abi.load_library()
+    h = C.c_void_p()
+    prm = pkg.abi.default_spray_params(dt=1 / 60)
+    assert lib.fluid_spray_create(None, 16, C.byref(prm), C.byref(h)) == \
+        pkg.abi.FLUID_ERR_INVALID_ARG
+    assert lib.fluid_spray_step(None, 0) == pkg.abi.FLUID_ERR_INVALID_ARG
+    assert lib.fluid_spray_read_stats(None, None) == pkg.abi.FLUID_ERR_INVALID_ARG
+    assert lib.fluid_spray_destroy(None) == pkg.abi.FLUID_OK
 
 
 def test_error_paths_without_device(pkg):
