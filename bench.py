@@ -3,8 +3,9 @@
 Headline: particle-updates/s of the 50,000-particle dam break (lattice 0.025,
 h 0.08), hipGraph-launched steps with all state resident in HBM.  The line
 also carries the 1,048,576-particle block throughput, per-stage times from
-HIP events on the library's stream, the roofline of the dominant kernel and
-the CPU oracle's rate on this host as a reported baseline.
+HIP events on the library's stream, the roofline of the dominant kernel, the
+whitewater passes' frame time and the CPU oracle's rate on this host as a
+reported baseline.
 
     python bench.py [--gpus 1] [--steps 1000] [--warmup 50]
 """
@@ -136,6 +137,38 @@ def run_batched(pkg, count: int, steps: int = 20):
             "speedup": round(t_seq / t_batch, 3),
             "batched_particle_updates_per_s": round(n / (t_batch / 1e3), 1),
             "autotuned": list(cfg), "stage_ms": {a: round(b, 4) for a, b in stages.items()}}
+
+
+def run_spray(pkg, frames_warm: int = 60, frames: int = 20, substeps: int = 2):
+    """Whitewater on the 50k dam break (SpraySystem, spray.rs): 60 SPH steps
+    to the splash, then frames of `substeps` SPH steps plus one spray step
+    (the app's loop, app.rs:1209-1238) to fill the 100k ring, then `frames`
+    spray steps timed on their own (emit: potentials, slot scan, spawns;
+    simulate) over the last sorted state.  Default SprayConfig; the frame
+    time is substeps x dt."""
+    scene = pkg.scenes.dam_break_50k()
+    sim = pkg.make_sim(scene)
+    sim.step(60)
+    n = sim.num_particles()
+    cap = 100_000
+    prm = pkg.abi.default_spray_params(dt=substeps * scene.dt, max_particles=cap,
+                                       num_sph_particles=n)
+    spray = pkg.SpraySystem(sim, cap, prm)
+    for f in range(1, frames_warm + 1):
+        sim.step(substeps)
+        prm.frame_count = f
+        spray.update_params(prm)
+        spray.step(n)
+    st = spray.state()
+    ms = spray.time_steps(2, frames) / frames
+    t_sph = sim.time_steps(0, 20) / 20
+    spray.close()
+    sim.close()
+    return {"workload": "dam_break_50k + 100k-slot ring", "frames_warm": frames_warm,
+            "frames": frames, "ms_per_frame": round(ms, 4),
+            "sph_ms_per_step": round(t_sph, 4), "live": st["live"],
+            "spawned_last_frame": st["emitted"],
+            "ceilings": [round(st["ta_limit"], 4), round(st["wc_limit"], 4)]}
 
 
 def run_adjoint(pkg, orc, steps: int = 64, every: int = 16, fd_coords: int = 64,
@@ -370,6 +403,7 @@ def main() -> int:
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-form comparison")
     ap.add_argument("--batch-scenes", type=int, default=256)
     ap.add_argument("--no-adjoint", action="store_true", help="skip the differentiable rollout")
+    ap.add_argument("--no-spray", action="store_true", help="skip the whitewater passes")
     ap.add_argument("--fd-coords", type=int, default=64)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a 256 MiB device copy (scales the FETCH_SIZE/WRITE_SIZE "
@@ -439,6 +473,9 @@ def main() -> int:
                       "stage_ms": {k: round(v, 4) for k, v in st1.items()},
                       "roofline": roof1}
         del big
+    if not args.no_spray:
+        progress("spray")
+        line["spray"] = run_spray(pkg)
     if not args.no_batch:
         progress("batched")
         line["batched"] = run_batched(pkg, args.batch_scenes)

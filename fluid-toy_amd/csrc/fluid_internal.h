@@ -242,7 +242,7 @@ hipError_t launch_count_pairs(const Uni &u, const Bufs &b, uint32_t n,
 struct SprayDev {
   uint32_t head;
   float ta_limit, wc_limit;
-  uint32_t live[4];        // total, spray, foam, bubble after the last simulate
+  uint32_t live[4];        // total, spray, foam, bubble (k_spray_live_sum)
   uint32_t emitted;
   uint32_t frame_max[2];   // f32 bits of the potential maxima since the last simulate
   uint32_t pad[6];
@@ -257,6 +257,7 @@ struct SprayBufs {
   float4 *pot;             // trapped air, crest, divergence, neighbours
   unsigned long long *tile_status;  // scan of emit_n
   uint32_t *tile_counter;
+  uint32_t *live_part;     // live counts per simulate block (4 each)
 };
 
 // Whitewater passes (spray.hip): emit = potentials, slot scan and spawns
@@ -265,6 +266,9 @@ hipError_t launch_spray_emit(const Uni &u, const Bufs &b, const SprayBufs &sb, u
                              const GpuSprayParams &p, hipStream_t st);
 hipError_t launch_spray_simulate(const Uni &u, const Bufs &b, const SprayBufs &sb,
                                  const GpuSprayParams &p, hipStream_t st);
+// Sums the last simulate pass's per-block live counts into SprayDev::live.
+hipError_t launch_spray_live_sum(const SprayBufs &sb, hipStream_t st);
+uint32_t spray_simulate_blocks(uint32_t cap);
 
 // Host-side parameter helpers (params.cpp).
 void fill_uniforms(DevUniforms &u, const GpuSphParams3D &p,

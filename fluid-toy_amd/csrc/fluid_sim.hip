@@ -1667,6 +1667,8 @@ int fluid_spray_create(fluid_sim *s, uint32_t max_spray, const GpuSprayParams *p
     sp->sb.tile_status =
         (unsigned long long *)alloc((nmax / kScanTile + 2) * sizeof(unsigned long long), &e);
   if (e == hipSuccess) sp->sb.tile_counter = (uint32_t *)alloc(4, &e);
+  if (e == hipSuccess)
+    sp->sb.live_part = (uint32_t *)alloc((size_t)spray_simulate_blocks(max_spray) * 16, &e);
   if (e == hipSuccess) e = spray_init_state(sp.get());
   if (e != hipSuccess) {
     for (void *p : sp->allocs) (void)hipFree(p);
@@ -1722,6 +1724,7 @@ int fluid_spray_read_state(fluid_spray *sp, fluid_spray_state *out) {
   if (rc) return rc;
   if (!out) return fail(FLUID_ERR_INVALID_ARG, "null argument");
   SprayDev d{};
+  HIP_TRY(launch_spray_live_sum(sp->sb, sp->sim->stream));
   HIP_TRY(hipMemcpyAsync(&d, sp->sb.d, sizeof d, hipMemcpyDeviceToHost, sp->sim->stream));
   HIP_TRY(hipStreamSynchronize(sp->sim->stream));
   out->write_head = d.head + d.emitted;
@@ -1795,6 +1798,8 @@ int fluid_spray_reset(fluid_spray *sp) {
   int rc = spray_check(sp);
   if (rc) return rc;
   HIP_TRY(hipMemsetAsync(sp->sb.ring, 0, (size_t)sp->sb.cap * sizeof(GpuSprayParticle),
+                         sp->sim->stream));
+  HIP_TRY(hipMemsetAsync(sp->sb.live_part, 0, (size_t)spray_simulate_blocks(sp->sb.cap) * 16,
                          sp->sim->stream));
   HIP_TRY(spray_init_state(sp));
   return FLUID_OK;

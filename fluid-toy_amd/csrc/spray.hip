@@ -15,8 +15,12 @@
 // This file is compiled without FMA contraction and with correctly rounded
 // division and square root, so every per-particle value that a gate, a
 // count or a spawn depends on (speed, normals, remaps, rates, the wall-cap
-// extrapolation, the integration) is the f32 oracle's to the bit; only the
-// neighbour sums differ, by their split over lanes.
+// extrapolation, the integration) is the f32 oracle's to the bit, given the
+// same neighbour sums.  The sums themselves are split over lanes, so their
+// terms may round differently at no cost to that: the pair terms use the
+// hardware reciprocal square root and reciprocal (a few ulp) and
+// algebraically equal forms, e.g. |u| (1 - u_hat . x_hat) = |u| - u . x_hat.
+// Neighbour counts and the in-range tests stay exact.
 #include "sph_device.h"
 
 namespace fluid {
@@ -43,10 +47,12 @@ constexpr float kAutoLimitDecay = 0.01f;
 constexpr float kAutoTaFloor = 0.2f;
 constexpr float kAutoWcFloor = 0.05f;
 
-// Walkers per emitter over its 8-lane in-range list, and lanes per diffuse
-// particle over its 27 cells.
-constexpr int kEmitLanes = kListLanes;
-constexpr int kSimLanes = 8;
+// Walkers per emitter over its 8-lane in-range list (two per list lane),
+// and lanes per diffuse particle over its 27 cells: a diffuse particle deep
+// in the fluid meets hundreds of candidates, and its lanes' dependent
+// gathers are the pass's critical path.
+constexpr int kEmitLanes = 2 * kListLanes;
+constexpr int kSimLanes = 16;
 
 inline uint32_t blocks_for(uint64_t threads, uint32_t per) {
   return (uint32_t)((threads + per - 1) / per);
@@ -123,10 +129,9 @@ __device__ __forceinline__ uint32_t group_sum_u(uint32_t v) {
 // Emit, part 1 (spray_emit.wgsl:190-320): per sorted particle, the trapped-
 // air, wave-crest and divergence sums over its in-range neighbours, the
 // gates, and the number of particles to spawn.  kEmitLanes lanes per
-// particle; no lane returns early, so the wave-wide maxima and the group
+// particle; no lane returns early, so the block-wide maxima and the group
 // sums see every lane.  Also clears the slot scan's tile flags and, in
-// thread 0, commits the previous pass's spawns to the head and zeroes the
-// live counts.
+// thread 0, commits the previous pass's spawns to the head.
 __global__ __launch_bounds__(256) void k_spray_potentials(Uni un, Bufs b, SprayBufs sb,
                                                           uint32_t n, GpuSprayParams p,
                                                           uint32_t nclear) {
@@ -137,7 +142,6 @@ __global__ __launch_bounds__(256) void k_spray_potentials(Uni un, Bufs b, SprayB
     *sb.tile_counter = 0u;
     d->head += d->emitted;
     d->emitted = 0u;
-    for (int k = 0; k < 4; ++k) d->live[k] = 0u;
   }
   const DevUniforms *__restrict__ U = un.u;
   const uint32_t s = g / kEmitLanes, sub = g % kEmitLanes;
@@ -165,26 +169,30 @@ __global__ __launch_bounds__(256) void k_spray_potentials(Uni un, Bufs b, SprayB
   float ta = 0.f, crest = 0.f, dv = 0.f;
   uint32_t nn = 0u;
   if (active) {
+    const float inv_h = 1.0f / h, mnorm = mass * norm;
+    // the list holds exactly the pairs the WGSL loop keeps, 1e-12 < r^2 < h^2
     for_each_neighbour<kEmitLanes, 2>(U, b, n, s, sub, r0, [&](uint32_t j, float4 pj) {
+      const float4 vj = b.srec[3 * (size_t)j + 1];
       const float rx = r0.x - pj.x, ry = r0.y - pj.y, rz = r0.z - pj.z;
       const float rsq = dot3(rx, ry, rz, rx, ry, rz);
-      const float r = sqrtf(rsq);
-      const float w = 1.0f - r / h;
-      const float xx = rx / r, xy = ry / r, xz = rz / r;
+      const float il = __builtin_amdgcn_rsqf(rsq);
+      const float w = 1.0f - rsq * il * inv_h;  // 1 - r / h
+      const float xx = rx * il, xy = ry * il, xz = rz * il;
       nn += 1u;
       const float d2 = h2 - rsq;
-      const float wvol = (mass / fmaxf(pj.w, 1e-6f)) * norm * d2 * d2 * d2;
-      const float4 vj = b.srec[3 * (size_t)j + 1];
+      const float wvol = mnorm * __builtin_amdgcn_rcpf(fmaxf(pj.w, 1e-6f)) * (d2 * d2 * d2);
       const float ux = vx - vj.x, uy = vy - vj.y, uz = vz - vj.z;
-      const float ulen = sqrtf(dot3(ux, uy, uz, ux, uy, uz));
-      if (ulen > 1e-6f)
-        ta += ulen * (1.0f - dot3(ux / ulen, uy / ulen, uz / ulen, xx, xy, xz)) * wvol;
-      dv += dot3(ux, uy, uz, xx, xy, xz) * w;
-      if (crest_ok && dot3(-xx, -xy, -xz, nhx, nhy, nhz) < 0.0f) {
+      const float uu = dot3(ux, uy, uz, ux, uy, uz), ux_hat = dot3(ux, uy, uz, xx, xy, xz);
+      // |u| > 1e-6: |u| (1 - u_hat . x_hat) = |u| - u . x_hat
+      if (uu > 1e-12f) ta += (uu * __builtin_amdgcn_rsqf(uu) - ux_hat) * wvol;
+      dv += ux_hat * w;
+      // -x_hat . n_hat < 0 exactly when r_vec . n_hat > 0
+      if (crest_ok && dot3(rx, ry, rz, nhx, nhy, nhz) > 0.0f) {
         const float4 nj = b.srec[3 * (size_t)j + 2];
-        const float nl = sqrtf(dot3(nj.x, nj.y, nj.z, nj.x, nj.y, nj.z));
-        if (nl > 1e-4f)
-          crest += (1.0f - dot3(nhx, nhy, nhz, nj.x / nl, nj.y / nl, nj.z / nl)) * wvol;
+        const float nsq = dot3(nj.x, nj.y, nj.z, nj.x, nj.y, nj.z);
+        if (nsq > 1e-8f)  // |n_j| > 1e-4
+          crest += (1.0f - dot3(nhx, nhy, nhz, nj.x, nj.y, nj.z) * __builtin_amdgcn_rsqf(nsq)) *
+                   wvol;
       }
     });
   }
@@ -214,11 +222,25 @@ __global__ __launch_bounds__(256) void k_spray_potentials(Uni un, Bufs b, SprayB
     if (hash_float(base_seed * 13u + 5u) < expected - fl) ne += 1u;
     ne = min(ne, kMaxEmitPerFrame);
   }
+  // Block maxima, then one atomicMax per block and value, skipped when the
+  // frame's maximum is already as large: atomics on one address serialise,
+  // and the maxima settle within a few blocks.
+  __shared__ float s_max[2][4];
   ta_max = wave_max(ta_max);
   wc_max = wave_max(wc_max);
   if ((threadIdx.x & 63u) == 0u) {
-    if (ta_max > 0.f) atomicMax(&d->frame_max[0], __float_as_uint(ta_max));
-    if (wc_max > 0.f) atomicMax(&d->frame_max[1], __float_as_uint(wc_max));
+    s_max[0][threadIdx.x >> 6] = ta_max;
+    s_max[1][threadIdx.x >> 6] = wc_max;
+  }
+  __syncthreads();
+  if (threadIdx.x < 2u) {
+    const float *m4 = s_max[threadIdx.x];
+    const float m = fmaxf(fmaxf(m4[0], m4[1]), fmaxf(m4[2], m4[3]));
+    uint32_t *slot = &d->frame_max[threadIdx.x];
+    // non-negative floats order as their bit patterns
+    if (m > 0.f && __float_as_uint(m) > __hip_atomic_load(slot, __ATOMIC_RELAXED,
+                                                          __HIP_MEMORY_SCOPE_AGENT))
+      atomicMax(slot, __float_as_uint(m));
   }
   if (in && sub == 0) {
     sb.emit_n[s] = ne;
@@ -293,8 +315,8 @@ __global__ __launch_bounds__(256) void k_spray_spawn(Uni un, Bufs b, SprayBufs s
 // fluid neighbours over the 27 cells (kSimLanes lanes per slot) with the
 // radially weighted mean velocity, is reclassified and advanced.  Thread 0
 // folds the maxima of the emit passes since the last simulate into the
-// potential ceilings (spray.rs:40-51) and clears them.  Live counts are
-// added once per wave from ballots.
+// potential ceilings (spray.rs:40-51) and clears them.  Each block leaves
+// its live counts in live_part.
 __global__ __launch_bounds__(256) void k_spray_simulate(Uni un, Bufs b, SprayBufs sb,
                                                         GpuSprayParams p) {
   const uint32_t g = blockIdx.x * 256u + threadIdx.x;
@@ -323,13 +345,16 @@ __global__ __launch_bounds__(256) void k_spray_simulate(Uni un, Bufs b, SprayBuf
     const int cx = cell_coord(q0.x, U->ox, U->inv_cell);
     const int cy = cell_coord(q0.y, U->oy, U->inv_cell);
     const int cz = cell_coord(q0.z, U->oz, U->inv_cell);
+    const float inv_h = 1.0f / h;
     sweep27<kSimLanes, false>(U, b.counts, b.starts, b.spos, cx, cy, cz, sub,
                               [&](uint32_t j, float4 pj, bool) {
+      // loaded whether or not the pair is in range, so it need not wait for
+      // the test (the gathers of a trip go out together)
+      const float4 vj = b.svel[j];
       const float rx = q0.x - pj.x, ry = q0.y - pj.y, rz = q0.z - pj.z;
       const float rsq = dot3(rx, ry, rz, rx, ry, rz);
       if (rsq < h2) {
-        const float w = 1.0f - sqrtf(rsq) / h;
-        const float4 vj = b.svel[j];
+        const float w = 1.0f - fast_sqrt(rsq) * inv_h;
         cnt += 1u;
         ws += w;
         ax += vj.x * w;
@@ -343,82 +368,110 @@ __global__ __launch_bounds__(256) void k_spray_simulate(Uni un, Bufs b, SprayBuf
   ax = group_sum<kSimLanes>(ax);
   ay = group_sum<kSimLanes>(ay);
   az = group_sum<kSimLanes>(az);
-  if (ws > 1e-6f) {  // :197-199
-    ax = ax / ws;
-    ay = ay / ws;
-    az = az / ws;
-  }
-  uint32_t kind = kKindFoam;  // :202-207
-  if (cnt <= kSprayMaxNeighbours) kind = kKindSpray;
-  else if (cnt >= kBubbleMinNeighbours) kind = kKindBubble;
-  if (kind == kKindFoam) {
-    // wall-truncated neighbourhoods (:213-233)
-    const float3 l = world_to_local(U, q0.x, q0.y, q0.z);
-    const float lo[3] = {U->hw + l.x, U->hh + l.y, U->hd + l.z};
-    const float hi[3] = {U->hw - l.x, U->hh - l.y, U->hd - l.z};
-    float missing = 0.0f;
-#pragma unroll
-    for (int a = 0; a < 3; ++a) {
-      if (lo[a] < h) missing += cap_volume_fraction(1.0f - lo[a] / h);
-      if (hi[a] < h) missing += cap_volume_fraction(1.0f - hi[a] / h);
-    }
-    const float visible = fmaxf(1.0f - missing, 0.25f);
-    if (to_u32(rintf((float)cnt / visible)) >= kBubbleMinNeighbours) kind = kKindBubble;
-  }
-  const float age = q2.y;
-  if (kind == kKindFoam && age < kNewbornSprayTime) kind = kKindSpray;  // :235-237
-  const float dt = p.dt, gy = p.gravity_y;
-  float vx = q1.x, vy = q1.y, vz = q1.z, decay = 0.0f;
-  if (kind == kKindSpray) {  // :244-256
-    vx = vx + dt * (0.0f - p.drag * vx);
-    vy = vy + dt * (gy - p.drag * vy);
-    vz = vz + dt * (0.0f - p.drag * vz);
-  } else if (kind == kKindFoam) {
-    vx = ax;
-    vy = ay;
-    vz = az;
-    decay = 1.0f;
-  } else {
-    const float nb = -p.bubble_buoyancy;
-    vx = vx + dt * (nb * 0.0f);
-    vy = vy + dt * (nb * gy);
-    vz = vz + dt * (nb * 0.0f);
-    const float t = clampf(p.bubble_drag, 0.0f, 1.0f);
-    vx = vx * (1.0f - t) + ax * t;
-    vy = vy * (1.0f - t) + ay * t;
-    vz = vz * (1.0f - t) + az * t;
-  }
-  const float nx = q0.x + dt * vx, ny = q0.y + dt * vy, nz = q0.z + dt * vz;  // :257
-  float life = q0.w - dt * decay;
-  const float3 ln = world_to_local(U, nx, ny, nz);  // :268-272
-  if (!(ln.x >= -U->hw && ln.x <= U->hw && ln.y >= -U->hh && ln.y <= U->hh &&
-        ln.z >= -U->hd && ln.z <= U->hd))
-    life = 0.0f;
+  uint32_t kind = kKindSpray;
+  float life = 0.0f;
+  // the slot's update (:197-272), for live slots only
   if (alive && sub == 0) {
+    if (ws > 1e-6f) {  // :197-199
+      ax = ax / ws;
+      ay = ay / ws;
+      az = az / ws;
+    }
+    kind = kKindFoam;  // :202-207
+    if (cnt <= kSprayMaxNeighbours) kind = kKindSpray;
+    else if (cnt >= kBubbleMinNeighbours) kind = kKindBubble;
+    if (kind == kKindFoam) {
+      // wall-truncated neighbourhoods (:213-233)
+      const float3 l = world_to_local(U, q0.x, q0.y, q0.z);
+      const float lo[3] = {U->hw + l.x, U->hh + l.y, U->hd + l.z};
+      const float hi[3] = {U->hw - l.x, U->hh - l.y, U->hd - l.z};
+      float missing = 0.0f;
+#pragma unroll
+      for (int a = 0; a < 3; ++a) {
+        if (lo[a] < h) missing += cap_volume_fraction(1.0f - lo[a] / h);
+        if (hi[a] < h) missing += cap_volume_fraction(1.0f - hi[a] / h);
+      }
+      const float visible = fmaxf(1.0f - missing, 0.25f);
+      if (to_u32(rintf((float)cnt / visible)) >= kBubbleMinNeighbours) kind = kKindBubble;
+    }
+    const float age = q2.y;
+    if (kind == kKindFoam && age < kNewbornSprayTime) kind = kKindSpray;  // :235-237
+    const float dt = p.dt, gy = p.gravity_y;
+    float vx = q1.x, vy = q1.y, vz = q1.z, decay = 0.0f;
+    if (kind == kKindSpray) {  // :244-256
+      vx = vx + dt * (0.0f - p.drag * vx);
+      vy = vy + dt * (gy - p.drag * vy);
+      vz = vz + dt * (0.0f - p.drag * vz);
+    } else if (kind == kKindFoam) {
+      vx = ax;
+      vy = ay;
+      vz = az;
+      decay = 1.0f;
+    } else {
+      const float nb = -p.bubble_buoyancy;
+      vx = vx + dt * (nb * 0.0f);
+      vy = vy + dt * (nb * gy);
+      vz = vz + dt * (nb * 0.0f);
+      const float t = clampf(p.bubble_drag, 0.0f, 1.0f);
+      vx = vx * (1.0f - t) + ax * t;
+      vy = vy * (1.0f - t) + ay * t;
+      vz = vz * (1.0f - t) + az * t;
+    }
+    const float nx = q0.x + dt * vx, ny = q0.y + dt * vy, nz = q0.z + dt * vz;  // :257
+    life = q0.w - dt * decay;
+    const float3 ln = world_to_local(U, nx, ny, nz);  // :268-272
+    if (!(ln.x >= -U->hw && ln.x <= U->hw && ln.y >= -U->hh && ln.y <= U->hh &&
+          ln.z >= -U->hd && ln.z <= U->hd))
+      life = 0.0f;
     float4 *out = sb.ring + 3 * (size_t)idx;
     out[0] = make_float4(nx, ny, nz, life);
     out[1] = make_float4(vx, vy, vz, q1.w);
     out[2] = make_float4(__uint_as_float(kind), age + dt, q2.z, q2.w);
   }
-  // live counts by kind (:275-278)
+  // live counts by kind (:275-278): per block, summed on demand by
+  // k_spray_live_sum (one partial per block, no atomics on shared addresses)
+  __shared__ uint32_t s_live[4][4];
   const bool live = alive && sub == 0 && life > 0.0f;
   const uint32_t c_all = (uint32_t)__popcll(__ballot(live));
   const uint32_t c_spray = (uint32_t)__popcll(__ballot(live && kind == kKindSpray));
   const uint32_t c_foam = (uint32_t)__popcll(__ballot(live && kind == kKindFoam));
   const uint32_t c_bubble = (uint32_t)__popcll(__ballot(live && kind == kKindBubble));
-  if ((threadIdx.x & 63u) == 0u && c_all) {
-    atomicAdd(&d->live[0], c_all);
-    if (c_spray) atomicAdd(&d->live[1], c_spray);
-    if (c_foam) atomicAdd(&d->live[2], c_foam);
-    if (c_bubble) atomicAdd(&d->live[3], c_bubble);
+  if ((threadIdx.x & 63u) == 0u) {
+    const uint32_t w = threadIdx.x >> 6;
+    s_live[0][w] = c_all;
+    s_live[1][w] = c_spray;
+    s_live[2][w] = c_foam;
+    s_live[3][w] = c_bubble;
   }
+  __syncthreads();
+  if (threadIdx.x < 4u) {
+    const uint32_t *c4 = s_live[threadIdx.x];
+    sb.live_part[4 * (size_t)blockIdx.x + threadIdx.x] = c4[0] + c4[1] + c4[2] + c4[3];
+  }
+}
+
+// Sum of the simulate pass's per-block live counts into the state block
+// (one workgroup; launched before a state readback).
+__global__ __launch_bounds__(256) void k_spray_live_sum(SprayBufs sb, uint32_t nblocks) {
+  __shared__ uint32_t s_acc[4][256];
+  uint32_t acc[4] = {0u, 0u, 0u, 0u};
+  for (uint32_t i = threadIdx.x; i < nblocks; i += 256u)
+    for (int k = 0; k < 4; ++k) acc[k] += sb.live_part[4 * (size_t)i + k];
+  for (int k = 0; k < 4; ++k) s_acc[k][threadIdx.x] = acc[k];
+  __syncthreads();
+  for (uint32_t o = 128u; o >= 1u; o >>= 1) {
+    if (threadIdx.x < o)
+      for (int k = 0; k < 4; ++k) s_acc[k][threadIdx.x] += s_acc[k][threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x < 4u) sb.d->live[threadIdx.x] = s_acc[threadIdx.x][0];
 }
 
 }  // namespace
 
 hipError_t launch_spray_emit(const Uni &u, const Bufs &b, const SprayBufs &sb, uint32_t n,
                              const GpuSprayParams &p, hipStream_t st) {
-  // one block at least: it commits the previous spawns and clears the counts
+  // one block at least: it commits the previous spawns
   const uint32_t nclear = blocks_for(n, kScanTile);
   const uint32_t grid = n ? blocks_for((uint64_t)n * kEmitLanes, 256) : 1u;
   hipLaunchKernelGGL(k_spray_potentials, dim3(grid), dim3(256), 0, st, u, b, sb, n, p, nclear);
@@ -436,10 +489,20 @@ hipError_t launch_spray_emit(const Uni &u, const Bufs &b, const SprayBufs &sb, u
   return hipGetLastError();
 }
 
+uint32_t spray_simulate_blocks(uint32_t cap) {
+  return blocks_for((uint64_t)cap * kSimLanes, 256);
+}
+
 hipError_t launch_spray_simulate(const Uni &u, const Bufs &b, const SprayBufs &sb,
                                  const GpuSprayParams &p, hipStream_t st) {
-  hipLaunchKernelGGL(k_spray_simulate, dim3(blocks_for((uint64_t)sb.cap * kSimLanes, 256)),
-                     dim3(256), 0, st, u, b, sb, p);
+  hipLaunchKernelGGL(k_spray_simulate, dim3(spray_simulate_blocks(sb.cap)), dim3(256), 0, st, u,
+                     b, sb, p);
+  return hipGetLastError();
+}
+
+hipError_t launch_spray_live_sum(const SprayBufs &sb, hipStream_t st) {
+  hipLaunchKernelGGL(k_spray_live_sum, dim3(1), dim3(256), 0, st, sb,
+                     spray_simulate_blocks(sb.cap));
   return hipGetLastError();
 }
 
