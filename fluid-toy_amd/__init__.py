@@ -3,13 +3,14 @@
 The compute library is libfluidhip.so (C ABI, include/fluid_abi.h); this
 package is its Python host mirror: `SphSimulation3DGrid` follows the
 reference's struct of the same name (src/simulation/sph_3d_grid.rs), and
-`SpraySystem` follows src/simulation/spray.rs, and `scenes` holds the
+`SpraySystem` follows src/simulation/spray.rs, `MarchingCubesField` the
+compute passes of src/render/marching_cubes.rs, and `scenes` holds the
 seeded workloads.  Every call goes to the HIP library;
 there is no CPU fallback, and a missing library raises FileNotFoundError.
 """
 from . import abi, scenes
-from .sim import (SphBatch, SphSimulation3DGrid, SpraySystem, derive_container_geometry,
-                  derive_sph_params, gravity)
+from .sim import (MarchingCubesField, SphBatch, SphSimulation3DGrid, SpraySystem,
+                  derive_container_geometry, derive_sph_params, gravity)
 
 
 def make_sim(scene, device: int = 0, max_particles: int = 0) -> SphSimulation3DGrid:
@@ -41,5 +42,6 @@ def make_batch(scene_list, device: int = 0) -> SphBatch:
     return SphBatch([scene_tuple(sc) for sc in scene_list], its.pop(), device=device)
 
 
-__all__ = ["abi", "scenes", "SphSimulation3DGrid", "SphBatch", "SpraySystem", "derive_sph_params",
+__all__ = ["abi", "scenes", "SphSimulation3DGrid", "SphBatch", "SpraySystem",
+           "MarchingCubesField", "derive_sph_params",
            "derive_container_geometry", "gravity", "make_sim", "make_batch", "scene_tuple"]

@@ -130,6 +130,36 @@ class GpuSprayParticle(C.Structure):
                 ("age", C.c_float), ("_pad", C.c_float * 2)]
 
 
+class GpuMcGridParams(C.Structure):
+    """src/render/marching_cubes.rs:55-66 (48 B; GpuGridParams there)."""
+    _fields_ = [("grid_min", f3), ("grid_size", C.c_uint32), ("grid_max", f3),
+                ("cell_size", C.c_float), ("kernel_radius", C.c_float), ("iso_value", C.c_float),
+                ("num_particles", C.c_uint32), ("max_vertices", C.c_uint32)]
+
+
+class GpuAnisoParams(C.Structure):
+    """src/render/marching_cubes.rs:41-51 (32 B)."""
+    _fields_ = [("enabled", C.c_uint32), ("strength", C.c_float),
+                ("support_radius", C.c_float), ("h_mc", C.c_float), ("kr", C.c_float),
+                ("lambda_", C.c_float), ("max_stretch", C.c_float), ("max_shift", C.c_float)]
+
+
+class GpuParticleAniso(C.Structure):
+    """ParticleAniso, mc_anisotropy.wgsl:46-50 (48 B)."""
+    _fields_ = [("q0", C.c_float * 4), ("q1", C.c_float * 4), ("q2", C.c_float * 4)]
+
+
+def default_aniso_params(h: float, enabled: bool = True, strength: float = 1.0,
+                         h_mc: float | None = None) -> GpuAnisoParams:
+    """update_aniso_params (marching_cubes.rs:2133-2146) with its constants
+    (:26-34): support 2h, k_r 4, lambda 0.9, stretch cap 1.6, shift cap 0.4h."""
+    a = GpuAnisoParams()
+    a.enabled, a.strength = int(enabled), min(max(strength, 0.0), 1.0)
+    a.support_radius, a.h_mc = 2.0 * h, h if h_mc is None else h_mc
+    a.kr, a.lambda_, a.max_stretch, a.max_shift = 4.0, 0.9, 1.6, 0.4 * h
+    return a
+
+
 class FluidSprayState(C.Structure):
     _fields_ = [("write_head", C.c_uint32), ("ta_limit", C.c_float), ("wc_limit", C.c_float),
                 ("live", C.c_uint32 * 4), ("emitted", C.c_uint32)]
@@ -247,6 +277,16 @@ PROTOTYPES = {
     "fluid_spray_read_emit": (C.c_int, [P, P, P, u32]),
     "fluid_spray_reset": (C.c_int, [P]),
     "fluid_spray_time_steps": (C.c_int, [P, u32, u32, u32, C.POINTER(C.c_float)]),
+    "fluid_mc_create": (C.c_int, [P, u32, C.POINTER(P)]),
+    "fluid_mc_destroy": (C.c_int, [P]),
+    "fluid_mc_update_params": (C.c_int, [P, C.POINTER(GpuMcGridParams)]),
+    "fluid_mc_update_aniso_params": (C.c_int, [P, C.POINTER(GpuAnisoParams)]),
+    "fluid_mc_update_container_geometry": (C.c_int, [P, C.POINTER(GpuContainerGeometry)]),
+    "fluid_mc_generate": (C.c_int, [P, u32]),
+    "fluid_mc_read_density": (C.c_int, [P, P, C.c_uint64]),
+    "fluid_mc_read_aniso": (C.c_int, [P, P, u32]),
+    "fluid_mc_density_from_aniso": (C.c_int, [P, P, u32]),
+    "fluid_mc_time": (C.c_int, [P, u32, u32, u32, C.POINTER(C.c_float)]),
 }
 
 FLUID_FUSE_XSPH_FORCE = 1

@@ -57,6 +57,9 @@ static_assert(sizeof(GpuSprayParams) == 80, "GpuSprayParams is 80 B");
 static_assert(offsetof(GpuSprayParams, wc_limit) == 68, "wc_limit @68");
 static_assert(sizeof(GpuSprayParticle) == 48, "GpuSprayParticle is 48 B");
 static_assert(offsetof(GpuSprayParticle, kind) == 32, "kind @32");
+static_assert(sizeof(GpuMcGridParams) == 48, "GpuMcGridParams is 48 B");
+static_assert(sizeof(GpuAnisoParams) == 32, "GpuAnisoParams is 32 B");
+static_assert(sizeof(GpuParticleAniso) == 48, "GpuParticleAniso is 48 B");
 
 namespace fluid {
 
@@ -269,6 +272,29 @@ hipError_t launch_spray_simulate(const Uni &u, const Bufs &b, const SprayBufs &s
 // Sums the last simulate pass's per-block live counts into SprayDev::live.
 hipError_t launch_spray_live_sum(const SprayBufs &sb, hipStream_t st);
 uint32_t spray_simulate_blocks(uint32_t cap);
+
+// Marching-cubes field passes (mc.hip).  Per-launch constants are derived on
+// the host in f32, in the WGSL expressions' order.
+struct McLaunch {
+  GpuMcGridParams g;
+  GpuAnisoParams a;
+  GpuContainerGeometry c;
+  float peak;        // 315 / (64 pi h_mc^3)          mc_anisotropy.wgsl:154
+  float coeff;       // 315 / (64 pi pow(h, 9))       mc_density.wgsl:81
+  int aniso_cells;   // ceil(support_radius * inv_cell)   mc_anisotropy.wgsl:173
+  int density_cells; // ceil(search_reach * inv_cell)     mc_density.wgsl:151-155
+};
+hipError_t launch_mc_aniso(const Uni &u, const Bufs &b, const McLaunch &m, float4 *aniso,
+                           uint32_t n, hipStream_t st);
+hipError_t launch_mc_density(const Uni &u, const Bufs &b, const McLaunch &m,
+                             const float4 *aniso, float *field, hipStream_t st);
+
+// The MC passes' per-launch constants (params.cpp, f32 without
+// contraction, in the WGSL order), and the app's default parameters.
+McLaunch make_mc_launch(const GpuMcGridParams &g, const GpuAnisoParams &a,
+                        const GpuContainerGeometry &c, float sph_inv_cell);
+GpuAnisoParams default_aniso_params(float h);
+GpuMcGridParams default_mc_grid(const GpuContainerGeometry &c, float h, uint32_t grid_size);
 
 // Host-side parameter helpers (params.cpp).
 void fill_uniforms(DevUniforms &u, const GpuSphParams3D &p,

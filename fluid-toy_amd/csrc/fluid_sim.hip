@@ -99,7 +99,7 @@ struct fluid_sim {
   bool tuned = false;  // cfg chosen by fluid_set_sweep_lanes / fluid_autotune
   uint32_t ntiles = 0;
   RolloutState roll;
-  uint32_t sprays = 0;  // spray handles bound to this simulation
+  uint32_t bound = 0;  // spray and MC field handles bound to this simulation
 
   Uni uni() const { return Uni{d_u, d_scene_of}; }
   bool batched() const { return sc.size() > 1; }
@@ -817,7 +817,8 @@ int fluid_batch_update_scene(fluid_sim *s, uint32_t scene, const GpuSphParams3D 
 
 int fluid_destroy(fluid_sim *s) {
   if (!s) return FLUID_OK;
-  if (s->sprays) return fail(FLUID_ERR_STATE, "destroy the simulation's spray handles first");
+  if (s->bound)
+    return fail(FLUID_ERR_STATE, "destroy the simulation's spray and MC field handles first");
   (void)hipSetDevice(s->device);
   if (s->stream) (void)hipStreamSynchronize(s->stream);
   drop_graph(s);
@@ -1674,7 +1675,7 @@ int fluid_spray_create(fluid_sim *s, uint32_t max_spray, const GpuSprayParams *p
     for (void *p : sp->allocs) (void)hipFree(p);
     return fail(FLUID_ERR_HIP, std::string("spray allocation: ") + hipGetErrorString(e));
   }
-  ++s->sprays;
+  ++s->bound;
   *out = sp.release();
   return FLUID_OK;
 }
@@ -1684,7 +1685,7 @@ int fluid_spray_destroy(fluid_spray *sp) {
   if (sp->sim) {
     (void)hipSetDevice(sp->sim->device);
     (void)hipStreamSynchronize(sp->sim->stream);
-    --sp->sim->sprays;
+    --sp->sim->bound;
   }
   for (void *p : sp->allocs) (void)hipFree(p);
   delete sp;
@@ -1829,6 +1830,189 @@ int fluid_spray_time_steps(fluid_spray *sp, uint32_t num_sph_particles, uint32_t
   (void)hipEventDestroy(e1);
   if (rc) return rc;
   return check_errors(sp->sim);
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------------------
+// Marching-cubes density field (MarchingCubesRenderer's compute passes 0 and
+// 1, src/render/marching_cubes.rs:2349-2470), bound to a simulation like a
+// spray handle.
+struct fluid_mc {
+  fluid_sim *sim = nullptr;
+  uint32_t G = 0;
+  GpuMcGridParams g{};
+  GpuAnisoParams a{};
+  GpuContainerGeometry c{};
+  float *field = nullptr;
+  float4 *aniso = nullptr;
+  std::vector<void *> allocs;
+};
+
+namespace {
+
+int mc_check(fluid_mc *mc) {
+  if (!mc || !mc->sim) return fail(FLUID_ERR_INVALID_ARG, "null MC field handle");
+  if (mc->sim->cursor != -1)
+    return fail(FLUID_ERR_STATE, "a partial step of the simulation is in progress");
+  HIP_TRY(hipSetDevice(mc->sim->device));
+  return FLUID_OK;
+}
+
+int mc_launch(fluid_mc *mc, uint32_t n, bool aniso_pass) {
+  fluid_sim *s = mc->sim;
+  const McLaunch m = make_mc_launch(mc->g, mc->a, mc->c, s->sc[0].grid.inv_cell_size);
+  if (aniso_pass && mc->a.enabled && n)
+    HIP_TRY(launch_mc_aniso(s->uni(), s->b, m, mc->aniso, n, s->stream));
+  HIP_TRY(launch_mc_density(s->uni(), s->b, m, mc->aniso, mc->field, s->stream));
+  return FLUID_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int fluid_mc_create(fluid_sim *s, uint32_t grid_size, fluid_mc **out) {
+  if (!s || !out) return fail(FLUID_ERR_INVALID_ARG, "null argument");
+  *out = nullptr;
+  if (s->batched()) return fail(FLUID_ERR_INVALID_ARG, "the MC field needs a plain simulation");
+  if (grid_size == 0 || grid_size > 512)
+    return fail(FLUID_ERR_INVALID_ARG, "grid_size must be 1 to 512");
+  auto mc = std::make_unique<fluid_mc>();
+  mc->sim = s;
+  mc->G = grid_size;
+  // the app's defaults (app.rs:1472-1519, marching_cubes.rs:2133-2146):
+  // isotropic h_mc = h, anisotropy on, bounds = the container +- margin
+  const float h = s->sc[0].params.kernel_radius;
+  const GpuContainerGeometry &cg = s->sc[0].cont;
+  mc->a = default_aniso_params(h);
+  mc->c = cg;
+  mc->c.clip_enabled = 0u;
+  mc->g = default_mc_grid(cg, h, grid_size);
+  HIP_TRY(hipSetDevice(s->device));
+  hipError_t e = hipSuccess;
+  auto alloc = [&](size_t bytes) {
+    void *p = nullptr;
+    if (e == hipSuccess) e = hipMalloc(&p, bytes ? bytes : 16);
+    if (e == hipSuccess) {
+      mc->allocs.push_back(p);
+      e = hipMemset(p, 0, bytes ? bytes : 16);
+    }
+    return p;
+  };
+  const size_t g3 = (size_t)grid_size * grid_size * grid_size;
+  mc->field = (float *)alloc(g3 * sizeof(float));
+  mc->aniso = (float4 *)alloc((size_t)std::max<uint32_t>(s->max_n, 1) * sizeof(GpuParticleAniso));
+  if (e != hipSuccess) {
+    for (void *p : mc->allocs) (void)hipFree(p);
+    return fail(FLUID_ERR_HIP, std::string("MC field allocation: ") + hipGetErrorString(e));
+  }
+  ++s->bound;
+  *out = mc.release();
+  return FLUID_OK;
+}
+
+int fluid_mc_destroy(fluid_mc *mc) {
+  if (!mc) return FLUID_OK;
+  if (mc->sim) {
+    (void)hipSetDevice(mc->sim->device);
+    (void)hipStreamSynchronize(mc->sim->stream);
+    --mc->sim->bound;
+  }
+  for (void *p : mc->allocs) (void)hipFree(p);
+  delete mc;
+  return FLUID_OK;
+}
+
+int fluid_mc_update_params(fluid_mc *mc, const GpuMcGridParams *p) {
+  if (!mc || !p) return fail(FLUID_ERR_INVALID_ARG, "null argument");
+  if (p->grid_size != mc->G)
+    return fail(FLUID_ERR_INVALID_ARG, "params.grid_size differs from the field's");
+  if (!(p->kernel_radius > 0.0f) || !(p->cell_size > 0.0f))
+    return fail(FLUID_ERR_INVALID_ARG, "kernel_radius and cell_size must be positive");
+  mc->g = *p;
+  return FLUID_OK;
+}
+
+int fluid_mc_update_aniso_params(fluid_mc *mc, const GpuAnisoParams *p) {
+  if (!mc || !p) return fail(FLUID_ERR_INVALID_ARG, "null argument");
+  if (p->enabled && !(p->support_radius > 0.0f && p->h_mc > 0.0f && p->kr > 0.0f))
+    return fail(FLUID_ERR_INVALID_ARG, "support_radius, h_mc and kr must be positive");
+  mc->a = *p;
+  return FLUID_OK;
+}
+
+int fluid_mc_update_container_geometry(fluid_mc *mc, const GpuContainerGeometry *geom) {
+  if (!mc || !geom) return fail(FLUID_ERR_INVALID_ARG, "null argument");
+  mc->c = *geom;
+  return FLUID_OK;
+}
+
+int fluid_mc_generate(fluid_mc *mc, uint32_t num_particles) {
+  int rc = mc_check(mc);
+  if (rc) return rc;
+  return mc_launch(mc, std::min(num_particles, mc->sim->n), true);
+}
+
+int fluid_mc_density_from_aniso(fluid_mc *mc, const GpuParticleAniso *in, uint32_t count) {
+  int rc = mc_check(mc);
+  if (rc) return rc;
+  if (!in || count != mc->sim->n)
+    return fail(FLUID_ERR_INVALID_ARG, "one record per particle of the simulation");
+  HIP_TRY(hipMemcpyAsync(mc->aniso, in, (size_t)count * sizeof(GpuParticleAniso),
+                         hipMemcpyHostToDevice, mc->sim->stream));
+  rc = mc_launch(mc, count, false);
+  if (rc) return rc;
+  HIP_TRY(hipStreamSynchronize(mc->sim->stream));
+  return FLUID_OK;
+}
+
+int fluid_mc_read_density(fluid_mc *mc, float *out, uint64_t count) {
+  int rc = mc_check(mc);
+  if (rc) return rc;
+  const uint64_t g3 = (uint64_t)mc->G * mc->G * mc->G;
+  if (!out) return fail(FLUID_ERR_INVALID_ARG, "null argument");
+  if (count < g3) return fail(FLUID_ERR_CAPACITY, "output smaller than grid_size^3");
+  HIP_TRY(hipMemcpyAsync(out, mc->field, g3 * sizeof(float), hipMemcpyDeviceToHost,
+                         mc->sim->stream));
+  HIP_TRY(hipStreamSynchronize(mc->sim->stream));
+  return FLUID_OK;
+}
+
+int fluid_mc_read_aniso(fluid_mc *mc, GpuParticleAniso *out, uint32_t count) {
+  int rc = mc_check(mc);
+  if (rc) return rc;
+  const uint32_t n = mc->sim->n;
+  if (!out && n) return fail(FLUID_ERR_INVALID_ARG, "null argument");
+  if (count < n) return fail(FLUID_ERR_CAPACITY, "output smaller than num_particles");
+  if (n)
+    HIP_TRY(hipMemcpyAsync(out, mc->aniso, (size_t)n * sizeof(GpuParticleAniso),
+                           hipMemcpyDeviceToHost, mc->sim->stream));
+  HIP_TRY(hipStreamSynchronize(mc->sim->stream));
+  return FLUID_OK;
+}
+
+int fluid_mc_time(fluid_mc *mc, uint32_t num_particles, uint32_t warmup, uint32_t iters,
+                  float *ms) {
+  int rc = mc_check(mc);
+  if (rc) return rc;
+  if (!ms) return fail(FLUID_ERR_INVALID_ARG, "null argument");
+  const uint32_t n = std::min(num_particles, mc->sim->n);
+  for (uint32_t k = 0; k < warmup; ++k)
+    if ((rc = mc_launch(mc, n, true))) return rc;
+  hipStream_t st = mc->sim->stream;
+  hipEvent_t e0, e1;
+  HIP_TRY(hipEventCreate(&e0));
+  HIP_TRY(hipEventCreate(&e1));
+  HIP_TRY(hipStreamSynchronize(st));
+  HIP_TRY(hipEventRecord(e0, st));
+  for (uint32_t k = 0; k < iters && rc == FLUID_OK; ++k) rc = mc_launch(mc, n, true);
+  HIP_TRY(hipEventRecord(e1, st));
+  HIP_TRY(hipEventSynchronize(e1));
+  HIP_TRY(hipEventElapsedTime(ms, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  return rc;
 }
 
 }  // extern "C"

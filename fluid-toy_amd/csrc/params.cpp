@@ -2,6 +2,7 @@
 // (src/state/simulation.rs) and the device uniform block.  Everything here is
 // f32 arithmetic in the reference's operation order.  Built without FMA
 // contraction (see the Makefile) so each expression rounds as written.
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 
@@ -243,3 +244,65 @@ int fluid_derive_container_geometry(float width, float height, float depth,
 }
 
 }  // extern "C"
+
+namespace fluid {
+
+// Per-launch constants of the MC passes, each in its WGSL expression order.
+McLaunch make_mc_launch(const GpuMcGridParams &g, const GpuAnisoParams &a,
+                        const GpuContainerGeometry &c, float sph_inv_cell) {
+  McLaunch m;
+  std::memset(&m, 0, sizeof(m));
+  m.g = g;
+  m.a = a;
+  m.c = c;
+  const float hm = a.h_mc;
+  m.peak = 315.0f / (64.0f * kPI * hm * hm * hm);                            // mc_anisotropy.wgsl:154
+  m.coeff = 315.0f / (64.0f * 3.14159265359f * std::pow(g.kernel_radius, 9.0f));  // mc_density.wgsl:81
+  m.aniso_cells = (int)std::ceil(a.support_radius * sph_inv_cell);          // mc_anisotropy.wgsl:173
+  float reach = g.kernel_radius;                                            // mc_density.wgsl:151-155
+  if (a.enabled) reach = g.kernel_radius * a.max_stretch + a.max_shift;
+  m.density_cells = (int)std::ceil(reach * sph_inv_cell);
+  return m;
+}
+
+// update_aniso_params with the app's settings (marching_cubes.rs:26-34,
+// 2133-2146; app.rs:1514-1519; mc_anisotropy on, strength 1, h_mc = h).
+GpuAnisoParams default_aniso_params(float h) {
+  GpuAnisoParams a;
+  a.enabled = 1u;
+  a.strength = 1.0f;
+  a.support_radius = 2.0f * h;
+  a.h_mc = h;
+  a.kr = 4.0f;
+  a.lambda = 0.9f;
+  a.max_stretch = 1.6f;
+  a.max_shift = 0.4f * h;
+  return a;
+}
+
+// update_params after set_bounds as the app calls them (app.rs:1472-1513,
+// marching_cubes.rs:2099-2131) for an untilted container: its box plus a
+// margin of 1.6 h + 0.05, cubic cells of the largest extent / grid_size,
+// iso value 0.95 of the poly6 peak (rendering.rs:313-319).
+GpuMcGridParams default_mc_grid(const GpuContainerGeometry &c, float h, uint32_t grid_size) {
+  GpuMcGridParams g;
+  std::memset(&g, 0, sizeof(g));
+  const float margin = h * 1.0f * 1.6f + 0.05f;
+  const float lo[3] = {-c.half_width, c.center_y - c.half_height, -c.half_depth};
+  const float hi[3] = {c.half_width, c.center_y + c.half_height, c.half_depth};
+  for (int k = 0; k < 3; ++k) {
+    g.grid_min[k] = lo[k] - margin;
+    g.grid_max[k] = hi[k] + margin;
+  }
+  g.grid_size = grid_size;
+  const float ex = g.grid_max[0] - g.grid_min[0], ey = g.grid_max[1] - g.grid_min[1];
+  const float ez = g.grid_max[2] - g.grid_min[2];
+  g.cell_size = std::max(ex, std::max(ey, ez)) / (float)grid_size;
+  g.kernel_radius = h;
+  const float pi = 3.14159274101257324f;  // std::f32::consts::PI
+  g.iso_value = 0.95f * (315.0f / (64.0f * pi * h * h * h));
+  g.max_vertices = 4000000u;
+  return g;
+}
+
+}  // namespace fluid

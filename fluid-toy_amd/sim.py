@@ -82,7 +82,7 @@ class SphSimulation3DGrid:
 
     # -- lifetime ---------------------------------------------------------
     def close(self) -> None:
-        for sp in list(getattr(self, "_sprays", ())):
+        for sp in list(getattr(self, "_bound", ())):
             sp.close()
         if getattr(self, "_h", None):
             self._lib.fluid_destroy(self._h)
@@ -382,6 +382,17 @@ class SphBatch(SphSimulation3DGrid):
         return self._read_u32(self._lib.fluid_read_cell_starts, total)
 
 
+def _bind(sim, obj) -> None:
+    if not hasattr(sim, "_bound"):
+        sim._bound = []
+    sim._bound.append(obj)
+
+
+def _unbind(sim, obj) -> None:
+    if obj in getattr(sim, "_bound", []):
+        sim._bound.remove(obj)
+
+
 class SpraySystem:
     """Mirror of the reference's SpraySystem (src/simulation/spray.rs:82-491)
     over fluid_spray_*: whitewater particles emitted from and advected by the
@@ -402,16 +413,13 @@ class SpraySystem:
         abi.check(self._lib, self._lib.fluid_spray_create(sim._h, max_spray, C.byref(params),
                                                           C.byref(handle)))
         self._h = handle
-        if not hasattr(sim, "_sprays"):
-            sim._sprays = []
-        sim._sprays.append(self)
+        _bind(sim, self)
 
     def close(self) -> None:
         if getattr(self, "_h", None):
             self._lib.fluid_spray_destroy(self._h)
             self._h = None
-            if self in getattr(self._sim, "_sprays", []):
-                self._sim._sprays.remove(self)
+            _unbind(self._sim, self)
 
     def __del__(self):
         try:
@@ -485,5 +493,77 @@ class SpraySystem:
         n = self._sim.num_particles() if num_sph_particles is None else num_sph_particles
         ms = C.c_float(0.0)
         self._chk(self._lib.fluid_spray_time_steps(self._h, n, warmup, frames, C.byref(ms)))
+        return ms.value
+
+
+class MarchingCubesField:
+    """The compute passes 0 and 1 of the reference's
+    MarchingCubesRenderer::generate (src/render/marching_cubes.rs:2349-2470)
+    over fluid_mc_*: the anisotropic kernel fit of the sorted particles and
+    the density field on a grid_size^3 voxel grid (x fastest, as the
+    reference's 3D texture)."""
+
+    def __init__(self, sim: SphSimulation3DGrid, grid_size: int = 200):
+        self._lib = sim._lib
+        self._sim = sim
+        self.grid_size = int(grid_size)
+        handle = C.c_void_p()
+        abi.check(self._lib, self._lib.fluid_mc_create(sim._h, grid_size, C.byref(handle)))
+        self._h = handle
+        _bind(sim, self)
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            self._lib.fluid_mc_destroy(self._h)
+            self._h = None
+            _unbind(self._sim, self)
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _chk(self, rc: int) -> None:
+        abi.check(self._lib, rc)
+
+    def update_params(self, params: abi.GpuMcGridParams) -> None:
+        """update_params + set_bounds (marching_cubes.rs:2099-2131, 2170-2174)."""
+        self._chk(self._lib.fluid_mc_update_params(self._h, C.byref(params)))
+
+    def update_aniso_params(self, params: abi.GpuAnisoParams) -> None:
+        """update_aniso_params (marching_cubes.rs:2133-2146)."""
+        self._chk(self._lib.fluid_mc_update_aniso_params(self._h, C.byref(params)))
+
+    def update_container_geometry(self, geom: abi.GpuContainerGeometry) -> None:
+        self._chk(self._lib.fluid_mc_update_container_geometry(self._h, C.byref(geom)))
+
+    def generate(self, num_particles: int | None = None) -> None:
+        """generate, passes 0 (anisotropy, when enabled) and 1 (density)."""
+        n = self._sim.num_particles() if num_particles is None else num_particles
+        self._chk(self._lib.fluid_mc_generate(self._h, n))
+
+    def read_density(self) -> np.ndarray:
+        """The density field as a (G, G, G) float32 array indexed [z, y, x]."""
+        G = self.grid_size
+        out = np.zeros((G, G, G), np.float32)
+        self._chk(self._lib.fluid_mc_read_density(self._h, out.ctypes.data, out.size))
+        return out
+
+    def read_aniso(self) -> np.ndarray:
+        """Ellipsoid records of the sorted particles, (N, 12) float32."""
+        n = self._sim.num_particles()
+        out = np.zeros((n, 12), np.float32)
+        self._chk(self._lib.fluid_mc_read_aniso(self._h, out.ctypes.data if n else None, n))
+        return out
+
+    def density_from_aniso(self, aniso: np.ndarray) -> None:
+        a = np.ascontiguousarray(aniso, dtype=np.float32).reshape(-1, 12)
+        self._chk(self._lib.fluid_mc_density_from_aniso(self._h, a.ctypes.data, len(a)))
+
+    def time(self, warmup: int, iters: int, num_particles: int | None = None) -> float:
+        n = self._sim.num_particles() if num_particles is None else num_particles
+        ms = C.c_float(0.0)
+        self._chk(self._lib.fluid_mc_time(self._h, n, warmup, iters, C.byref(ms)))
         return ms.value
 

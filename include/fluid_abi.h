@@ -569,6 +569,80 @@ int fluid_spray_reset(fluid_spray *spray);
 int fluid_spray_time_steps(fluid_spray *spray, uint32_t num_sph_particles,
                            uint32_t warmup, uint32_t frames, float *ms);
 
+/* ---- Marching-cubes density field: anisotropy and splat ------------------
+ * The compute passes of the reference's MarchingCubesRenderer::generate
+ * (src/render/marching_cubes.rs:2349-2470) that read the step's sorted
+ * particles and grid: pass 0, the per-particle anisotropic kernel fit
+ * (mc_anisotropy.wgsl, Yu & Turk), and pass 1, the density field on the MC
+ * voxel grid (mc_density.wgsl).  The blur, the surface extraction and the
+ * shading are rendering and stay with the caller.  A field handle is bound
+ * to one plain simulation handle, shares its stream and must be destroyed
+ * before it.  The density field is grid_size^3 floats, x fastest
+ * (texture texel (x, y, z) at x + y*G + z*G*G); voxels outside a clipping
+ * container hold -1. */
+
+/* src/render/marching_cubes.rs:55-66 (48 B; named GpuGridParams there) */
+typedef struct GpuMcGridParams {
+  float grid_min[3];
+  uint32_t grid_size;
+  float grid_max[3];
+  float cell_size;
+  float kernel_radius;
+  float iso_value;
+  uint32_t num_particles;
+  uint32_t max_vertices;
+} GpuMcGridParams;
+
+/* src/render/marching_cubes.rs:41-51 (32 B) */
+typedef struct GpuAnisoParams {
+  uint32_t enabled;
+  float strength;
+  float support_radius;
+  float h_mc;
+  float kr;
+  float lambda;
+  float max_stretch;
+  float max_shift;
+} GpuAnisoParams;
+
+/* ParticleAniso, mc_anisotropy.wgsl:46-50 (48 B): (Gxx, Gxy, Gxz, cx),
+ * (Gyy, Gyz, Gzz, cy), (cz, reach, amplitude, 0) */
+typedef struct GpuParticleAniso {
+  float q0[4];
+  float q1[4];
+  float q2[4];
+} GpuParticleAniso;
+
+typedef struct fluid_mc fluid_mc;
+
+/* The field's device resources of MarchingCubesRenderer::new
+ * (marching_cubes.rs:388-520): a grid_size^3 density field (1 to 512 per
+ * axis) and per-particle ellipsoid records for the simulation's
+ * max_particles. */
+int fluid_mc_create(fluid_sim *sim, uint32_t grid_size, fluid_mc **out);
+int fluid_mc_destroy(fluid_mc *mc);
+/* update_params + set_bounds (marching_cubes.rs:2099-2131, 2170-2174);
+ * params->grid_size must equal the handle's. */
+int fluid_mc_update_params(fluid_mc *mc, const GpuMcGridParams *params);
+/* update_aniso_params (marching_cubes.rs:2133-2146) */
+int fluid_mc_update_aniso_params(fluid_mc *mc, const GpuAnisoParams *params);
+/* update_container_geometry: the MC pass's own copy, with its clip margin
+ * (app.rs:1487-1501) */
+int fluid_mc_update_container_geometry(fluid_mc *mc, const GpuContainerGeometry *geom);
+/* generate, passes 0 and 1 (marching_cubes.rs:2402-2470): the anisotropy
+ * fit of the first num_particles sorted particles when enabled, then the
+ * density field. */
+int fluid_mc_generate(fluid_mc *mc, uint32_t num_particles);
+int fluid_mc_read_density(fluid_mc *mc, float *out, uint64_t count);
+int fluid_mc_read_aniso(fluid_mc *mc, GpuParticleAniso *out, uint32_t count);
+/* Replace the ellipsoid records (count = num_particles of the next
+ * generate) and run only the density pass on them. */
+int fluid_mc_density_from_aniso(fluid_mc *mc, const GpuParticleAniso *in, uint32_t count);
+/* Wall time of `iters` generate calls after `warmup` untimed ones (HIP
+ * events on the stream). */
+int fluid_mc_time(fluid_mc *mc, uint32_t num_particles, uint32_t warmup, uint32_t iters,
+                  float *ms);
+
 #ifdef __cplusplus
 }
 #endif

@@ -20,9 +20,10 @@
 // FP64 variant promotes their values.
 //
 // Parity status of each function is recorded next to it ("pinned by ...").
-// The file also restates the whitewater passes that consume the step's sorted
-// state (spray_emit.wgsl, spray_simulate.wgsl, src/simulation/spray.rs), in
-// f32 only.
+// The file also restates, in f32 only, the passes that consume the step's
+// sorted state: the whitewater passes (spray_emit.wgsl, spray_simulate.wgsl,
+// src/simulation/spray.rs) and the marching-cubes density field
+// (mc_anisotropy.wgsl, mc_density.wgsl).
 //
 // What only running the real wgpu pipeline could pin — naga/driver rounding
 // of the sums, sin/cos of the close-range fallback direction, the racing
@@ -924,6 +925,50 @@ float cap_volume_fraction(float t) {
 
 }  // namespace spray
 
+// ---------------------------------------------------------------------------
+// Marching-cubes density field: mc_anisotropy.wgsl and mc_density.wgsl
+// (f32, per voxel / per particle in the reference's loop order).
+namespace mc {
+
+constexpr float kPi = 3.14159265359f;
+
+struct M3 {
+  float m[3][3];  // m[column][row], as WGSL mat3x3
+};
+
+// jacobi_rotate (mc_anisotropy.wgsl:79-107)
+void jacobi_rotate(M3 &A, M3 &V, int p, int q, int k) {
+  const float apq = A.m[p][q];
+  if (std::fabs(apq) < 1e-12f) return;
+  const float app = A.m[p][p], aqq = A.m[q][q];
+  const float tau = (aqq - app) / (2.0f * apq);
+  const float sign_tau = tau < 0.0f ? -1.0f : 1.0f;
+  const float t = sign_tau / (std::fabs(tau) + std::sqrt(1.0f + tau * tau));
+  const float c = 1.0f / std::sqrt(1.0f + t * t);
+  const float s = t * c;
+  A.m[p][p] = app - t * apq;
+  A.m[q][q] = aqq + t * apq;
+  A.m[p][q] = 0.0f;
+  A.m[q][p] = 0.0f;
+  const float akp = A.m[k][p], akq = A.m[k][q];
+  A.m[k][p] = c * akp - s * akq;
+  A.m[p][k] = A.m[k][p];
+  A.m[k][q] = s * akp + c * akq;
+  A.m[q][k] = A.m[k][q];
+  for (int r = 0; r < 3; ++r) {
+    const float vp = V.m[p][r], vq = V.m[q][r];
+    V.m[p][r] = c * vp - s * vq;
+    V.m[q][r] = s * vp + c * vq;
+  }
+}
+
+float smoothstep(float e0, float e1, float x) {
+  const float t = clampv((x - e0) / (e1 - e0), 0.0f, 1.0f);
+  return t * t * (3.0f - 2.0f * t);
+}
+
+}  // namespace mc
+
 }  // namespace
 
 extern "C" {
@@ -1148,6 +1193,197 @@ void oracle_spray_simulate(GpuSprayParticle *ring, const void *sorted_particles,
       live[1u + kind] += 1u;
     }
   }
+}
+
+// mc_anisotropy.wgsl:110-276 for sorted particles 0..n-1: out = 12 floats
+// per particle (ParticleAniso).
+void oracle_mc_aniso(const void *sorted_particles, uint32_t n, const uint32_t *counts,
+                     const uint32_t *starts, const GpuGridParams *grid,
+                     const GpuAnisoParams *prm, float *out) {
+  using namespace mc;
+  const Part<float> *sp = (const Part<float> *)sorted_particles;
+  const spray::Grid G{*grid};
+  const float r_support = prm->support_radius, h = prm->h_mc;
+  const float peak = 315.0f / (64.0f * kPi * h * h * h);
+  const int cell_radius = (int)std::ceil(r_support * grid->inv_cell_size);
+  for (uint32_t i = 0; i < n; ++i) {
+    const V3<float> xi = P3(sp[i].pos);
+    float w_sum = 0.0f, n_count = 0.0f;
+    V3<float> m1 = mk(0.0f, 0.0f, 0.0f);
+    float cxx = 0.0f, cyy = 0.0f, czz = 0.0f, cxy = 0.0f, cxz = 0.0f, cyz = 0.0f;
+    int32_t cc[3];
+    G.cell(xi, cc);
+    for (int dz = -cell_radius; dz <= cell_radius; ++dz)
+      for (int dy = -cell_radius; dy <= cell_radius; ++dy)
+        for (int dx = -cell_radius; dx <= cell_radius; ++dx) {
+          const int32_t c[3] = {cc[0] + dx, cc[1] + dy, cc[2] + dz};
+          if (!G.valid(c)) continue;
+          const uint32_t idx = G.index(c), count = counts[idx];
+          if (count == 0u) continue;
+          const uint32_t start = starts[idx] - count;
+          for (uint32_t k = 0; k < count; ++k) {
+            const V3<float> d = sub(P3(sp[start + k].pos), xi);
+            const float r = length(d);
+            if (r < r_support) {
+              const float u = r / r_support;
+              const float w = 1.0f - u * u * u;
+              w_sum += w;
+              n_count += 1.0f;
+              m1 = add(m1, smul(w, d));
+              cxx += w * d.x * d.x;
+              cyy += w * d.y * d.y;
+              czz += w * d.z * d.z;
+              cxy += w * d.x * d.y;
+              cxz += w * d.x * d.z;
+              cyz += w * d.y * d.z;
+            }
+          }
+        }
+    const float t_n = smoothstep(4.0f, 12.0f, n_count);
+    const float s_eff = prm->strength * t_n;
+    const float inv_w = 1.0f / std::max(w_sum, 1e-9f);
+    const V3<float> mu = muls(m1, inv_w);
+    M3 A, V;
+    A.m[0][0] = cxx * inv_w - mu.x * mu.x;
+    A.m[0][1] = cxy * inv_w - mu.x * mu.y;
+    A.m[0][2] = cxz * inv_w - mu.x * mu.z;
+    A.m[1][0] = cxy * inv_w - mu.x * mu.y;
+    A.m[1][1] = cyy * inv_w - mu.y * mu.y;
+    A.m[1][2] = cyz * inv_w - mu.y * mu.z;
+    A.m[2][0] = cxz * inv_w - mu.x * mu.z;
+    A.m[2][1] = cyz * inv_w - mu.y * mu.z;
+    A.m[2][2] = czz * inv_w - mu.z * mu.z;
+    for (int c = 0; c < 3; ++c)
+      for (int r = 0; r < 3; ++r) V.m[c][r] = c == r ? 1.0f : 0.0f;
+    for (int sweep = 0; sweep < 5; ++sweep) {
+      jacobi_rotate(A, V, 0, 1, 2);
+      jacobi_rotate(A, V, 0, 2, 1);
+      jacobi_rotate(A, V, 1, 2, 0);
+    }
+    float e[3] = {A.m[0][0], A.m[1][1], A.m[2][2]};
+    V3<float> v[3] = {P3(V.m[0]), P3(V.m[1]), P3(V.m[2])};
+    if (e[0] < e[1]) { std::swap(e[0], e[1]); std::swap(v[0], v[1]); }
+    if (e[0] < e[2]) { std::swap(e[0], e[2]); std::swap(v[0], v[2]); }
+    if (e[1] < e[2]) { std::swap(e[1], e[2]); std::swap(v[1], v[2]); }
+    const float s1_raw = std::sqrt(std::max(e[0], 0.0f));
+    const float s3_raw = std::sqrt(std::max(e[2], 0.0f));
+    float flatness = 0.0f;
+    if (s1_raw > 1e-7f) {
+      const float roundness = s3_raw / s1_raw;
+      flatness = 1.0f - roundness * roundness;
+    }
+    V3<float> shift = smul(prm->lambda * s_eff * flatness, mu);
+    const float shift_len = length(shift);
+    if (shift_len > prm->max_shift) shift = muls(shift, prm->max_shift / shift_len);
+    const V3<float> center = add(xi, shift);
+    float a[3] = {1.0f, 1.0f, 1.0f};
+    const float s1 = s1_raw;
+    if (s1 > 1e-7f && s_eff > 0.0f) {
+      const float s_min = s1 / prm->kr;
+      const float s2 = std::max(std::sqrt(std::max(e[1], 0.0f)), s_min);
+      const float s3 = std::max(s3_raw, s_min);
+      const float norm = std::pow(s1 * s2 * s3, 1.0f / 3.0f);
+      const float sv[3] = {s1, s2, s3};
+      for (int k = 0; k < 3; ++k) {
+        a[k] = sv[k] / norm;
+        a[k] = std::pow(a[k], s_eff);
+        a[k] = clampv(a[k], 0.25f, prm->max_stretch);
+      }
+    }
+    const float ir[3] = {1.0f / (a[0] * h), 1.0f / (a[1] * h), 1.0f / (a[2] * h)};
+    auto gent = [&](int r, int c) {
+      const float *vr0 = &v[0].x, *vr1 = &v[1].x, *vr2 = &v[2].x;
+      return ir[0] * vr0[r] * vr0[c] + ir[1] * vr1[r] * vr1[c] + ir[2] * vr2[r] * vr2[c];
+    };
+    const float amplitude = peak / std::max(a[0] * a[1] * a[2], 1.0f);
+    const float reach = h * std::max(a[0], std::max(a[1], a[2]));
+    float *o = out + 12 * (size_t)i;
+    const float rec[12] = {gent(0, 0), gent(0, 1), gent(0, 2), center.x,
+                           gent(1, 1), gent(1, 2), gent(2, 2), center.y,
+                           center.z,   reach,      amplitude,  0.0f};
+    std::memcpy(o, rec, sizeof rec);
+  }
+}
+
+// mc_density.wgsl:133-230 over the G^3 voxels (x fastest); aniso = 12
+// floats per sorted particle (read when aniso_params->enabled).
+void oracle_mc_density(const void *sorted_particles, const uint32_t *counts,
+                       const uint32_t *starts, const GpuGridParams *grid,
+                       const GpuMcGridParams *prm, const GpuAnisoParams *aprm,
+                       const GpuContainerGeometry *cont, const float *aniso, float *field) {
+  const Part<float> *sp = (const Part<float> *)sorted_particles;
+  const spray::Grid G{*grid};
+  const uint32_t N = prm->grid_size;
+  const float h = prm->kernel_radius, h_sq = h * h;
+  const bool aniso_on = aprm->enabled != 0u;
+  const float coeff = 315.0f / (64.0f * 3.14159265359f * std::pow(h, 9.0f));
+  float search_reach = h;
+  if (aniso_on) search_reach = h * aprm->max_stretch + aprm->max_shift;
+  const int cell_radius = (int)std::ceil(search_reach * grid->inv_cell_size);
+#pragma omp parallel for schedule(dynamic, 4)
+  for (int64_t vz = 0; vz < (int64_t)N; ++vz)
+    for (uint32_t vy = 0; vy < N; ++vy)
+      for (uint32_t vx = 0; vx < N; ++vx) {
+        const float cs = prm->cell_size;
+        const V3<float> wp = mk(prm->grid_min[0] + ((float)vx + 0.5f) * cs,
+                                prm->grid_min[1] + ((float)vy + 0.5f) * cs,
+                                prm->grid_min[2] + ((float)vz + 0.5f) * cs);
+        int32_t cc[3];
+        G.cell(wp, cc);
+        float density = 0.0f;
+        for (int dz = -cell_radius; dz <= cell_radius; ++dz)
+          for (int dy = -cell_radius; dy <= cell_radius; ++dy)
+            for (int dx = -cell_radius; dx <= cell_radius; ++dx) {
+              const int32_t c[3] = {cc[0] + dx, cc[1] + dy, cc[2] + dz};
+              if (!G.valid(c)) continue;
+              const uint32_t idx = G.index(c), count = counts[idx];
+              if (count == 0u) continue;
+              const uint32_t start = starts[idx] - count;
+              for (uint32_t k = 0; k < count; ++k) {
+                const uint32_t j = start + k;
+                if (aniso_on) {
+                  const float *an = aniso + 12 * (size_t)j;
+                  const V3<float> diff = sub(wp, mk(an[3], an[7], an[8]));
+                  const float r_sq = dot(diff, diff);
+                  const float reach = an[9];
+                  if (r_sq < reach * reach) {
+                    const V3<float> gd = mk(an[0] * diff.x + an[1] * diff.y + an[2] * diff.z,
+                                            an[1] * diff.x + an[4] * diff.y + an[5] * diff.z,
+                                            an[2] * diff.x + an[5] * diff.y + an[6] * diff.z);
+                    const float q_sq = dot(gd, gd);
+                    if (q_sq < 1.0f) {
+                      const float f = 1.0f - q_sq;
+                      density += an[10] * f * f * f;
+                    }
+                  }
+                } else {
+                  const V3<float> diff = sub(wp, P3(sp[j].pos));
+                  const float r_sq = dot(diff, diff);
+                  if (r_sq < h_sq) {
+                    const float d = h_sq - r_sq;
+                    density += coeff * d * d * d;
+                  }
+                }
+              }
+            }
+        const V3<float> local = spray::world_to_local(*cont, wp);
+        float out;
+        const bool inside = local.x >= -cont->half_width && local.x <= cont->half_width &&
+                            local.y >= -cont->half_height && local.y <= cont->half_height &&
+                            local.z >= -cont->half_depth && local.z <= cont->half_depth;
+        if (cont->clip_enabled != 0u && !inside) {
+          out = -1.0f;
+        } else {
+          const float dist[5] = {local.x + cont->half_width, cont->half_width - local.x,
+                                 local.y + cont->half_height, local.z + cont->half_depth,
+                                 cont->half_depth - local.z};
+          float gamma = 1.0f;
+          for (int k = 0; k < 5; ++k)
+            if (dist[k] < h) gamma *= 0.5f + 0.5f * clampv(dist[k] / h, 0.0f, 1.0f);
+          out = density / std::max(gamma, 0.15f);
+        }
+        field[(size_t)vx + (size_t)vy * N + (size_t)vz * N * N] = out;
+      }
 }
 
 // update_auto_limit (src/simulation/spray.rs:40-51)

@@ -48,6 +48,8 @@ def load():
         lib.oracle_spray_spawn.restype = C.c_uint32
         lib.oracle_spray_spawn.argtypes = [P, C.c_uint32, P, P, P, C.c_uint32, P]
         lib.oracle_spray_simulate.argtypes = [P, P, P, P, P, P, P, P, P]
+        lib.oracle_mc_aniso.argtypes = [P, C.c_uint32, P, P, P, P, P]
+        lib.oracle_mc_density.argtypes = [P, P, P, P, P, P, P, P, P]
         lib.oracle_spray_update_limit.restype = C.c_float
         lib.oracle_spray_update_limit.argtypes = [C.c_float] * 3
         lib.oracle_spray_hash_float.restype = C.c_float
@@ -251,4 +253,36 @@ def spray_simulate(ring: np.ndarray, sorted_particles: np.ndarray, counts: np.nd
 
 def spray_update_limit(current: float, frame_max: float, floor: float) -> float:
     return float(load().oracle_spray_update_limit(current, frame_max, floor))
+
+
+# ---------------------------------------------------------------------------
+# Marching-cubes density field (mc_anisotropy.wgsl, mc_density.wgsl).
+
+def mc_aniso(sorted_particles: np.ndarray, counts: np.ndarray, starts: np.ndarray, grid,
+             aniso_params) -> np.ndarray:
+    """(N, 12) float32 ellipsoid records of the sorted particles."""
+    lib = load()
+    sp = np.ascontiguousarray(sorted_particles, dtype=np.float32).reshape(-1, 16)
+    cnt = np.ascontiguousarray(counts, dtype=np.uint32)
+    st = np.ascontiguousarray(starts, dtype=np.uint32)
+    out = np.zeros((len(sp), 12), np.float32)
+    lib.oracle_mc_aniso(_ptr(sp), len(sp), _ptr(cnt), _ptr(st), C.byref(grid),
+                        C.byref(aniso_params), _ptr(out))
+    return out
+
+
+def mc_density(sorted_particles: np.ndarray, counts: np.ndarray, starts: np.ndarray, grid,
+               mc_params, aniso_params, container, aniso: np.ndarray | None = None) -> np.ndarray:
+    """The (G, G, G) float32 field indexed [z, y, x]."""
+    lib = load()
+    sp = np.ascontiguousarray(sorted_particles, dtype=np.float32).reshape(-1, 16)
+    cnt = np.ascontiguousarray(counts, dtype=np.uint32)
+    st = np.ascontiguousarray(starts, dtype=np.uint32)
+    an = np.zeros((max(len(sp), 1), 12), np.float32) if aniso is None else \
+        np.ascontiguousarray(aniso, dtype=np.float32)
+    G = mc_params.grid_size
+    out = np.zeros((G, G, G), np.float32)
+    lib.oracle_mc_density(_ptr(sp), _ptr(cnt), _ptr(st), C.byref(grid), C.byref(mc_params),
+                          C.byref(aniso_params), C.byref(container), _ptr(an), _ptr(out))
+    return out
 

@@ -60,13 +60,15 @@ def c_layout(expr_list):
     ("SphParticle3D", 64), ("GpuSphParams3D", 64), ("GpuGridParams", 48),
     ("GpuContainerGeometry", 128), ("GpuGravity", 16), ("GpuMouseForce", 48),
     ("GpuRigidBody", 96), ("GpuRigidBodyAccum", 32), ("PredictedState", 32),
-    ("GpuSprayParams", 80), ("GpuSprayParticle", 48)])
+    ("GpuSprayParams", 80), ("GpuSprayParticle", 48), ("GpuMcGridParams", 48),
+    ("GpuAnisoParams", 32), ("GpuParticleAniso", 48)])
 def test_struct_sizes_match_reference_layouts(pkg, name, size):
     cls = getattr(pkg.abi, name)
     assert C.sizeof(cls) == size
     assert c_layout([f"sizeof({name})"]) == [size]
     fields = [f[0] for f in cls._fields_]
-    offs = c_layout([f"offsetof({name}, {f})" for f in fields])
+    # a Python keyword field carries a trailing underscore (lambda_)
+    offs = c_layout([f"offsetof({name}, {f.rstrip('_') or f})" for f in fields])
     assert offs == [getattr(cls, f).offset for f in fields]
 
 
