@@ -4,8 +4,8 @@ Headline: particle-updates/s of the 50,000-particle dam break (lattice 0.025,
 h 0.08), hipGraph-launched steps with all state resident in HBM.  The line
 also carries the 1,048,576-particle block throughput, per-stage times from
 HIP events on the library's stream, the roofline of the dominant kernel, the
-whitewater passes' frame time and the CPU oracle's rate on this host as a
-reported baseline.
+whitewater passes' frame time, the marching-cubes field's time and the CPU
+oracle's rate on this host as a reported baseline.
 
     python bench.py [--gpus 1] [--steps 1000] [--warmup 50]
 """
@@ -169,6 +169,27 @@ def run_spray(pkg, frames_warm: int = 60, frames: int = 20, substeps: int = 2):
             "sph_ms_per_step": round(t_sph, 4), "live": st["live"],
             "spawned_last_frame": st["emitted"],
             "ceilings": [round(st["ta_limit"], 4), round(st["wc_limit"], 4)]}
+
+
+def run_mc(pkg, grid_size: int = 200, iters: int = 10):
+    """Marching-cubes density field of the 50k dam break after 60 steps
+    (MarchingCubesRenderer::generate passes 0 and 1 at the app's default
+    High 200^3 grid, anisotropy on, h_mc = h, bounds = container + margin),
+    timed over `iters` generate calls; then the isotropic field alone."""
+    scene = pkg.scenes.dam_break_50k()
+    sim = pkg.make_sim(scene)
+    sim.step(60)
+    mc = pkg.MarchingCubesField(sim, grid_size)
+    ms_aniso = mc.time(2, iters) / iters
+    h = sim.sph_params().kernel_radius
+    mc.update_aniso_params(pkg.abi.default_aniso_params(h, enabled=False))
+    ms_iso = mc.time(2, iters) / iters
+    mc.close()
+    sim.close()
+    vox = grid_size ** 3
+    return {"workload": f"dam_break_50k, {grid_size}^3 voxels", "iters": iters,
+            "ms_aniso": round(ms_aniso, 4), "ms_isotropic": round(ms_iso, 4),
+            "voxels_per_s_aniso": round(vox / (ms_aniso / 1e3), 1)}
 
 
 def run_adjoint(pkg, orc, steps: int = 64, every: int = 16, fd_coords: int = 64,
@@ -404,6 +425,7 @@ def main() -> int:
     ap.add_argument("--batch-scenes", type=int, default=256)
     ap.add_argument("--no-adjoint", action="store_true", help="skip the differentiable rollout")
     ap.add_argument("--no-spray", action="store_true", help="skip the whitewater passes")
+    ap.add_argument("--no-mc", action="store_true", help="skip the marching-cubes field")
     ap.add_argument("--fd-coords", type=int, default=64)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a 256 MiB device copy (scales the FETCH_SIZE/WRITE_SIZE "
@@ -476,6 +498,9 @@ def main() -> int:
     if not args.no_spray:
         progress("spray")
         line["spray"] = run_spray(pkg)
+    if not args.no_mc:
+        progress("mc field")
+        line["mc_field"] = run_mc(pkg)
     if not args.no_batch:
         progress("batched")
         line["batched"] = run_batched(pkg, args.batch_scenes)

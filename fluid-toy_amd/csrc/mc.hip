@@ -271,20 +271,54 @@ __global__ __launch_bounds__(256) void k_mc_density(Uni un, Bufs b, McLaunch m,
   const int cx = cell_coord(wx, U->ox, U->inv_cell);
   const int cy = cell_coord(wy, U->oy, U->inv_cell);
   const int cz = cell_coord(wz, U->oz, U->inv_cell);
-  // union of the block's stencils, clipped to the grid
-  const int x0 = max(wave_min_i(cx) - R, 0), x1 = min(wave_max_i(cx) + R, gx - 1);
-  const int y0 = max(wave_min_i(cy) - R, 0), y1 = min(wave_max_i(cy) + R, gy - 1);
-  const int z0 = max(wave_min_i(cz) - R, 0), z1 = min(wave_max_i(cz) + R, gz - 1);
+  // union of the block's stencils, clipped to the grid (wave-uniform: read
+  // from lane 0 so the walk below runs on scalar registers and loads)
+  const int x0 = __builtin_amdgcn_readfirstlane(max(wave_min_i(cx) - R, 0));
+  const int x1 = __builtin_amdgcn_readfirstlane(min(wave_max_i(cx) + R, gx - 1));
+  const int y0 = __builtin_amdgcn_readfirstlane(max(wave_min_i(cy) - R, 0));
+  const int y1 = __builtin_amdgcn_readfirstlane(min(wave_max_i(cy) + R, gy - 1));
+  const int z0 = __builtin_amdgcn_readfirstlane(max(wave_min_i(cz) - R, 0));
+  const int z1 = __builtin_amdgcn_readfirstlane(min(wave_max_i(cz) + R, gz - 1));
   const float h = m.g.kernel_radius, h_sq = h * h;
   float density = 0.0f;
   // this voxel's own stencil (:157-166), clipped to the grid
   const int ox0 = max(cx - R, 0), ox1 = min(cx + R, gx - 1);
+  // Cells whose box is farther than the largest splat reach (plus the
+  // centre shift) from every voxel of the block hold no contributing
+  // particle and are skipped; boundary cells extend to infinity (the build
+  // clamps particles outside the grid into them).  The bound carries a
+  // 1e-4 relative margin over f32 rounding.
+  const float cell = 1.0f / U->inv_cell;
+  const float L = (ANISO ? h * m.a.max_stretch + m.a.max_shift : h) * 1.0001f + 1e-6f;
+  const uint32_t bx = blockIdx.x * 4u, by = blockIdx.y * 4u, bz = (blockIdx.z * 4u + wave) * 4u;
+  const float bx_lo = m.g.grid_min[0] + ((float)bx + 0.5f) * cs, bx_hi = bx_lo + 3.0f * cs;
+  const float by_lo = m.g.grid_min[1] + ((float)by + 0.5f) * cs, by_hi = by_lo + 3.0f * cs;
+  const float bz_lo = m.g.grid_min[2] + ((float)bz + 0.5f) * cs, bz_hi = bz_lo + 3.0f * cs;
+  auto gap = [&](int k, int kmax, float o, float lo, float hi) {
+    const float c_lo = o + (float)k * cell, c_hi = c_lo + cell;
+    if (k > 0 && hi < c_lo) return c_lo - hi;
+    if (k < kmax && lo > c_hi) return lo - c_hi;
+    return 0.0f;
+  };
   if (x0 <= x1) {
     for (int z = z0; z <= z1; ++z) {
+      const float gz2 = gap(z, gz - 1, U->oz, bz_lo, bz_hi);
       for (int y = y0; y <= y1; ++y) {
+        const float gy2 = gap(y, gy - 1, U->oy, by_lo, by_hi);
+        // (the builtin moves 32-bit integers: pass the float's bits)
+        const float rem = __int_as_float(
+            __builtin_amdgcn_readfirstlane(__float_as_int(L * L - gz2 * gz2 - gy2 * gy2)));
+        if (!(rem > 0.0f)) continue;
+        const float allow = sqrtf(rem);
+        const int xa = __builtin_amdgcn_readfirstlane(
+            max(x0, (int)floorf((bx_lo - allow - U->ox) * U->inv_cell)));
+        const int xb = __builtin_amdgcn_readfirstlane(
+            min(x1, (int)floorf((bx_hi + allow - U->ox) * U->inv_cell)));
+        if (xa > xb) continue;
         const uint32_t row = (uint32_t)y * U->gx + (uint32_t)z * U->gxy;
-        const uint32_t c0 = row + (uint32_t)x0, c1 = row + (uint32_t)x1;
-        const uint32_t end = b.starts[c1];
+        const uint32_t c0 = row + (uint32_t)xa, c1 = row + (uint32_t)xb;
+        const uint32_t begin = __builtin_amdgcn_readfirstlane(b.starts[c0] - b.counts[c0]);
+        const uint32_t end = __builtin_amdgcn_readfirstlane(b.starts[c1]);
         // the slots of this row that lie in this voxel's stencil
         uint32_t lo = 0u, hi = 0u;
         if (ox0 <= ox1 && y >= cy - R && y <= cy + R && z >= cz - R && z <= cz + R) {
@@ -292,15 +326,15 @@ __global__ __launch_bounds__(256) void k_mc_density(Uni un, Bufs b, McLaunch m,
           lo = b.starts[a0] - b.counts[a0];
           hi = b.starts[a1];
         }
-        for (uint32_t j = b.starts[c0] - b.counts[c0]; j < end; ++j) {
-          if (j < lo || j >= hi) continue;
-          if (ANISO) {
-            const float4 q0 = aniso[3 * (size_t)j], q1 = aniso[3 * (size_t)j + 1];
-            const float4 q2 = aniso[3 * (size_t)j + 2];
+        // four candidates per trip, their records loaded together (scalar
+        // loads: the index is wave-uniform); terms are added in slot order
+        uint32_t j = begin;
+        if (ANISO) {
+          auto term = [&](uint32_t jj, float4 q0, float4 q1, float4 q2) {
             const float dx = wx - q0.w, dy = wy - q1.w, dz = wz - q2.x;
             const float r_sq = dot3(dx, dy, dz, dx, dy, dz);
             const float reach = q2.y;
-            if (r_sq < reach * reach) {
+            if (jj >= lo && jj < hi && r_sq < reach * reach) {
               const float gdx = q0.x * dx + q0.y * dy + q0.z * dz;
               const float gdy = q0.y * dx + q1.x * dy + q1.y * dz;
               const float gdz = q0.z * dx + q1.y * dy + q1.z * dz;
@@ -310,15 +344,33 @@ __global__ __launch_bounds__(256) void k_mc_density(Uni un, Bufs b, McLaunch m,
                 density += q2.z * f * f * f;
               }
             }
-          } else {
-            const float4 p = b.spos[j];
+          };
+          for (; j + 4 <= end; j += 4) {
+            float4 q[12];
+#pragma unroll
+            for (int t = 0; t < 12; ++t) q[t] = aniso[3 * (size_t)j + t];
+#pragma unroll
+            for (int t = 0; t < 4; ++t) term(j + t, q[3 * t], q[3 * t + 1], q[3 * t + 2]);
+          }
+          for (; j < end; ++j)
+            term(j, aniso[3 * (size_t)j], aniso[3 * (size_t)j + 1], aniso[3 * (size_t)j + 2]);
+        } else {
+          auto term = [&](uint32_t jj, float4 p) {
             const float dx = wx - p.x, dy = wy - p.y, dz = wz - p.z;
             const float r_sq = dot3(dx, dy, dz, dx, dy, dz);
-            if (r_sq < h_sq) {
+            if (jj >= lo && jj < hi && r_sq < h_sq) {
               const float diff = h_sq - r_sq;
               density += m.coeff * diff * diff * diff;
             }
+          };
+          for (; j + 4 <= end; j += 4) {
+            float4 q[4];
+#pragma unroll
+            for (int t = 0; t < 4; ++t) q[t] = b.spos[j + t];
+#pragma unroll
+            for (int t = 0; t < 4; ++t) term(j + t, q[t]);
           }
+          for (; j < end; ++j) term(j, b.spos[j]);
         }
       }
     }
