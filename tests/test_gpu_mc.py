@@ -5,8 +5,11 @@ Bars: the density field is the oracle's to the bit, isotropic or from the
 GPU's own ellipsoid records (each voxel sums the same candidates in the
 same order with the same f32 operations); the ellipsoid records (their
 covariance sums are split over 16 lanes) agree to 1e-4 of the kernel scale
-1/h in G, 1e-5 h in centre and reach, 1e-4 relative in amplitude (observed:
-3e-6, 2e-6, 4e-6, 6e-7).
+1/h in G, 1e-5 h in reach, 1e-4 relative in amplitude, and 5e-5 h in the
+centre (1e-4 of the largest centre shift 0.4 h: the shift scales with the
+flatness 1 - (s3/s1)^2, an eigenvalue ratio that carries the sums' rounding
+into it).  Observed at 8k: 3e-6, 4e-6, 6e-7, 2e-6; at 50k: 2e-5 (G), 7e-6
+(centre).
 """
 import numpy as np
 import pytest
@@ -80,7 +83,7 @@ def test_aniso_records_and_field(pkg, orc):
     ea = (np.abs(rec[:, 10] - want[:, 10]) / want[:, 10]).max()
     print(f"  G {eg:.2e} (x 1/h)  centre {ec:.2e} h  reach {er:.2e} h  amplitude {ea:.2e}",
           flush=True)
-    assert eg < 1e-4 and ec < 1e-5 and er < 1e-5 and ea < 1e-4
+    assert eg < 1e-4 and ec < 5e-5 and er < 1e-5 and ea < 1e-4
     # the field from the GPU's own records, to the bit
     got = mc.read_density()
     field = orc.mc_density(recs, cnt, st, grid, g, a, cont, rec)
@@ -132,5 +135,38 @@ def test_mc_guards(pkg):
     with pytest.raises(pkg.abi.FluidError):
         mc.update_params(g)
     assert lib.fluid_destroy(sim._h) == pkg.abi.FLUID_ERR_STATE
+    mc.close()
+    sim.close()
+
+
+def test_mc_full_size_50k(pkg, orc):
+    """The 50k dam break after 60 steps (the bench workload) on a 100^3
+    grid: isotropic field and the field from the GPU's records to the bit,
+    records within the bars above."""
+    sim = pkg.make_sim(pkg.scenes.dam_break_50k())
+    sim.step(60)
+    G = 100
+    mc = pkg.MarchingCubesField(sim, G)
+    h = sim.sph_params().kernel_radius
+    g, cont = _default_params(pkg, sim, G)
+    mc.update_params(g)
+    recs, cnt, st, grid = _state(sim)
+    print("case: MC full size, isotropic", flush=True)
+    iso = pkg.abi.default_aniso_params(h, enabled=False)
+    mc.update_aniso_params(iso)
+    mc.generate()
+    assert np.array_equal(mc.read_density(), orc.mc_density(recs, cnt, st, grid, g, iso, cont))
+    print("case: MC full size, anisotropic", flush=True)
+    a = pkg.abi.default_aniso_params(h)
+    mc.update_aniso_params(a)
+    mc.generate()
+    rec = mc.read_aniso()
+    want = orc.mc_aniso(recs, cnt, st, grid, a)
+    eg = np.abs(rec[:, [0, 1, 2, 4, 5, 6]] - want[:, [0, 1, 2, 4, 5, 6]]).max() * h
+    ec = np.abs(rec[:, [3, 7, 8]] - want[:, [3, 7, 8]]).max() / h
+    print(f"  G {eg:.2e} (x 1/h)  centre {ec:.2e} h", flush=True)
+    assert eg < 1e-4 and ec < 5e-5
+    field = orc.mc_density(recs, cnt, st, grid, g, a, cont, rec)
+    assert np.array_equal(mc.read_density(), field)
     mc.close()
     sim.close()

@@ -228,3 +228,33 @@ def test_spray_reset_guards_and_no_step(pkg):
         pkg.abi.FLUID_ERR_INVALID_ARG
     batch.close()
     sim.close()
+
+
+def test_spray_full_size_50k(pkg, orc):
+    """The 50k dam break after 60 steps (the bench workload), default
+    SprayConfig with a 100k ring: one frame against the oracle, teacher-
+    forced as above, at full size."""
+    sim = pkg.make_sim(pkg.scenes.dam_break_50k())
+    sim.step(60)
+    n = sim.num_particles()
+    cap = 100_000
+    prm = pkg.abi.default_spray_params(dt=2 * 0.008, max_particles=cap, num_sph_particles=n,
+                                       frame_count=3)
+    spray = pkg.SpraySystem(sim, cap, prm)
+    recs, cnt, st, grid, sph, cont = _sim_state(sim)
+    s0 = spray.state()
+    ring0 = spray.read_particles()
+    print(f"case: spray full size n={n} cap={cap}", flush=True)
+    spray.step(n)
+    pot, ne = spray.read_emit(n)
+    pot_o, ne_o, fm_o = orc.spray_potentials(recs, cnt, st, grid, sph, prm,
+                                             s0["ta_limit"], s0["wc_limit"])
+    assert ne_o.sum() > 100
+    _check_emit(pot, ne, pot_o, ne_o, "50k")
+    ring_o = ring0.copy()
+    head_o = orc.spray_spawn(recs, sph, prm, ne, s0["write_head"], ring_o)
+    orc.spray_simulate(ring_o, recs, cnt, st, grid, sph, cont, prm)
+    assert spray.state()["write_head"] == head_o
+    _check_ring(spray.read_particles(), ring_o, "50k", recs, prm.dt)
+    spray.close()
+    sim.close()
