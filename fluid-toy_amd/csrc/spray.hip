@@ -52,7 +52,7 @@ constexpr float kAutoWcFloor = 0.05f;
 // in the fluid meets hundreds of candidates, and its lanes' dependent
 // gathers are the pass's critical path.
 constexpr int kEmitLanes = 2 * kListLanes;
-constexpr int kSimLanes = 16;
+constexpr int kSimLanes = 32;  // 8 / 16 / 32 / 64 lanes: 97 / 72 / 67 / 66 us per 50k frame
 
 inline uint32_t blocks_for(uint64_t threads, uint32_t per) {
   return (uint32_t)((threads + per - 1) / per);
@@ -116,6 +116,50 @@ __device__ __forceinline__ float cap_volume_fraction(float t) {
 __device__ __forceinline__ float wave_max(float v) {
   for (int o = 32; o >= 1; o >>= 1) v = fmaxf(v, __shfl_xor(v, o));
   return v;
+}
+
+// The 27-cell stencil as nine x-rows in sweep27's order (z, y, then slot),
+// with all nine rows' slot ranges read up front: a diffuse particle deep in
+// the fluid walks hundreds of candidates, and with few live slots in a
+// launch the row-by-row bound reads would sit on its critical path.  Lane
+// `sub` of L takes every L-th candidate of each row, four in flight.
+template <int L, class F>
+__device__ __forceinline__ void sweep27_rows_first(const DevUniforms *__restrict__ U,
+                                                   const uint32_t *__restrict__ counts,
+                                                   const uint32_t *__restrict__ starts,
+                                                   const float4 *__restrict__ pts, int cx,
+                                                   int cy, int cz, uint32_t sub, F &&f) {
+  const int gx = (int)U->gx, gy = (int)U->gy, gz = (int)U->gz;
+  cx = min(max(cx, -2), gx + 1);
+  cy = min(max(cy, -2), gy + 1);
+  cz = min(max(cz, -2), gz + 1);
+  const int x0 = max(cx - 1, 0), x1 = min(cx + 1, gx - 1);
+  if (x0 > x1) return;
+  uint32_t beg[9], end[9];
+#pragma unroll
+  for (int k = 0; k < 9; ++k) {
+    const int z = cz + k / 3 - 1, y = cy + k % 3 - 1;
+    beg[k] = end[k] = 0u;
+    if (z >= 0 && z < gz && y >= 0 && y < gy) {
+      const uint32_t row = U->cell_offset + (uint32_t)y * U->gx + (uint32_t)z * U->gxy;
+      const uint32_t c0 = row + (uint32_t)x0, c1 = row + (uint32_t)x1;
+      beg[k] = starts[c0] - counts[c0];
+      end[k] = starts[c1];
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < 9; ++k) {
+    uint32_t j = beg[k] + sub;
+    const uint32_t e = end[k];
+    for (; j + 3 * L < e; j += 4 * L) {
+      const float4 p0 = pts[j], p1 = pts[j + L], p2 = pts[j + 2 * L], p3 = pts[j + 3 * L];
+      f(j, p0);
+      f(j + L, p1);
+      f(j + 2 * L, p2);
+      f(j + 3 * L, p3);
+    }
+    for (; j < e; j += L) f(j, pts[j]);
+  }
 }
 
 template <int L>
@@ -338,6 +382,12 @@ __global__ __launch_bounds__(256) void k_spray_simulate(Uni un, Bufs b, SprayBuf
     q2 = rec[2];  // kind, age, pad
   }
   const bool alive = in && !(q0.w <= 0.0f);  // :154-156
+  // a block without a live slot only reports zero counts
+  __shared__ uint32_t s_live[4][4];
+  if (!__syncthreads_or(alive)) {
+    if (threadIdx.x < 4u) sb.live_part[4 * (size_t)blockIdx.x + threadIdx.x] = 0u;
+    return;
+  }
   const float h = U->h, h2 = U->h2;
   uint32_t cnt = 0u;
   float ws = 0.f, ax = 0.f, ay = 0.f, az = 0.f;
@@ -346,8 +396,8 @@ __global__ __launch_bounds__(256) void k_spray_simulate(Uni un, Bufs b, SprayBuf
     const int cy = cell_coord(q0.y, U->oy, U->inv_cell);
     const int cz = cell_coord(q0.z, U->oz, U->inv_cell);
     const float inv_h = 1.0f / h;
-    sweep27<kSimLanes, false>(U, b.counts, b.starts, b.spos, cx, cy, cz, sub,
-                              [&](uint32_t j, float4 pj, bool) {
+    sweep27_rows_first<kSimLanes>(U, b.counts, b.starts, b.spos, cx, cy, cz, sub,
+                                  [&](uint32_t j, float4 pj) {
       // loaded whether or not the pair is in range, so it need not wait for
       // the test (the gathers of a trip go out together)
       const float4 vj = b.svel[j];
@@ -430,7 +480,6 @@ __global__ __launch_bounds__(256) void k_spray_simulate(Uni un, Bufs b, SprayBuf
   }
   // live counts by kind (:275-278): per block, summed on demand by
   // k_spray_live_sum (one partial per block, no atomics on shared addresses)
-  __shared__ uint32_t s_live[4][4];
   const bool live = alive && sub == 0 && life > 0.0f;
   const uint32_t c_all = (uint32_t)__popcll(__ballot(live));
   const uint32_t c_spray = (uint32_t)__popcll(__ballot(live && kind == kKindSpray));
