@@ -170,3 +170,42 @@ def test_mc_full_size_50k(pkg, orc):
     assert np.array_equal(mc.read_density(), field)
     mc.close()
     sim.close()
+
+
+def test_mc_and_spray_on_an_empty_then_spawned_simulation(pkg, orc):
+    """No particles: the field is zero (with the wall factor, still zero)
+    and nothing spawns; after a spawn and a step both passes see the new
+    particles, the field again the oracle's to the bit; a 1-voxel grid."""
+    scene = pkg.scenes.small_block(1, 1, 1)
+    scene.particles = scene.particles[:0]
+    sim = pkg.make_sim(scene, max_particles=64)
+    sim.step(1)
+    mc = pkg.MarchingCubesField(sim, 1)
+    mc.generate()
+    assert mc.read_density().shape == (1, 1, 1) and mc.read_density()[0, 0, 0] == 0.0
+    mc.close()
+    cap = 32
+    prm = pkg.abi.default_spray_params(dt=0.016, max_particles=cap, num_sph_particles=0)
+    spray = pkg.SpraySystem(sim, cap, prm)
+    spray.step(0)
+    assert spray.read_stats()[0] == 0 and spray.state()["write_head"] == 0
+    assert sim.spawn_particles([0.0, 0.5, 0.0], 40, 0.05) == 40
+    sim.step(2)
+    prm.num_sph_particles = sim.num_particles()
+    spray.update_params(prm)
+    spray.step(sim.num_particles())
+    pot, ne = spray.read_emit(sim.num_particles())
+    assert pot.shape == (40, 4)
+    spray.close()
+    mc = pkg.MarchingCubesField(sim, 24)
+    h = sim.sph_params().kernel_radius
+    a = pkg.abi.default_aniso_params(h, enabled=False)
+    mc.update_aniso_params(a)
+    g, cont = _default_params(pkg, sim, 24)
+    mc.update_params(g)
+    mc.generate()
+    recs, cnt, st, grid = _state(sim)
+    want = orc.mc_density(recs, cnt, st, grid, g, a, cont)
+    assert (want > 0).any() and np.array_equal(mc.read_density(), want)
+    mc.close()
+    sim.close()
