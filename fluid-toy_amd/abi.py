@@ -295,6 +295,8 @@ FLUID_WALK_16 = 4
 FLUID_WALK_32 = 8
 FLUID_SOLVE_WALK_X2 = 16
 FLUID_SOLVE_WALK_X4 = 32
+FLUID_SOLVE_FIRST_X2 = 64
+FLUID_SOLVE_FIRST_X4 = 128
 
 # Gradient slots per scene written by fluid_backward_rollout.
 FLUID_GRAD_PARAMS = 5

@@ -199,6 +199,9 @@ struct SweepCfg {
   int solve_walk;   // 0 / 1 / 2 (global sweeps, skin lists on, 2 <= solve
                     // lanes <= 8): the later PCISPH iterations walk the skin
                     // lists with 1 / 2 / 4 walkers per list lane
+  int solve_first;  // 0 / 1 / 2: the first PCISPH iteration sweeps with 1 / 2 / 4
+                    // lanes per skin-list lane (skin lists on, global sweeps,
+                    // solve_lanes <= 8)
 };
 
 // Host-side launchers (sph_kernels.hip).  All enqueue on `st`.

@@ -91,6 +91,7 @@ struct fluid_sim {
   int graph_lanes = 0, graph_solve_lanes = 0, graph_xcd = 0, graph_tiled = 0, graph_wgs = 0;
   uint32_t graph_tile_w = 0;
   int graph_fuse_xf = 0, graph_solve_list = 0, graph_wide = 0, graph_solve_walk = 0;
+  int graph_solve_first = 0;
   uint32_t num_cu = 256;
   uint32_t *d_scene_first = nullptr;  // first tile id of each scene (+ total)
   int cursor = -1;  // last completed stage of a partial step, -1 = none
@@ -121,7 +122,7 @@ SweepCfg pick_cfg(uint32_t) {
   // Default before tuning: 8 lanes per particle, the fastest setting the
   // autotuner found at every size measured (8k to 1M particles, compressed
   // and relaxed states); global-memory sweeps.
-  return SweepCfg{8, 8, 0, 0, 1024, 1, 1, 0, 0};
+  return SweepCfg{8, 8, 0, 0, 1024, 1, 1, 0, 0, 0};
 }
 
 // Per-scene tile counts for tiles of w cells along x; uploads the scene
@@ -147,13 +148,14 @@ int setup_tiles(fluid_sim *s, uint32_t w) {
 void update_cfg(fluid_sim *s) {
   if (s->tuned) return;
   const int wgs = s->cfg.tile_wgs, fuse = s->cfg.fuse_xf, sl = s->cfg.solve_list;
-  const int wide = s->cfg.wide_walk, sw = s->cfg.solve_walk;
+  const int wide = s->cfg.wide_walk, sw = s->cfg.solve_walk, sf = s->cfg.solve_first;
   s->cfg = pick_cfg(s->n);
   s->cfg.tile_wgs = wgs;
   s->cfg.fuse_xf = fuse;
   s->cfg.solve_list = sl;
   s->cfg.wide_walk = wide;
   s->cfg.solve_walk = sw;
+  s->cfg.solve_first = sf;
 }
 
 int push_uniforms(fluid_sim *s) {
@@ -251,7 +253,8 @@ int ensure_graph(fluid_sim *s) {
       s->graph_xcd == s->cfg.xcd_tiles && s->graph_tiled == s->cfg.tiled &&
       s->graph_wgs == s->cfg.tile_wgs && s->graph_tile_w == s->b.tiles.w &&
       s->graph_fuse_xf == s->cfg.fuse_xf && s->graph_solve_list == s->cfg.solve_list &&
-      s->graph_wide == s->cfg.wide_walk && s->graph_solve_walk == s->cfg.solve_walk)
+      s->graph_wide == s->cfg.wide_walk && s->graph_solve_walk == s->cfg.solve_walk &&
+      s->graph_solve_first == s->cfg.solve_first)
     return FLUID_OK;
   drop_graph(s);
   hipGraph_t g = nullptr;
@@ -281,6 +284,7 @@ int ensure_graph(fluid_sim *s) {
   s->graph_solve_list = s->cfg.solve_list;
   s->graph_wide = s->cfg.wide_walk;
   s->graph_solve_walk = s->cfg.solve_walk;
+  s->graph_solve_first = s->cfg.solve_first;
   return FLUID_OK;
 }
 
@@ -1216,18 +1220,23 @@ int fluid_set_tile_config(fluid_sim *s, const uint32_t cfg[3]) {
 int fluid_set_fusion(fluid_sim *s, uint32_t flags) {
   if (!s) return fail(FLUID_ERR_INVALID_ARG, "null handle");
   if (flags & ~(FLUID_FUSE_XSPH_FORCE | FLUID_FUSE_SOLVE_LIST | FLUID_WALK_16 | FLUID_WALK_32 |
-                FLUID_SOLVE_WALK_X2 | FLUID_SOLVE_WALK_X4))
+                FLUID_SOLVE_WALK_X2 | FLUID_SOLVE_WALK_X4 | FLUID_SOLVE_FIRST_X2 |
+                FLUID_SOLVE_FIRST_X4))
     return fail(FLUID_ERR_INVALID_ARG, "unknown fusion flag");
   if ((flags & FLUID_WALK_16) && (flags & FLUID_WALK_32))
     return fail(FLUID_ERR_INVALID_ARG, "FLUID_WALK_16 and FLUID_WALK_32 exclude each other");
   if ((flags & FLUID_SOLVE_WALK_X2) && (flags & FLUID_SOLVE_WALK_X4))
     return fail(FLUID_ERR_INVALID_ARG,
                 "FLUID_SOLVE_WALK_X2 and FLUID_SOLVE_WALK_X4 exclude each other");
+  if ((flags & FLUID_SOLVE_FIRST_X2) && (flags & FLUID_SOLVE_FIRST_X4))
+    return fail(FLUID_ERR_INVALID_ARG,
+                "FLUID_SOLVE_FIRST_X2 and FLUID_SOLVE_FIRST_X4 exclude each other");
   if (s->cursor != -1) return fail(FLUID_ERR_STATE, "a partial step is in progress");
   s->cfg.fuse_xf = (flags & FLUID_FUSE_XSPH_FORCE) ? 1 : 0;
   s->cfg.solve_list = (flags & FLUID_FUSE_SOLVE_LIST) ? 1 : 0;
   s->cfg.wide_walk = (flags & FLUID_WALK_16) ? 1 : (flags & FLUID_WALK_32) ? 2 : 0;
   s->cfg.solve_walk = (flags & FLUID_SOLVE_WALK_X2) ? 1 : (flags & FLUID_SOLVE_WALK_X4) ? 2 : 0;
+  s->cfg.solve_first = (flags & FLUID_SOLVE_FIRST_X2) ? 1 : (flags & FLUID_SOLVE_FIRST_X4) ? 2 : 0;
   s->roll.valid = false;
   return FLUID_OK;
 }
@@ -1239,7 +1248,9 @@ uint32_t fluid_fusion(const fluid_sim *s) {
          (s->cfg.wide_walk == 1 ? FLUID_WALK_16 : 0u) |
          (s->cfg.wide_walk == 2 ? FLUID_WALK_32 : 0u) |
          (s->cfg.solve_walk == 1 ? FLUID_SOLVE_WALK_X2 : 0u) |
-         (s->cfg.solve_walk == 2 ? FLUID_SOLVE_WALK_X4 : 0u);
+         (s->cfg.solve_walk == 2 ? FLUID_SOLVE_WALK_X4 : 0u) |
+         (s->cfg.solve_first == 1 ? FLUID_SOLVE_FIRST_X2 : 0u) |
+         (s->cfg.solve_first == 2 ? FLUID_SOLVE_FIRST_X4 : 0u);
 }
 
 int fluid_get_tile_config(const fluid_sim *s, uint32_t out[3]) {
@@ -1312,6 +1323,7 @@ int fluid_autotune(fluid_sim *s, uint32_t trials, uint32_t out[3]) {
     mc.xcd_tiles = m.xcd;
     mc.wide_walk = 0;  // decided with the fusion below
     mc.solve_walk = 0;
+    mc.solve_first = 0;
     for (int li = 0; li < 5 && e == hipSuccess && rc == FLUID_OK; ++li) {
       if (m.tiled && lanes[li] > 8) continue;  // tiled sweeps run at most 8 lanes
       s->cfg = mc;
@@ -1396,6 +1408,28 @@ int fluid_autotune(fluid_sim *s, uint32_t trials, uint32_t out[3]) {
       }
     }
     best.solve_walk = sw;
+  }
+  // the first PCISPH iteration's sweep lanes per skin-list lane, as whole
+  // steps; the wider sweeps regroup the sums, so they must win by 1% (their
+  // measured gains are 0.7-1.5%)
+  if (best.solve_list && !best.tiled && best.solve_lanes <= 8) {
+    float best_f = 1e30f;
+    int sf = 0;
+    for (int v = 0; v < 3 && e == hipSuccess && rc == FLUID_OK; ++v) {
+      s->cfg = best;
+      s->cfg.solve_first = v;
+      float tot = 0.f, ms = 0.f;
+      for (uint32_t t = 0; t < trials && rc == FLUID_OK; ++t) {
+        if ((e = copy(false)) != hipSuccess) break;
+        rc = fluid_time_steps(s, 0, 2, &ms);
+        tot += ms;
+      }
+      if (v == 0 || tot < 0.99f * best_f) {
+        best_f = tot;
+        sf = v;
+      }
+    }
+    best.solve_first = sf;
   }
   if (e == hipSuccess) e = copy(false);
   if (e == hipSuccess)

@@ -61,12 +61,12 @@ __device__ __forceinline__ size_t lane_block(uint32_t s, uint32_t cap) {
 // group (L lanes) calls it at the same point of a group-synchronous walk; the
 // group's k-th entry (ranked across its lanes by a ballot, lanes in order)
 // goes to list lane k % LW at depth k / LW, i.e. blk[(k / LW) * 64 + k % LW]
-// with blk = list + lane_block<LW>(slot, kListCapacity), so each lane of a
-// later walk takes an equal share.  run counts the group's entries (above
-// kListCapacity the list is incomplete and the count marks it).  For L <= 8
+// with blk = list + lane_block<LW>(slot, CAP), so each lane of a later walk
+// takes an equal share.  run counts the group's entries (above CAP entries
+// per slot the list is incomplete and the count marks it).  For L <= 8
 // the group's ballot bits are byte lane / 8 of the wave's ballot (one byte
 // permute), shifted down for L < 8.
-template <int L, int LW>
+template <int L, int LW, uint32_t CAP = kListCapacity>
 __device__ __forceinline__ void group_list_append(uint32_t *__restrict__ blk, uint32_t sub,
                                                   uint32_t &run, bool pair, uint32_t j) {
   const uint32_t lane = threadIdx.x & 63u;
@@ -81,7 +81,7 @@ __device__ __forceinline__ void group_list_append(uint32_t *__restrict__ blk, ui
   }
   if (pair) {
     const uint32_t e = run + (uint32_t)__popc(gm & ((1u << sub) - 1u));
-    if (e < kListCapacity) blk[(e / LW) * 64u + (e % LW)] = j;
+    if (e < CAP) blk[(e / LW) * 64u + (e % LW)] = j;
   }
   run += (uint32_t)__popc(gm);
 }

@@ -623,6 +623,24 @@ __device__ __forceinline__ void solve_particle(const DevUniforms *__restrict__ U
       cnt += keep ? 1u : 0u;
     });
     if (use_list) b.scnt[(size_t)s * LS + kl] = cnt <= kCap ? cnt : kSolveListOverflow;
+  } else if (M > 1 && first_iter) {
+    // First iteration with M sweep lanes per list lane (a group-synchronous
+    // sweep): the group's skin entries are ranked by ballot into the LS list
+    // lanes, entry e at lane e % LS, depth e / LS, the layout the later
+    // iterations walk.
+    const float skin2 = U->solve_skin2, mps = mass * ps;
+    uint32_t *__restrict__ blk = b.slist + lane_block<LS>(s, kSolveListCapacity);
+    uint32_t run = 0;
+    sweep([&](uint32_t j, float4 xj, bool valid) {
+      const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
+      const float r2 = dot3(rx, ry, rz, rx, ry, rz);
+      const float d = valid ? fmaxf(h2 - r2, 0.f) : 0.f;
+      pd = __builtin_fmaf(mps, d * d * d, pd);
+      group_list_append<L, LS, kSolveListCapacity>(blk, sub, run, valid && r2 < skin2, j);
+    });
+    if (sub < LS)
+      b.scnt[(size_t)s * LS + sub] = run > kSolveListCapacity ? kSolveListOverflow
+                                     : run > sub ? (run - 1 - sub) / LS + 1 : 0u;
   } else {
     auto body = [&](uint32_t, float4 xj, bool) {
       const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
@@ -727,6 +745,27 @@ __global__ __launch_bounds__(256) void k_solve_w(Uni un, Bufs b, uint32_t n, Pre
     sweep27<L, false>(U, b.counts, b.starts, in.xq, cx, cy, cz, sub, f);
   };
   solve_particle<L, decltype(sweep) &, LS>(U, b, n, in, out, ap_out, 0u, 1u, s, sub, xi, p0,
+                                           sweep);
+}
+
+// The first PCISPH iteration with L = LS * M lanes sweeping (group-
+// synchronously) and recording LS-lane skin lists for the later iterations.
+template <int L, int LS, bool B>
+__global__ __launch_bounds__(256) void k_solve_fw(Uni un, Bufs b, uint32_t n, PredBuf in,
+                                                  PredBuf out, float4 *ap_out, uint32_t xcd) {
+  const uint32_t g = xcd_block(xcd) * blockDim.x + threadIdx.x;
+  const uint32_t s = g / L, sub = g % L;
+  if (s >= n) return;
+  const DevUniforms *__restrict__ U = uni_at<B>(un, s);
+  const float4 xi = in.xq[s];
+  const float4 p0 = b.spos[s];
+  const int cx = cell_coord(p0.x, U->ox, U->inv_cell);
+  const int cy = cell_coord(p0.y, U->oy, U->inv_cell);
+  const int cz = cell_coord(p0.z, U->oz, U->inv_cell);
+  auto sweep = [&](auto &&f) {
+    sweep27<L, true>(U, b.counts, b.starts, in.xq, cx, cy, cz, sub, f);
+  };
+  solve_particle<L, decltype(sweep) &, LS>(U, b, n, in, out, ap_out, 1u, 1u, s, sub, xi, p0,
                                            sweep);
 }
 
@@ -1199,8 +1238,28 @@ hipError_t launch_solve(const Uni &u, const Bufs &b, uint32_t n, const PredBuf &
   if (c.tiled)
     FLUID_DISPATCH_TILED(k_solve_t, c.lanes, u, b, n, in, out, ap_out,
                          (uint32_t)(first_iter ? 1 : 0), (uint32_t)c.solve_list);
-  else if (!first_iter && c.solve_list && c.solve_walk && c.solve_lanes >= 2 &&
-           c.solve_lanes <= 8) {
+  else if (first_iter && c.solve_list && c.solve_first && c.solve_lanes <= 8) {
+    // first iteration: 2 or 4 sweep lanes per skin-list lane
+    const int M = 1 << c.solve_first, LS = c.solve_lanes;
+    const uint32_t grid = blocks_for((uint64_t)n * (uint64_t)(LS * M), 256);
+#define FLUID_SOLVE_FW(LL, LSS)                                                                 \
+  if (u.scene_of)                                                                               \
+    hipLaunchKernelGGL((k_solve_fw<LL, LSS, true>), dim3(grid), dim3(256), 0, st, u, b, n, in,  \
+                       out, ap_out, (uint32_t)c.xcd_tiles);                                      \
+  else                                                                                          \
+    hipLaunchKernelGGL((k_solve_fw<LL, LSS, false>), dim3(grid), dim3(256), 0, st, u, b, n,     \
+                       in, out, ap_out, (uint32_t)c.xcd_tiles);
+    if (LS == 1 && M == 2) { FLUID_SOLVE_FW(2, 1) }
+    else if (LS == 1) { FLUID_SOLVE_FW(4, 1) }
+    else if (LS == 2 && M == 2) { FLUID_SOLVE_FW(4, 2) }
+    else if (LS == 2) { FLUID_SOLVE_FW(8, 2) }
+    else if (LS == 4 && M == 2) { FLUID_SOLVE_FW(8, 4) }
+    else if (LS == 4) { FLUID_SOLVE_FW(16, 4) }
+    else if (M == 2) { FLUID_SOLVE_FW(16, 8) }
+    else { FLUID_SOLVE_FW(32, 8) }
+#undef FLUID_SOLVE_FW
+  } else if (!first_iter && c.solve_list && c.solve_walk && c.solve_lanes >= 2 &&
+             c.solve_lanes <= 8) {
     // later iterations: 2 or 4 walkers per skin-list lane
     const int M = 1 << c.solve_walk, LS = c.solve_lanes;
     const uint32_t grid = blocks_for((uint64_t)n * (uint64_t)(LS * M), 256);

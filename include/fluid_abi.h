@@ -392,13 +392,19 @@ int fluid_set_tile_config(fluid_sim *sim, const uint32_t cfg[3]);
  *     lane) instead of 8;
  *   FLUID_SOLVE_WALK_X2, FLUID_SOLVE_WALK_X4 (at most one; with skin lists,
  *     global sweeps and 2 to 8 solve lanes): the later PCISPH iterations walk
- *     the skin lists with two or four walkers per list lane. */
+ *     the skin lists with two or four walkers per list lane;
+ *   FLUID_SOLVE_FIRST_X2, FLUID_SOLVE_FIRST_X4 (at most one; with skin lists,
+ *     global sweeps and at most 8 solve lanes): the first PCISPH iteration
+ *     sweeps with two or four lanes per skin-list lane and ranks its entries
+ *     into the lists by ballot. */
 #define FLUID_FUSE_XSPH_FORCE 1u
 #define FLUID_FUSE_SOLVE_LIST 2u
 #define FLUID_WALK_16 4u
 #define FLUID_WALK_32 8u
 #define FLUID_SOLVE_WALK_X2 16u
 #define FLUID_SOLVE_WALK_X4 32u
+#define FLUID_SOLVE_FIRST_X2 64u
+#define FLUID_SOLVE_FIRST_X4 128u
 int fluid_set_fusion(fluid_sim *sim, uint32_t flags);
 uint32_t fluid_fusion(const fluid_sim *sim);
 int fluid_get_tile_config(const fluid_sim *sim, uint32_t out[3]);

@@ -193,16 +193,23 @@ def test_density_16_lanes(pkg, wide):
     b.close()
 
 
-@pytest.mark.parametrize("solve_lanes", [2, 4, 8])
-@pytest.mark.parametrize("walk", ["x2", "x4"])
+@pytest.mark.parametrize("solve_lanes", [1, 2, 4, 8])
+@pytest.mark.parametrize("walk", ["x2", "x4", "first_x2", "first_x4", "both_x2"])
 def test_solve_wide_walk(pkg, solve_lanes, walk):
     """The later PCISPH iterations with two or four walkers per skin-list
-    lane sum the same pairs as one walker, grouped otherwise: from the same
-    state, one step agrees to the solve's rounding noise (predicted density
-    1e-5; velocity and position 2e-3, per particle with a floor of 1% of the
+    lane, and the first iteration sweeping with two or four lanes per list
+    lane (its entries ranked into the lists by ballot), sum the same pairs
+    as one lane per list lane, grouped otherwise: from the same state, one
+    step agrees to the solve's rounding noise (predicted density 1e-5;
+    velocity and position 2e-3, per particle with a floor of 1% of the
     field RMS, as the oracle parity tests; pressure 2e-3 likewise)."""
     from conftest import rowwise_rel_err
-    flag = pkg.abi.FLUID_SOLVE_WALK_X2 if walk == "x2" else pkg.abi.FLUID_SOLVE_WALK_X4
+    A = pkg.abi
+    flag = {"x2": A.FLUID_SOLVE_WALK_X2, "x4": A.FLUID_SOLVE_WALK_X4,
+            "first_x2": A.FLUID_SOLVE_FIRST_X2, "first_x4": A.FLUID_SOLVE_FIRST_X4,
+            "both_x2": A.FLUID_SOLVE_WALK_X2 | A.FLUID_SOLVE_FIRST_X2}[walk]
+    if solve_lanes == 1 and walk in ("x2", "x4", "both_x2"):
+        pytest.skip("walkers per list lane need at least 2 solve lanes")
     base = pkg.abi.FLUID_FUSE_XSPH_FORCE | pkg.abi.FLUID_FUSE_SOLVE_LIST
     scene = pkg.scenes.dam_break_8k()
     a = pkg.make_sim(scene)
@@ -236,10 +243,13 @@ def test_solve_wide_walk(pkg, solve_lanes, walk):
 
 
 def test_solve_walk_flags(pkg):
-    """The two solve walks exclude each other; the flags read back as set."""
+    """The two solve walks exclude each other, and so do the two wide first
+    iterations; the flags read back as set."""
     sim = pkg.make_sim(pkg.scenes.dam_break_8k())
     with pytest.raises(pkg.abi.FluidError):
         sim.set_fusion(pkg.abi.FLUID_SOLVE_WALK_X2 | pkg.abi.FLUID_SOLVE_WALK_X4)
+    with pytest.raises(pkg.abi.FluidError):
+        sim.set_fusion(pkg.abi.FLUID_SOLVE_FIRST_X2 | pkg.abi.FLUID_SOLVE_FIRST_X4)
     flags = pkg.abi.FLUID_SOLVE_WALK_X4 | pkg.abi.FLUID_FUSE_SOLVE_LIST
     sim.set_fusion(flags)
     assert sim.fusion() == flags
