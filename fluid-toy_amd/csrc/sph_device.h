@@ -81,7 +81,8 @@ __device__ __forceinline__ void group_list_append(uint32_t *__restrict__ blk, ui
   }
   if (pair) {
     const uint32_t e = run + (uint32_t)__popc(gm & ((1u << sub) - 1u));
-    if (e < CAP) blk[(e / LW) * 64u + (e % LW)] = j;
+    // (e / LW) * 64 + e % LW, as one multiply-add
+    if (e < CAP) blk[e + (e & ~(uint32_t)(LW - 1)) * (64u / LW - 1u)] = j;
   }
   run += (uint32_t)__popc(gm);
 }
