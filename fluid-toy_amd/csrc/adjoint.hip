@@ -538,12 +538,13 @@ __global__ __launch_bounds__(256) void k_adj_force(Uni un, AdjBufs a, TapeStep t
   // lanes of the wave stored in pass 1: the workgroup fence between the
   // passes waits for those stores and keeps the loads after them.
   uint32_t run = 0;
+  const PairTest in_pair(h2);
   sweep27<L, true>(U, t.counts, t.starts, t.x, cx, cy, cz, sub,
                    [&](uint32_t j, float4 xj, bool valid) {
     const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
     const float r2 = dot3(rx, ry, rz, rx, ry, rz);
     group_list_append<L, L>(a.plist + lane_block<L>(s, kListCapacity), sub, run,
-                         valid && r2 < h2 && r2 > 1e-12f, j);
+                         valid && in_pair(r2), j);
   });
   if (sub == 0) a.pcnt[s] = run;
   __threadfence_block();

@@ -87,6 +87,22 @@ __device__ __forceinline__ void group_list_append(uint32_t *__restrict__ blk, ui
   run += (uint32_t)__popc(gm);
 }
 
+// The pair test 1e-12 < r^2 < h^2 as one unsigned compare of bit patterns:
+// r^2, a sum of squares, is +0 or more and orders as its bits, and NaN falls
+// outside.  A single compare is a mask a ballot takes as it is (two compares
+// and an AND are turned back into a lane value and compared again).
+struct PairTest {
+  uint32_t lo, span;
+  __device__ __forceinline__ explicit PairTest(float h2) {
+    lo = __float_as_uint(1e-12f) + 1u;
+    const uint32_t hb = __float_as_uint(h2);
+    span = (h2 > 0.f && hb > lo) ? hb - lo : 0u;
+  }
+  __device__ __forceinline__ bool operator()(float r2) const {
+    return __float_as_uint(r2) - lo < span;
+  }
+};
+
 // fmaxf(x, 1.0f) for every non-NaN x in one integer max: non-negative floats
 // order as their bit patterns, and negative ones (sign bit set) are negative
 // as int32, below 1.0f's pattern.  fmaxf itself costs two instructions here

@@ -303,18 +303,14 @@ __device__ __forceinline__ void density_particle(const DevUniforms *__restrict__
   uint32_t *__restrict__ nbr = b.nbr + lane_block<kListLanes>(s, kListCapacity);
   uint32_t run = 0;  // pairs recorded so far by the whole group
   float dens = 0.f, near = 0.f, nx = 0.f, ny = 0.f, nz = 0.f;
-  // 1e-12 < r^2 < h^2 as one unsigned compare of bit patterns: r^2, a sum of
-  // squares, is +0 or more and orders as its bits, and NaN falls outside.  A
-  // single compare is a mask the ballot takes as it is.
-  const uint32_t pair_lo = __float_as_uint(1e-12f) + 1u, hb = __float_as_uint(h2);
-  const uint32_t pair_span = (h2 > 0.f && hb > pair_lo) ? hb - pair_lo : 0u;
+  const PairTest in_pair(h2);
   // Branch-free body: a pair outside the support contributes exact zeros
   // through selects, so the lanes of a wave stay converged.
   sweep([&](uint32_t j, float4 pj, bool valid) {
     const float rx = pi.x - pj.x, ry = pi.y - pj.y, rz = pi.z - pj.z;
     const float r2 = dot3(rx, ry, rz, rx, ry, rz);
     const bool inr = valid && r2 < h2;
-    const bool pair = valid && __float_as_uint(r2) - pair_lo < pair_span;
+    const bool pair = valid && in_pair(r2);
     const float d = inr ? h2 - r2 : 0.f;
     const float d2 = d * d;
     dens = __builtin_fmaf(mps, d2 * d, dens);
