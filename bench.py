@@ -70,10 +70,13 @@ def profiled_traffic(kernel: str):
     """HBM bytes per dispatch of `kernel` (e.g. "k_force<8,false>") from the
     committed FETCH_SIZE / WRITE_SIZE summary (copy-calibrated), or None."""
     try:
+        col = None
         for line in TRAFFIC_PROFILE.read_text().splitlines():
             cols = [c.strip() for c in line.strip().strip("|").split("|")]
-            if len(cols) >= 5 and cols[0] == f"`{kernel}`":
-                return int(float(cols[4]))
+            if cols and cols[0] == "kernel" and "HBM bytes" in cols:
+                col = cols.index("HBM bytes")
+            elif col is not None and len(cols) > col and cols[0] == f"`{kernel}`":
+                return int(float(cols[col]))
     except OSError:
         return None
     return None
