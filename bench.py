@@ -408,7 +408,20 @@ def cpu_baseline(pkg, orc, scene, start, warmup: int, budget_s: float = 15.0,
             "kind": "port",
             "sample": f"{scene.name}: {k} f32 oracle steps from the GPU state after "
                       f"{warmup} warm-up steps (+1 untimed) ({dt:.1f} s, OpenMP {threads} "
-                      f"threads)"}
+                      f"threads)",
+            "host_cpus": os.cpu_count(), "cpu_model": cpu_model()}
+
+
+def cpu_model() -> str:
+    """The host CPU's model name (/proc/cpuinfo), or "unknown"."""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
 
 
 def progress(msg: str) -> None:
