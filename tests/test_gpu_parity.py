@@ -262,6 +262,29 @@ def test_out_of_grid_particles_and_no_xsph(pkg, orc):
     sim.close()
 
 
+def test_pair_test_lower_bound(pkg, orc):
+    """The pair test 1e-12 < r^2 < h^2 (sph_density_3d_grid.wgsl:150-159,
+    xsph.wgsl:126, sph_force_3d_grid.wgsl:157) at its lower edge: a coincident pair
+    (r^2 = 0), a pair at r = 2^-20 (r^2 = 2^-40 < 1e-12: density only, no pair
+    terms) and one at r = 2^-19 (r^2 = 2^-38 > 1e-12: a pair, inside the
+    separation range).  The offsets are exact in float32 for |x| < 1, so r^2
+    is exact on both sides; a misplaced bound adds or drops a separation
+    force and fails the force comparison."""
+    scene = pkg.scenes.dam_break_8k()
+    parts = scene.particles.copy()
+    pairs = [(4000, 4001, 0.0), (5000, 5001, 2.0 ** -20), (6000, 6001, 2.0 ** -19)]
+    for a, b, dx in pairs:
+        assert np.all(np.abs(parts[a, 0:3]) < 1.0)
+        parts[b, 0:3] = parts[a, 0:3]
+        parts[b, 0] = np.float32(parts[a, 0] + np.float32(dx))
+        assert float(parts[b, 0]) - float(parts[a, 0]) == dx
+    scene.particles = parts
+    sim, o = make_pair(pkg, orc, scene)
+    print("case: pair-test lower bound (r^2 = 0, 2^-40, 2^-38)", flush=True)
+    one_step_compare(sim, o, "r2 edge")
+    sim.close()
+
+
 @pytest.mark.parametrize("mode", [0, 1, 2, 3, 4])
 def test_mouse_force_modes(pkg, orc, mode):
     scene = pkg.scenes.dam_break_8k()
