@@ -91,6 +91,7 @@ class Scene:
     mass: float = 1.0
     xsph_epsilon: float = 0.305
     gravity: tuple = (0.0, -9.81, 0.0)
+    tilt: tuple = (0.0, 0.0)         # container tilt about x, then z (radians)
     pcisph_iterations: int = 4
     bounds_extent: float = 2.0
     steps: int = 200
@@ -224,7 +225,7 @@ def make_params(scene: Scene, abi_mod, derive_sph_params, derive_container_geome
                                scene.xsph_epsilon, scene.dt, scene.n,
                                rest_density=scene.rest_density)
     w, hgt, d = scene.box
-    cont = derive_container_geometry(w, hgt, d, scene.floor_y, 0.0, 0.0,
+    cont = derive_container_geometry(w, hgt, d, scene.floor_y, scene.tilt[0], scene.tilt[1],
                                      scene.wall_stiffness, scene.damping)
     g = abi_mod.GpuGravity()
     g.direction[0], g.direction[1], g.direction[2] = scene.gravity

@@ -6,6 +6,7 @@ loads it.  See fluid_oracle.cpp for what it restates and how it is pinned.
 """
 from __future__ import annotations
 
+import contextlib
 import ctypes as C
 from pathlib import Path
 
@@ -20,44 +21,61 @@ _LIB = None
 def load():
     global _LIB
     if _LIB is None:
-        if not LIB_PATH.exists():
-            raise FileNotFoundError(f"{LIB_PATH} missing: run make -C oracle")
-        lib = C.CDLL(str(LIB_PATH))
-        P = C.c_void_p
-        lib.oracle_create.restype = P
-        lib.oracle_create.argtypes = [C.c_int, P, C.c_uint32, P, P, P, P, P, P, C.c_uint32,
-                                      P, C.c_uint32]
-        lib.oracle_destroy.argtypes = [P]
-        lib.oracle_stage.argtypes = [P, C.c_uint32]
-        lib.oracle_step.argtypes = [P, C.c_uint32]
-        lib.oracle_set_params.argtypes = [P, P, P, P, C.c_uint32]
-        lib.oracle_get_particles.argtypes = [P, C.c_int, P]
-        lib.oracle_set_particles.argtypes = [P, P]
-        lib.oracle_get_u32.argtypes = [P, C.c_int, P]
-        lib.oracle_get_pred.argtypes = [P, C.c_int, P]
-        lib.oracle_get_accum.argtypes = [P, P]
-        lib.oracle_count_pairs.argtypes = [P, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]
-        lib.oracle_rest_density_f32.restype = C.c_float
-        lib.oracle_rest_density_f32.argtypes = [C.c_float, C.c_float]
-        lib.oracle_pcisph_delta_f32.restype = C.c_float
-        lib.oracle_pcisph_delta_f32.argtypes = [C.c_float] * 4
-        lib.oracle_powi_f32.restype = C.c_float
-        lib.oracle_powi_f32.argtypes = [C.c_float, C.c_int]
-        lib.oracle_spray_potentials.argtypes = [P, C.c_uint32, P, P, P, P, P, C.c_float,
-                                                C.c_float, P, P, P]
-        lib.oracle_spray_spawn.restype = C.c_uint32
-        lib.oracle_spray_spawn.argtypes = [P, C.c_uint32, P, P, P, C.c_uint32, P]
-        lib.oracle_spray_simulate.argtypes = [P, P, P, P, P, P, P, P, P]
-        lib.oracle_mc_aniso.argtypes = [P, C.c_uint32, P, P, P, P, P]
-        lib.oracle_mc_density.argtypes = [P, P, P, P, P, P, P, P, P]
-        lib.oracle_spray_update_limit.restype = C.c_float
-        lib.oracle_spray_update_limit.argtypes = [C.c_float] * 3
-        lib.oracle_spray_hash_float.restype = C.c_float
-        lib.oracle_spray_hash_float.argtypes = [C.c_uint32]
-        lib.oracle_spray_hash.restype = C.c_uint32
-        lib.oracle_spray_hash.argtypes = [C.c_uint32]
-        _LIB = lib
+        _LIB = _open(LIB_PATH)
     return _LIB
+
+
+@contextlib.contextmanager
+def library(path):
+    """Use another build of fluid_oracle.cpp (a mutant built by the pins'
+    sensitivity test) for every oracle call inside the block."""
+    global _LIB
+    saved = _LIB
+    _LIB = _open(Path(path))
+    try:
+        yield _LIB
+    finally:
+        _LIB = saved
+
+
+def _open(path: Path):
+    if not path.exists():
+        raise FileNotFoundError(f"{path} missing: run make -C oracle")
+    lib = C.CDLL(str(path))
+    P = C.c_void_p
+    lib.oracle_create.restype = P
+    lib.oracle_create.argtypes = [C.c_int, P, C.c_uint32, P, P, P, P, P, P, C.c_uint32,
+                                  P, C.c_uint32]
+    lib.oracle_destroy.argtypes = [P]
+    lib.oracle_stage.argtypes = [P, C.c_uint32]
+    lib.oracle_step.argtypes = [P, C.c_uint32]
+    lib.oracle_set_params.argtypes = [P, P, P, P, C.c_uint32]
+    lib.oracle_get_particles.argtypes = [P, C.c_int, P]
+    lib.oracle_set_particles.argtypes = [P, P]
+    lib.oracle_get_u32.argtypes = [P, C.c_int, P]
+    lib.oracle_get_pred.argtypes = [P, C.c_int, P]
+    lib.oracle_get_accum.argtypes = [P, P]
+    lib.oracle_count_pairs.argtypes = [P, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]
+    lib.oracle_rest_density_f32.restype = C.c_float
+    lib.oracle_rest_density_f32.argtypes = [C.c_float, C.c_float]
+    lib.oracle_pcisph_delta_f32.restype = C.c_float
+    lib.oracle_pcisph_delta_f32.argtypes = [C.c_float] * 4
+    lib.oracle_powi_f32.restype = C.c_float
+    lib.oracle_powi_f32.argtypes = [C.c_float, C.c_int]
+    lib.oracle_spray_potentials.argtypes = [P, C.c_uint32, P, P, P, P, P, C.c_float,
+                                            C.c_float, P, P, P]
+    lib.oracle_spray_spawn.restype = C.c_uint32
+    lib.oracle_spray_spawn.argtypes = [P, C.c_uint32, P, P, P, C.c_uint32, P]
+    lib.oracle_spray_simulate.argtypes = [P, P, P, P, P, P, P, P, P]
+    lib.oracle_mc_aniso.argtypes = [P, C.c_uint32, P, P, P, P, P]
+    lib.oracle_mc_density.argtypes = [P, P, P, P, P, P, P, P, P]
+    lib.oracle_spray_update_limit.restype = C.c_float
+    lib.oracle_spray_update_limit.argtypes = [C.c_float] * 3
+    lib.oracle_spray_hash_float.restype = C.c_float
+    lib.oracle_spray_hash_float.argtypes = [C.c_uint32]
+    lib.oracle_spray_hash.restype = C.c_uint32
+    lib.oracle_spray_hash.argtypes = [C.c_uint32]
+    return lib
 
 
 def grid_for(h: float, bounds_extent: float, n: int, abi_mod):
