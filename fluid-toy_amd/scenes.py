@@ -169,6 +169,29 @@ def dam_break_32k(seed: int = 4) -> Scene:
     return sc
 
 
+def tilted(scene: Scene, tilt_x: float, tilt_z: float) -> Scene:
+    """The scene in its container tilted by tilt_x about x, then tilt_z about
+    z (ContainerConfig tilt, src/state/simulation.rs:139-159): particle
+    positions and velocities are carried into the tilted frame about the
+    container centre, so the block keeps its place against the local walls."""
+    cx, sx, cz, sz = np.cos(tilt_x), np.sin(tilt_x), np.cos(tilt_z), np.sin(tilt_z)
+    rx = np.array([[1, 0, 0], [0, cx, -sx], [0, sx, cx]])
+    rz = np.array([[cz, -sz, 0], [sz, cz, 0], [0, 0, 1]])
+    r = rz @ rx
+    centre = np.array([0.0, scene.floor_y + scene.box[1] / 2, 0.0])
+    parts = scene.particles.copy()
+    pos = parts[:, 0:3].astype(np.float64)
+    parts[:, 0:3] = ((pos - centre) @ r.T + centre).astype(np.float32)
+    parts[:, 4:7] = (parts[:, 4:7].astype(np.float64) @ r.T).astype(np.float32)
+    out = Scene(f"{scene.name}_tilted", parts, scene.box, scene.floor_y,
+                **{k: getattr(scene, k) for k in (
+                    "h", "rest_density", "stiffness", "near_stiffness", "viscosity",
+                    "surface_tension", "wall_stiffness", "damping", "dt", "mass",
+                    "xsph_epsilon", "gravity", "pcisph_iterations", "bounds_extent", "steps")})
+    out.tilt = (tilt_x, tilt_z)
+    return out
+
+
 def small_block(nx: int, ny: int, nz: int, seed: int = 7, box=(1.6, 1.2, 0.8)) -> Scene:
     """A small dam-break block for fast parity cases."""
     pts = lattice(nx, ny, nz, 0.025)

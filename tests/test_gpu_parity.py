@@ -384,7 +384,7 @@ def test_empty_simulation(pkg):
 
 def test_block_1m_full_size(pkg, orc):
     """Full-size 1,048,576-particle block: exact grid invariants, and one
-    graph step against the float32 oracle."""
+    graph step against the float32 and float64 oracles at the full bar."""
     scene = pkg.scenes.block_1m()
     print(f"case: block_1m n={scene.n}", flush=True)
     params, cont, g = pkg.scenes.make_params(scene, pkg.abi, pkg.derive_sph_params,
@@ -397,21 +397,23 @@ def test_block_1m_full_size(pkg, orc):
     np.testing.assert_array_equal(starts, np.cumsum(counts))
     s2o = sim.read_sorted_to_orig()
     assert np.array_equal(np.sort(s2o), np.arange(scene.n))
-    o = orc.from_scene(scene, pkg.abi, params, cont, g)
+    o = Lockstep(orc, pkg, scene, params, cont, g)
     o.step()
     np.testing.assert_array_equal(sim.read_cell_counts(), o.cell_counts())
     np.testing.assert_array_equal(s2o, o.sorted_to_orig())
-    gp, op = sim.read_particles(), o.particles()
-    check_fp("1m position", gp[:, 0:3], op[:, 0:3])
-    err = rowwise_rel_err(gp[:, 4:7], op[:, 4:7])
-    report("1m velocity", err)
-    assert np.quantile(err, 0.9999) <= 1e-3
+    gp, op, o6 = sim.read_particles(), o.particles(), o.o64.particles()
+    check_fp("1m position", gp[:, 0:3], op[:, 0:3], o6[:, 0:3])
+    check_fp("1m velocity", gp[:, 4:7], op[:, 4:7], o6[:, 4:7])
+    check_fp("1m force", gp[:, 8:11], op[:, 8:11], o6[:, 8:11])
+    gs, os_, o6s = sim.read_sorted_particles(), o.sorted_particles(), o.o64.sorted_particles()
+    check_fp("1m density", gs[:, 11], os_[:, 11], o6s[:, 11])
     sim.close()
 
 
 def test_block_4m_full_size(pkg, orc):
     """Full-size 4,194,304-particle block (the second throughput size):
-    exact grid invariants, and one graph step against the float32 oracle."""
+    exact grid invariants, and one graph step against the float32 and float64
+    oracles at the full bar."""
     scene = pkg.scenes.block_4m()
     print(f"case: block_4m n={scene.n}", flush=True)
     params, cont, g = pkg.scenes.make_params(scene, pkg.abi, pkg.derive_sph_params,
@@ -424,13 +426,14 @@ def test_block_4m_full_size(pkg, orc):
     np.testing.assert_array_equal(starts, np.cumsum(counts))
     s2o = sim.read_sorted_to_orig()
     assert np.array_equal(np.sort(s2o), np.arange(scene.n))
-    o = orc.from_scene(scene, pkg.abi, params, cont, g)
+    o = Lockstep(orc, pkg, scene, params, cont, g)
     o.step()
     np.testing.assert_array_equal(sim.read_cell_counts(), o.cell_counts())
     np.testing.assert_array_equal(s2o, o.sorted_to_orig())
-    gp, op = sim.read_particles(), o.particles()
-    check_fp("4m position", gp[:, 0:3], op[:, 0:3])
-    err = rowwise_rel_err(gp[:, 4:7], op[:, 4:7])
-    report("4m velocity", err)
-    assert np.quantile(err, 0.9999) <= 1e-3
+    gp, op, o6 = sim.read_particles(), o.particles(), o.o64.particles()
+    check_fp("4m position", gp[:, 0:3], op[:, 0:3], o6[:, 0:3])
+    check_fp("4m velocity", gp[:, 4:7], op[:, 4:7], o6[:, 4:7])
+    check_fp("4m force", gp[:, 8:11], op[:, 8:11], o6[:, 8:11])
+    gs, os_, o6s = sim.read_sorted_particles(), o.sorted_particles(), o.o64.sorted_particles()
+    check_fp("4m density", gs[:, 11], os_[:, 11], o6s[:, 11])
     sim.close()
