@@ -310,10 +310,12 @@ __global__ __launch_bounds__(256) void k_mc_density(Uni un, Bufs b, McLaunch m,
             __builtin_amdgcn_readfirstlane(__float_as_int(L * L - gz2 * gz2 - gy2 * gy2)));
         if (!(rem > 0.0f)) continue;
         const float allow = sqrtf(rem);
+        // the cells within reach along x; the border cells 0 and gx - 1
+        // extend to infinity, as in gap()
         const int xa = __builtin_amdgcn_readfirstlane(
-            max(x0, (int)floorf((bx_lo - allow - U->ox) * U->inv_cell)));
+            max(x0, min((int)floorf((bx_lo - allow - U->ox) * U->inv_cell), gx - 1)));
         const int xb = __builtin_amdgcn_readfirstlane(
-            min(x1, (int)floorf((bx_hi + allow - U->ox) * U->inv_cell)));
+            min(x1, max((int)floorf((bx_hi + allow - U->ox) * U->inv_cell), 0)));
         if (xa > xb) continue;
         const uint32_t row = (uint32_t)y * U->gx + (uint32_t)z * U->gxy;
         const uint32_t c0 = row + (uint32_t)xa, c1 = row + (uint32_t)xb;

@@ -122,6 +122,44 @@ def test_clipped_tilted_container(pkg, orc):
     sim.close()
 
 
+@pytest.mark.parametrize("aniso", [False, True])
+def test_field_beyond_particle_grid(pkg, orc, aniso):
+    """A simulation grid smaller than the container (extent 0.5: cells span
+    [-0.58, 0.62] per axis): the build clamps the particles outside it into
+    the border cells, which the field's stencils and its per-block cull
+    treat as reaching to infinity.  The field covers the whole container, so
+    voxel blocks lie beyond the grid on every side; it must still be the
+    oracle's to the bit."""
+    scene = pkg.scenes.dam_break_8k()
+    scene.bounds_extent = 0.5
+    sim = pkg.make_sim(scene)
+    sim.step(10)
+    grid = sim.grid_params()
+    lo = grid.grid_origin_x
+    hi = lo + grid.grid_size_x * grid.cell_size
+    pos = sim.read_particles()[:, 0:3]
+    outside = (pos < lo).any(1) | (pos >= hi).any(1)
+    print(f"case: MC beyond the particle grid, aniso={aniso}, "
+          f"{int(outside.sum())} particles outside [{lo:.2f}, {hi:.2f})", flush=True)
+    assert outside.sum() > 100
+    G = 40
+    mc = pkg.MarchingCubesField(sim, G)
+    h = sim.sph_params().kernel_radius
+    a = pkg.abi.default_aniso_params(h, enabled=aniso)
+    mc.update_aniso_params(a)
+    g, cont = _default_params(pkg, sim, G)
+    mc.update_params(g)
+    mc.generate()
+    got = mc.read_density()
+    recs, cnt, st, grid = _state(sim)
+    want = orc.mc_density(recs, cnt, st, grid, g, a, cont, mc.read_aniso() if aniso else None)
+    assert (want > 0).sum() > 1000
+    diff = np.argwhere(got != want)
+    assert len(diff) == 0, (len(diff), diff[:3])
+    mc.close()
+    sim.close()
+
+
 def test_mc_guards(pkg):
     import ctypes as C
     sim = pkg.make_sim(pkg.scenes.dam_break_8k())
