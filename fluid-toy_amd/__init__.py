@@ -10,7 +10,7 @@ there is no CPU fallback, and a missing library raises FileNotFoundError.
 """
 from . import abi, scenes
 from .sim import (MarchingCubesField, SphBatch, SphSimulation3DGrid, SpraySystem,
-                  derive_container_geometry, derive_sph_params, gravity)
+                  derive_container_geometry, derive_sph_params, gravity, mc_generate_noalloc)
 
 
 def make_sim(scene, device: int = 0, max_particles: int = 0) -> SphSimulation3DGrid:
@@ -44,4 +44,4 @@ def make_batch(scene_list, device: int = 0) -> SphBatch:
 
 __all__ = ["abi", "scenes", "SphSimulation3DGrid", "SphBatch", "SpraySystem",
            "MarchingCubesField", "derive_sph_params",
-           "derive_container_geometry", "gravity", "make_sim", "make_batch", "scene_tuple"]
+           "derive_container_geometry", "gravity", "mc_generate_noalloc", "make_sim", "make_batch", "scene_tuple"]

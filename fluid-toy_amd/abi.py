@@ -287,6 +287,9 @@ PROTOTYPES = {
     "fluid_mc_read_aniso": (C.c_int, [P, P, u32]),
     "fluid_mc_density_from_aniso": (C.c_int, [P, P, u32]),
     "fluid_mc_time": (C.c_int, [P, u32, u32, u32, C.POINTER(C.c_float)]),
+    "fluid_mc_generate_noalloc": (C.c_int, [P, C.POINTER(GpuMcGridParams),
+                                            C.POINTER(GpuAnisoParams),
+                                            C.POINTER(GpuContainerGeometry), P]),
 }
 
 FLUID_FUSE_XSPH_FORCE = 1

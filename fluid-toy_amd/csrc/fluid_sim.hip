@@ -11,6 +11,7 @@
 #include <cstdio>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -2050,3 +2051,70 @@ int fluid_mc_time(fluid_mc *mc, uint32_t num_particles, uint32_t warmup, uint32_
 }
 
 }  // extern "C"
+
+// ------------------------------------------------------------------------
+// The marching-cubes field without a field handle and without allocating:
+// the ellipsoid records between the two passes live in `scratch`, a fixed
+// array of this library (one copy per device, held until the process
+// ends).  Host code never passes the array's name to a kernel: the name is
+// a host-side stand-in.  Each device's address comes from
+// hipGetSymbolAddress, taken once per device with that device current.
+static __device__ __align__(256) float4 scratch[3 * (size_t)FLUID_MC_NOALLOC_MAX_PARTICLES];
+
+namespace {
+
+constexpr int kMaxScratchDevices = 64;
+std::mutex g_scratch_mu;
+float4 *g_scratch_ptr[kMaxScratchDevices] = {};
+
+// The calling thread's current device must be `device`.
+int scratch_on(int device, float4 **out) {
+  if (device < 0 || device >= kMaxScratchDevices)
+    return fail(FLUID_ERR_INVALID_ARG, "device ordinal out of range for the scratch table");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[device]) {
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return fail(FLUID_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") +
+                                     hipGetErrorString(e));
+    g_scratch_ptr[device] = (float4 *)p;
+  }
+  *out = g_scratch_ptr[device];
+  return FLUID_OK;
+}
+
+}  // namespace
+
+extern "C" int fluid_mc_generate_noalloc(fluid_sim *s, const GpuMcGridParams *grid,
+                                         const GpuAnisoParams *aniso,
+                                         const GpuContainerGeometry *container,
+                                         float *field) {
+  if (!s || !grid || !aniso || !container || !field)
+    return fail(FLUID_ERR_INVALID_ARG, "null argument");
+  if (s->batched()) return fail(FLUID_ERR_INVALID_ARG, "the MC field needs a plain simulation");
+  if (s->cursor != -1)
+    return fail(FLUID_ERR_STATE, "a partial step of the simulation is in progress");
+  if (grid->grid_size == 0 || grid->grid_size > 512)
+    return fail(FLUID_ERR_INVALID_ARG, "grid_size must be 1 to 512");
+  if (!(grid->kernel_radius > 0.0f) || !(grid->cell_size > 0.0f))
+    return fail(FLUID_ERR_INVALID_ARG, "kernel_radius and cell_size must be positive");
+  if (aniso->enabled && !(aniso->support_radius > 0.0f && aniso->h_mc > 0.0f && aniso->kr > 0.0f))
+    return fail(FLUID_ERR_INVALID_ARG, "support_radius, h_mc and kr must be positive");
+  const uint32_t n = s->n;
+  if (aniso->enabled && n > FLUID_MC_NOALLOC_MAX_PARTICLES)
+    return fail(FLUID_ERR_CAPACITY, "anisotropy without allocation takes at most " +
+                                        std::to_string(FLUID_MC_NOALLOC_MAX_PARTICLES) +
+                                        " particles (the simulation holds " +
+                                        std::to_string(n) + ")");
+  HIP_TRY(hipSetDevice(s->device));
+  float4 *recs = nullptr;
+  const int rc = scratch_on(s->device, &recs);
+  if (rc) return rc;
+  const McLaunch m = make_mc_launch(*grid, *aniso, *container, s->sc[0].grid.inv_cell_size);
+  // the fit writes every record the splat reads (particles 0 .. n-1, the
+  // slots of the simulation's grid), so nothing in the array needs clearing
+  if (aniso->enabled && n) HIP_TRY(launch_mc_aniso(s->uni(), s->b, m, recs, n, s->stream));
+  HIP_TRY(launch_mc_density(s->uni(), s->b, m, recs, field, s->stream));
+  return FLUID_OK;
+}

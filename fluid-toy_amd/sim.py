@@ -496,6 +496,20 @@ class SpraySystem:
         return ms.value
 
 
+MC_NOALLOC_MAX_PARTICLES = 65536
+
+
+def mc_generate_noalloc(sim: SphSimulation3DGrid, grid: abi.GpuMcGridParams,
+                        aniso: abi.GpuAnisoParams, container: abi.GpuContainerGeometry,
+                        field_ptr: int) -> None:
+    """The marching-cubes field of all of `sim`'s particles written to the
+    caller's device buffer at `field_ptr` (grid.grid_size^3 float32, x
+    fastest) without a field handle and without allocating device memory
+    (fluid_mc_generate_noalloc).  Asynchronous on the simulation's stream."""
+    sim._chk(sim._lib.fluid_mc_generate_noalloc(sim._h, C.byref(grid), C.byref(aniso),
+                                                C.byref(container), C.c_void_p(field_ptr)))
+
+
 class MarchingCubesField:
     """The compute passes 0 and 1 of the reference's
     MarchingCubesRenderer::generate (src/render/marching_cubes.rs:2349-2470)

@@ -649,6 +649,24 @@ int fluid_mc_density_from_aniso(fluid_mc *mc, const GpuParticleAniso *in, uint32
 int fluid_mc_time(fluid_mc *mc, uint32_t num_particles, uint32_t warmup, uint32_t iters,
                   float *ms);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): fluid_mc_generate's two passes over all of the
+ * simulation's particles without a field handle, allocating nothing.  The
+ * ellipsoid records between the passes live in a fixed array of the
+ * library (48 bytes x FLUID_MC_NOALLOC_MAX_PARTICLES = 3 MiB per device,
+ * held until the process ends), so with anisotropy enabled the simulation
+ * may hold at most FLUID_MC_NOALLOC_MAX_PARTICLES particles
+ * (FLUID_ERR_CAPACITY beyond; the isotropic splat has no limit).  `field`
+ * is a device pointer to grid->grid_size^3 floats (x fastest), written on
+ * the simulation's stream; the call returns without waiting.  Every call on
+ * one device shares that device's array: calls that overlap on one GPU
+ * (other streams, other threads) race. */
+#define FLUID_MC_NOALLOC_MAX_PARTICLES 65536u
+int fluid_mc_generate_noalloc(fluid_sim *sim, const GpuMcGridParams *grid,
+                              const GpuAnisoParams *aniso,
+                              const GpuContainerGeometry *container, float *field);
+
 #ifdef __cplusplus
 }
 #endif
