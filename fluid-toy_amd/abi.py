@@ -149,6 +149,16 @@ class GpuParticleAniso(C.Structure):
     _fields_ = [("q0", C.c_float * 4), ("q1", C.c_float * 4), ("q2", C.c_float * 4)]
 
 
+class FluidDeviceView(C.Structure):
+    """fluid_device_view: device pointers of a handle's buffers (interop)."""
+    _fields_ = [("cell_counts", C.c_void_p), ("cell_starts", C.c_void_p),
+                ("num_cells", C.c_uint32), ("sorted_to_orig", C.c_void_p),
+                ("num_particles", C.c_uint32), ("first_particle", C.c_uint32),
+                ("sph_params", C.c_void_p), ("grid_params", C.c_void_p),
+                ("container_geometry", C.c_void_p), ("gravity", C.c_void_p),
+                ("stream", C.c_void_p)]
+
+
 def default_aniso_params(h: float, enabled: bool = True, strength: float = 1.0,
                          h_mc: float | None = None) -> GpuAnisoParams:
     """update_aniso_params (marching_cubes.rs:2133-2146) with its constants
@@ -242,6 +252,9 @@ PROTOTYPES = {
     "fluid_read_sph_params": (C.c_int, [P, C.POINTER(GpuSphParams3D)]),
     "fluid_read_container_geometry": (C.c_int, [P, C.POINTER(GpuContainerGeometry)]),
     "fluid_sync": (C.c_int, [P]),
+    "fluid_export_particles_device": (C.c_int, [P, P, u32]),
+    "fluid_export_sorted_particles_device": (C.c_int, [P, P, u32]),
+    "fluid_device_buffers": (C.c_int, [P, u32, C.POINTER(FluidDeviceView)]),
     "fluid_stream": (P, [P]),
     "fluid_time_steps": (C.c_int, [P, u32, u32, C.POINTER(C.c_float)]),
     "fluid_time_stages": (C.c_int, [P, u32, C.POINTER(C.c_float)]),

@@ -40,6 +40,17 @@ constexpr int kUniformRing = 16;
 
 }  // namespace
 
+// The uniform buffers of one scene in the reference's layouts (the blocks
+// behind sph_params_buffer, grid_params_buffer and container_geom_buffer,
+// sph_3d_grid.rs:1512-1526, plus gravity), kept on the device for callers
+// that bind them.
+struct RefBlock {
+  GpuSphParams3D sph;
+  GpuGridParams grid;
+  GpuContainerGeometry container;
+  GpuGravity gravity;
+};
+
 // A forward rollout kept for its reverse: state checkpoints every `every`
 // steps, the final state, one segment's tape and the adjoint buffers.
 struct RolloutState {
@@ -78,6 +89,8 @@ struct fluid_sim {
   uint32_t sdf_res = 0;
   DevUniforms *d_u = nullptr;     // one block per scene
   DevUniforms *h_ring = nullptr;  // pinned staging ring: kUniformRing x scenes
+  RefBlock *d_ref = nullptr;      // one block per scene, reference layouts
+  RefBlock *h_ref = nullptr;      // pinned staging ring of d_ref, same slots
   hipEvent_t ring_ev[kUniformRing] = {};
   bool ring_used[kUniformRing] = {};
   int ring_pos = 0;
@@ -87,6 +100,7 @@ struct fluid_sim {
   // canonical density / near density / normal per original index; the step
   // never writes these canonical fields (sph_density_3d_grid.wgsl:184-193)
   std::vector<float> extra;
+  float *d_extra = nullptr;  // the same five floats per original index, on the device
   hipGraphExec_t graph = nullptr;
   uint32_t graph_n = 0, graph_iters = 0;
   int graph_lanes = 0, graph_solve_lanes = 0, graph_xcd = 0, graph_tiled = 0, graph_wgs = 0;
@@ -166,13 +180,16 @@ int push_uniforms(fluid_sim *s) {
   if (s->ring_used[slot]) HIP_TRY(hipEventSynchronize(s->ring_ev[slot]));
   const size_t S = s->sc.size();
   DevUniforms *h = s->h_ring + (size_t)slot * S;
+  RefBlock *r = s->h_ref + (size_t)slot * S;
   for (size_t k = 0; k < S; ++k) {
     const SceneState &c = s->sc[k];
     fill_uniforms(h[k], c.params, c.grid, c.cont, c.grav, s->mouse, s->rb, s->sdf_res,
                   c.cell_offset);
+    r[k] = RefBlock{c.params, c.grid, c.cont, c.grav};
   }
   HIP_TRY(hipMemcpyAsync(s->d_u, h, S * sizeof(DevUniforms), hipMemcpyHostToDevice,
                          s->stream));
+  HIP_TRY(hipMemcpyAsync(s->d_ref, r, S * sizeof(RefBlock), hipMemcpyHostToDevice, s->stream));
   HIP_TRY(hipEventRecord(s->ring_ev[slot], s->stream));
   s->ring_used[slot] = true;
   return FLUID_OK;
@@ -319,6 +336,8 @@ int upload_canonical(fluid_sim *s, const SphParticle3D *p, uint32_t n) {
     HIP_TRY(hipMemcpyAsync(s->b.vel, vel.data(), n * 16, hipMemcpyHostToDevice, s->stream));
     HIP_TRY(hipMemcpyAsync(s->b.frc, frc.data(), n * 16, hipMemcpyHostToDevice, s->stream));
     HIP_TRY(hipMemcpyAsync(s->b.sid, sid.data(), n * 4, hipMemcpyHostToDevice, s->stream));
+    HIP_TRY(hipMemcpyAsync(s->d_extra, s->extra.data(), (size_t)n * 5 * 4,
+                           hipMemcpyHostToDevice, s->stream));
     HIP_TRY(hipStreamSynchronize(s->stream));
   }
   return FLUID_OK;
@@ -587,6 +606,54 @@ int d2h(fluid_sim *s, std::vector<T> &out, const void *src, size_t count) {
   return FLUID_OK;
 }
 
+// SphParticle3D records (64 B, four float4 stores each) written from slot-
+// order state into a caller's device buffer.  Canonical: slot k's particle
+// goes to record sid[k], with the canonical density / near density / normal
+// kept per original index (the step never writes those fields,
+// sph_density_3d_grid.wgsl:184-193).  Sorted: record k is slot k, with the
+// fields the density pass left in the neighbour records.
+__global__ __launch_bounds__(256) void k_export_canonical(
+    const float4 *__restrict__ pos, const float4 *__restrict__ vel,
+    const float4 *__restrict__ frc, const uint32_t *__restrict__ sid,
+    const float *__restrict__ extra, float4 *__restrict__ out, uint32_t n) {
+  const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  const uint32_t i = sid[k];
+  if (i >= n) return;
+  const float4 p = pos[k], v = vel[k], f = frc[k];
+  const float *e = extra + (size_t)i * 5;
+  float4 *o = out + (size_t)i * 4;
+  o[0] = make_float4(p.x, p.y, p.z, 0.0f);
+  o[1] = make_float4(v.x, v.y, v.z, 0.0f);
+  o[2] = make_float4(f.x, f.y, f.z, e[0]);
+  o[3] = make_float4(e[1], e[2], e[3], e[4]);
+}
+
+__global__ __launch_bounds__(256) void k_export_sorted(
+    const float4 *__restrict__ spos, const float4 *__restrict__ svel,
+    const float4 *__restrict__ sfrc, const float4 *__restrict__ srec,
+    float4 *__restrict__ out, uint32_t n) {
+  const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  const float4 p = spos[k], v = svel[k], f = sfrc[k];
+  const float rho = srec[3 * (size_t)k].w, near = srec[3 * (size_t)k + 1].w;
+  const float4 nr = srec[3 * (size_t)k + 2];
+  float4 *o = out + (size_t)k * 4;
+  o[0] = make_float4(p.x, p.y, p.z, 0.0f);
+  o[1] = make_float4(v.x, v.y, v.z, 0.0f);
+  o[2] = make_float4(f.x, f.y, f.z, rho);
+  o[3] = make_float4(near, nr.x, nr.y, nr.z);
+}
+
+int export_check(fluid_sim *s, const void *dst, uint32_t cap) {
+  if (!s || (!dst && s->n)) return fail(FLUID_ERR_INVALID_ARG, "null argument");
+  if (cap < s->n) return fail(FLUID_ERR_CAPACITY, "output smaller than num_particles");
+  if (((uintptr_t)dst & 15u) != 0)
+    return fail(FLUID_ERR_INVALID_ARG, "output must be 16-byte aligned device memory");
+  HIP_TRY(hipSetDevice(s->device));
+  return FLUID_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -697,6 +764,10 @@ static int create_handle(int device, std::vector<SceneState> scenes,
   if (e != hipSuccess) return bail(fail(FLUID_ERR_HIP, "hipMalloc pairs"));
   s->d_u = (DevUniforms *)dev_alloc(s, S * sizeof(DevUniforms), &e);
   if (e != hipSuccess) return bail(fail(FLUID_ERR_HIP, "hipMalloc uniforms"));
+  s->d_ref = (RefBlock *)dev_alloc(s, S * sizeof(RefBlock), &e);
+  if (e != hipSuccess) return bail(fail(FLUID_ERR_HIP, "hipMalloc uniform blocks"));
+  s->d_extra = (float *)dev_alloc(s, N * 5 * sizeof(float), &e);
+  if (e != hipSuccess) return bail(fail(FLUID_ERR_HIP, "hipMalloc canonical fields"));
   if (S > 1) {
     std::vector<uint16_t> scene_of(N, 0);
     for (size_t k = 0; k < S; ++k)
@@ -715,7 +786,8 @@ static int create_handle(int device, std::vector<SceneState> scenes,
     s->b.sdf = d;
     s->sdf_res = sdf_res;
   }
-  if (hipHostMalloc((void **)&s->h_ring, sizeof(DevUniforms) * kUniformRing * S) != hipSuccess)
+  if (hipHostMalloc((void **)&s->h_ring, sizeof(DevUniforms) * kUniformRing * S) != hipSuccess ||
+      hipHostMalloc((void **)&s->h_ref, sizeof(RefBlock) * kUniformRing * S) != hipSuccess)
     return bail(fail(FLUID_ERR_HIP, "hipHostMalloc"));
   for (int k = 0; k < kUniformRing; ++k)
     if (hipEventCreateWithFlags(&s->ring_ev[k], hipEventDisableTiming) != hipSuccess)
@@ -832,6 +904,7 @@ int fluid_destroy(fluid_sim *s) {
   for (int k = 0; k < kUniformRing; ++k)
     if (s->ring_ev[k]) (void)hipEventDestroy(s->ring_ev[k]);
   if (s->h_ring) (void)hipHostFree(s->h_ring);
+  if (s->h_ref) (void)hipHostFree(s->h_ref);
   if (s->stream) (void)hipStreamDestroy(s->stream);
   delete s;
   return FLUID_OK;
@@ -958,8 +1031,10 @@ int fluid_spawn_particles(fluid_sim *s, const float position[3], uint32_t count,
   HIP_TRY(hipMemcpyAsync(s->b.vel + s->n, zero.data(), c * 16, hipMemcpyHostToDevice, s->stream));
   HIP_TRY(hipMemcpyAsync(s->b.frc + s->n, zero.data(), c * 16, hipMemcpyHostToDevice, s->stream));
   HIP_TRY(hipMemcpyAsync(s->b.sid + s->n, sid.data(), c * 4, hipMemcpyHostToDevice, s->stream));
-  HIP_TRY(hipStreamSynchronize(s->stream));
   s->extra.resize((size_t)(s->n + c) * 5, 0.0f);
+  HIP_TRY(hipMemcpyAsync(s->d_extra + (size_t)s->n * 5, s->extra.data() + (size_t)s->n * 5,
+                         (size_t)c * 5 * 4, hipMemcpyHostToDevice, s->stream));
+  HIP_TRY(hipStreamSynchronize(s->stream));
   s->n += c;
   s->sc[0].count = s->n;
   s->sc[0].grid.num_particles = s->n;
@@ -1046,6 +1121,51 @@ int fluid_read_sorted_particles(fluid_sim *s, SphParticle3D *out, uint32_t cap) 
     o.normal[2] = hr[3 * (size_t)k + 2].z;
   }
   return check_errors(s);
+}
+
+int fluid_export_particles_device(fluid_sim *s, SphParticle3D *dst, uint32_t cap) {
+  const int rc = export_check(s, dst, cap);
+  if (rc || s->n == 0) return rc;
+  // the buffers holding each canonical field at the current stage boundary,
+  // as fluid_read_particles picks them
+  const int c = s->cursor;
+  const bool reordered = c >= FLUID_STAGE_REORDER;
+  const float4 *pos = reordered ? s->b.spos : s->b.pos;
+  const float4 *vel = !reordered ? s->b.vel : (c >= FLUID_STAGE_XSPH ? s->b.vel : s->b.svel);
+  const float4 *frc = !reordered ? s->b.frc : (c >= FLUID_STAGE_FORCE ? s->b.frc : s->b.sfrc);
+  hipLaunchKernelGGL(k_export_canonical, dim3((s->n + 255) / 256), dim3(256), 0, s->stream, pos,
+                     vel, frc, (const uint32_t *)s->b.sid, (const float *)s->d_extra,
+                     (float4 *)dst, s->n);
+  HIP_TRY(hipGetLastError());
+  return FLUID_OK;
+}
+
+int fluid_export_sorted_particles_device(fluid_sim *s, SphParticle3D *dst, uint32_t cap) {
+  const int rc = export_check(s, dst, cap);
+  if (rc || s->n == 0) return rc;
+  hipLaunchKernelGGL(k_export_sorted, dim3((s->n + 255) / 256), dim3(256), 0, s->stream,
+                     (const float4 *)s->b.spos, (const float4 *)s->b.svel,
+                     (const float4 *)s->b.sfrc, (const float4 *)s->b.srec, (float4 *)dst, s->n);
+  HIP_TRY(hipGetLastError());
+  return FLUID_OK;
+}
+
+int fluid_device_buffers(const fluid_sim *s, uint32_t scene, fluid_device_view *out) {
+  if (!s || !out) return fail(FLUID_ERR_INVALID_ARG, "null argument");
+  if (scene >= s->sc.size()) return fail(FLUID_ERR_INVALID_ARG, "scene out of range");
+  const SceneState &c = s->sc[scene];
+  out->cell_counts = s->b.counts + c.cell_offset;
+  out->cell_starts = s->b.starts + c.cell_offset;
+  out->num_cells = c.grid.total_cells;
+  out->sorted_to_orig = s->b.sid + c.first;
+  out->num_particles = c.count;
+  out->first_particle = c.first;
+  out->sph_params = &s->d_ref[scene].sph;
+  out->grid_params = &s->d_ref[scene].grid;
+  out->container_geometry = &s->d_ref[scene].container;
+  out->gravity = &s->d_ref[scene].gravity;
+  out->stream = (void *)s->stream;
+  return FLUID_OK;
 }
 
 static int read_u32(fluid_sim *s, const uint32_t *src, size_t count, uint32_t *out,

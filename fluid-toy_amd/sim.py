@@ -220,6 +220,24 @@ class SphSimulation3DGrid:
     def sync(self) -> None:
         self._chk(self._lib.fluid_sync(self._h))
 
+    # -- device-side access (the buffer getters, sph_3d_grid.rs:1500-1526) --
+    def export_particles_device(self, dst_ptr: int, cap: int) -> None:
+        """particle_buffer's records written to device memory at dst_ptr
+        (asynchronous on the handle's stream)."""
+        self._chk(self._lib.fluid_export_particles_device(self._h, C.c_void_p(dst_ptr), cap))
+
+    def export_sorted_particles_device(self, dst_ptr: int, cap: int) -> None:
+        """sorted_particle_buffer's records written to device memory."""
+        self._chk(self._lib.fluid_export_sorted_particles_device(self._h, C.c_void_p(dst_ptr),
+                                                                 cap))
+
+    def device_buffers(self, scene: int = 0) -> abi.FluidDeviceView:
+        """Device pointers of the cell counts / starts, the slot map and the
+        uniform blocks (reference layouts) of one scene."""
+        v = abi.FluidDeviceView()
+        self._chk(self._lib.fluid_device_buffers(self._h, scene, C.byref(v)))
+        return v
+
     # -- measurement -------------------------------------------------------
     def time_steps(self, warmup: int, steps: int) -> float:
         ms = C.c_float(0.0)

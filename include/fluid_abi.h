@@ -302,6 +302,45 @@ int fluid_read_sph_params(const fluid_sim *sim, GpuSphParams3D *out);
 int fluid_read_container_geometry(const fluid_sim *sim,
                                   GpuContainerGeometry *out);
 
+/* ---- device-side access (interop) -------------------------------------
+ * The reference hands its wgpu::Buffers to the spray system and the
+ * renderers (the getters of sph_3d_grid.rs:1500-1526, read by spray.rs:
+ * 105-113 and app.rs:1365,1430,1523-1528).  These are the same buffers on
+ * the device, for a consumer that binds them without a host round trip
+ * (e.g. memory it imported from a graphics API with hipImportExternalMemory
+ * or allocated with hipMalloc).  Work is enqueued on the handle's stream
+ * (fluid_stream) and the calls return without waiting: order consumers
+ * after it with a HIP event or fluid_sync. */
+
+/* particle_buffer: the canonical (original index order) SphParticle3D
+ * records written into `dst`, device memory of at least num_particles
+ * records, 16-byte aligned. */
+int fluid_export_particles_device(fluid_sim *sim, SphParticle3D *dst, uint32_t cap);
+/* sorted_particle_buffer: the records in the grid-sorted order of the last
+ * reorder, with the density, near density and normal of the last density
+ * pass (what the reference's density pass writes there). */
+int fluid_export_sorted_particles_device(fluid_sim *sim, SphParticle3D *dst, uint32_t cap);
+
+/* Device pointers the handle owns (valid until fluid_destroy; the step
+ * rewrites their contents).  For a batched handle, `scene` selects one
+ * scene's grid and uniform blocks; its cell_starts and sorted_to_orig hold
+ * indices into the whole concatenated particle array (subtract
+ * first_particle for the scene's own).  A plain handle has scene 0 only. */
+typedef struct fluid_device_view {
+  const uint32_t *cell_counts;        /* cell_counts_buffer, num_cells u32     */
+  const uint32_t *cell_starts;        /* cell_starts_buffer (inclusive scan)   */
+  uint32_t num_cells;
+  const uint32_t *sorted_to_orig;     /* the reorder's slot -> original index  */
+  uint32_t num_particles;
+  uint32_t first_particle;
+  const GpuSphParams3D *sph_params;   /* sph_params_buffer (64 B)              */
+  const GpuGridParams *grid_params;   /* grid_params_buffer (48 B)             */
+  const GpuContainerGeometry *container_geometry; /* container_geom_buffer     */
+  const GpuGravity *gravity;          /* the gravity uniform (16 B)            */
+  void *stream;                       /* the hipStream_t that writes them      */
+} fluid_device_view;
+int fluid_device_buffers(const fluid_sim *sim, uint32_t scene, fluid_device_view *out);
+
 /* Wait for all queued work of the handle. */
 int fluid_sync(fluid_sim *sim);
 
