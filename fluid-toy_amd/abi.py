@@ -275,6 +275,7 @@ PROTOTYPES = {
     "fluid_fusion": (u32, [P]),
     "fluid_forward_rollout": (C.c_int, [P, u32, u32]),
     "fluid_backward_rollout": (C.c_int, [P, P, P, P, P, P]),
+    "fluid_backward_rollout_scaled": (C.c_int, [P, P, P, P, P, P, P]),
     "fluid_rollout_timing": (C.c_int, [P, C.POINTER(C.c_float)]),
     "fluid_spray_create": (C.c_int, [P, u32, C.POINTER(GpuSprayParams), C.POINTER(P)]),
     "fluid_spray_destroy": (C.c_int, [P]),
@@ -317,6 +318,9 @@ FLUID_SOLVE_FIRST_X4 = 128
 # Gradient slots per scene written by fluid_backward_rollout.
 FLUID_GRAD_PARAMS = 5
 GRAD_PARAM_NAMES = ("viscosity", "surface_tension", "gravity_x", "gravity_y", "gravity_z")
+# ... and by fluid_backward_rollout_scaled (stiffness appended: always zero,
+# no stage of the step reads it).
+FLUID_GRAD_PARAMS_ALL = 6
 
 _LIB = None
 

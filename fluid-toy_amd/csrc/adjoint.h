@@ -59,6 +59,19 @@ hipError_t launch_adj_load(const uint32_t *sid, const float *dl_x3, const float 
 hipError_t launch_adj_store(const uint32_t *sid, const float4 *lx, const float4 *lv,
                             float *out_x3, float *out_v3, uint32_t n, hipStream_t st);
 
+// The carried adjoint state (lambda of position and velocity per slot, and
+// the parameter accumulators) as mantissa x 2^E(scene): max_cur receives each
+// scene's largest |lambda| (as float bits), e_step the exponent k with which
+// the state is then scaled by 2^-k, e_total the running sum of the k; max_next
+// (the other parity) is cleared for the next call.  scene_of null: one scene.
+hipError_t launch_adj_rescale_step(const uint16_t *scene_of, float4 *lx, float4 *lv,
+                                   double *acc, uint32_t n, uint32_t scenes, uint32_t *max_cur,
+                                   uint32_t *max_next, int32_t *e_total, int32_t *e_step,
+                                   hipStream_t st);
+// lambda (and acc when given) *= 2^(mul * e[scene]).
+hipError_t launch_adj_apply_scale(const uint16_t *scene_of, float4 *lx, float4 *lv, double *acc,
+                                  uint32_t n, const int32_t *e, int mul, hipStream_t st);
+
 // out[scene * kGradParams + q] = sum of acc[slot * kAccStride + q] over the
 // scene's slots [slot_first[scene], slot_first[scene + 1]).
 hipError_t launch_adj_reduce(const double *acc, const uint32_t *slot_first, uint32_t scenes,

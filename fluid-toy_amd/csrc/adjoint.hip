@@ -761,7 +761,112 @@ __global__ __launch_bounds__(256) void k_adj_reduce(const double *__restrict__ a
 
 inline uint32_t nblocks(uint64_t threads) { return (uint32_t)((threads + 255) / 256); }
 
+// ------------------------------------------------------------------------
+// Power-of-two rescaling of the carried adjoint state, per scene.  The
+// reverse of a violent rollout grows by 10^1-10^2 per step (the forward
+// map's stretching), past f32's range within ~16 steps.  The state carried
+// from step to step -- lambda of position and velocity, and the parameter
+// accumulators -- is therefore kept as mantissa x 2^E(scene): after every
+// reversed step its largest magnitude m is brought into [1, 2) by 2^-k,
+// k = floor(log2 m), and k is added to E on the device.  Scaling by a power
+// of two is exact (barring underflow of entries 2^-126 below the largest).
+
+// maxbits[scene] = bits of the largest |component| of the scene's lambdas
+// (non-negative floats order as their bit patterns).
+template <bool B>
+__global__ __launch_bounds__(256) void k_adj_absmax(const uint16_t *__restrict__ scene_of,
+                                                    const float4 *__restrict__ lx,
+                                                    const float4 *__restrict__ lv, uint32_t n,
+                                                    uint32_t *__restrict__ maxbits) {
+  const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+  float m = 0.0f;
+  uint32_t scene = 0;
+  if (s < n) {
+    const float4 a = lx[s], b = lv[s];
+    m = fmaxf(fmaxf(fmaxf(fabsf(a.x), fabsf(a.y)), fmaxf(fabsf(a.z), fabsf(b.x))),
+              fmaxf(fabsf(b.y), fabsf(b.z)));
+    if (B) scene = scene_of[s];
+  }
+  uint32_t bits = __float_as_uint(m);
+  const uint32_t first = (uint32_t)__builtin_amdgcn_readfirstlane((int)scene);
+  if (!B || __ballot(s < n && scene != first) == 0) {
+    // one scene in this wave: reduce, then one atomic
+    for (int o = 32; o >= 1; o >>= 1) bits = max(bits, (uint32_t)__shfl_xor((int)bits, o));
+    if ((threadIdx.x & 63u) == 0 && bits) atomicMax(&maxbits[first], bits);
+  } else if (s < n && bits) {
+    atomicMax(&maxbits[scene], bits);
+  }
+}
+
+// Per scene: k = floor(log2 m) (0 when the scene's lambdas are all zero or
+// not finite), E += k; clears the other parity's maxima for the next step.
+__global__ void k_adj_scale_update(const uint32_t *__restrict__ cur, uint32_t *__restrict__ next,
+                                   int32_t *__restrict__ e_total, int32_t *__restrict__ e_step,
+                                   uint32_t scenes) {
+  const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= scenes) return;
+  const uint32_t bits = cur[s];
+  const uint32_t ex = (bits >> 23) & 0xFFu;
+  int32_t k = 0;
+  if (bits != 0u && ex != 0xFFu) k = ex == 0u ? -126 : (int32_t)ex - 127;
+  e_step[s] = k;
+  e_total[s] += k;
+  next[s] = 0u;
+}
+
+// lambda *= 2^(mul * e[scene]); the accumulators too when acc is given.
+template <bool B>
+__global__ __launch_bounds__(256) void k_adj_rescale(const uint16_t *__restrict__ scene_of,
+                                                     float4 *__restrict__ lx,
+                                                     float4 *__restrict__ lv,
+                                                     double *__restrict__ acc, uint32_t n,
+                                                     const int32_t *__restrict__ e, int mul) {
+  const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n) return;
+  const int k = mul * e[B ? scene_of[s] : 0];
+  if (k == 0) return;
+  float4 a = lx[s], b = lv[s];
+  a = make_float4(ldexpf(a.x, k), ldexpf(a.y, k), ldexpf(a.z, k), a.w);
+  b = make_float4(ldexpf(b.x, k), ldexpf(b.y, k), ldexpf(b.z, k), b.w);
+  lx[s] = a;
+  lv[s] = b;
+  if (acc) {
+    double *q = acc + (size_t)s * kAccStride;
+#pragma unroll
+    for (int j = 0; j < kGradParams; ++j) q[j] = ldexp(q[j], k);
+  }
+}
+
 }  // namespace
+
+hipError_t launch_adj_rescale_step(const uint16_t *scene_of, float4 *lx, float4 *lv,
+                                   double *acc, uint32_t n, uint32_t scenes, uint32_t *max_cur,
+                                   uint32_t *max_next, int32_t *e_total, int32_t *e_step,
+                                   hipStream_t st) {
+  if (n == 0) return hipSuccess;
+  if (scene_of) {
+    hipLaunchKernelGGL(k_adj_absmax<true>, dim3(nblocks(n)), dim3(256), 0, st, scene_of, lx, lv,
+                       n, max_cur);
+  } else {
+    hipLaunchKernelGGL(k_adj_absmax<false>, dim3(nblocks(n)), dim3(256), 0, st, scene_of, lx,
+                       lv, n, max_cur);
+  }
+  hipLaunchKernelGGL(k_adj_scale_update, dim3(nblocks(scenes)), dim3(256), 0, st, max_cur,
+                     max_next, e_total, e_step, scenes);
+  return launch_adj_apply_scale(scene_of, lx, lv, acc, n, e_step, -1, st);
+}
+
+hipError_t launch_adj_apply_scale(const uint16_t *scene_of, float4 *lx, float4 *lv, double *acc,
+                                  uint32_t n, const int32_t *e, int mul, hipStream_t st) {
+  if (n == 0) return hipSuccess;
+  if (scene_of)
+    hipLaunchKernelGGL(k_adj_rescale<true>, dim3(nblocks(n)), dim3(256), 0, st, scene_of, lx, lv,
+                       acc, n, e, mul);
+  else
+    hipLaunchKernelGGL(k_adj_rescale<false>, dim3(nblocks(n)), dim3(256), 0, st, scene_of, lx,
+                       lv, acc, n, e, mul);
+  return hipGetLastError();
+}
 
 hipError_t launch_adj_reduce(const double *acc, const uint32_t *slot_first, uint32_t scenes,
                              double *out, hipStream_t st) {

@@ -412,6 +412,8 @@ struct AdjViews {
   uint32_t *slot_first;               // scenes + 1 slot offsets
   double *grads;                      // scenes x kGradParams
   uint32_t *pcnt;                     // in-range pair count per slot
+  uint32_t *maxbits;                  // 2 x scenes: largest |lambda| per scene (bits)
+  int32_t *e_total, *e_step;          // scenes: lambda scale exponents
 };
 
 AdjViews carve_adj(Carve &c, size_t n, size_t scenes) {
@@ -438,6 +440,9 @@ AdjViews carve_adj(Carve &c, size_t n, size_t scenes) {
   a.slot_first = c.take<uint32_t>(scenes + 1);
   a.grads = c.take<double>(scenes * kGradParams);
   a.pcnt = c.take<uint32_t>(n);
+  a.maxbits = c.take<uint32_t>(2 * scenes);
+  a.e_total = c.take<int32_t>(scenes);
+  a.e_step = c.take<int32_t>(scenes);
   return a;
 }
 
@@ -1632,8 +1637,18 @@ int fluid_forward_rollout(fluid_sim *s, uint32_t steps, uint32_t checkpoint_ever
   return FLUID_OK;
 }
 
-int fluid_backward_rollout(fluid_sim *s, const float *dl_dpos, const float *dl_dvel,
-                           float *dl_dpos0, float *dl_dvel0, double *dl_dparams) {
+}  // extern "C"
+
+namespace {
+
+// The reverse of the last forward rollout.  The carried adjoint state is
+// kept as mantissa x 2^E per scene (rescaled after every reversed step);
+// with log2_scale, the state gradients are returned as mantissas and E,
+// otherwise as values (2^E applied in f32).  Parameter gradients are
+// values (double).
+int backward_rollout(fluid_sim *s, const float *dl_dpos, const float *dl_dvel,
+                     float *dl_dpos0, float *dl_dvel0, double *dl_dparams,
+                     int32_t *log2_scale) {
   if (!s) return fail(FLUID_ERR_INVALID_ARG, "null handle");
   RolloutState &R = s->roll;
   if (!R.valid)
@@ -1649,13 +1664,26 @@ int fluid_backward_rollout(fluid_sim *s, const float *dl_dpos, const float *dl_d
   auto drop_events = [&]() {
     for (auto &e : ev) (void)hipEventDestroy(e);
   };
+  const size_t S = s->sc.size();
+  int par = 0;  // parity of the rescale maxima
+  auto rescale = [&](int c) -> int {
+    HIP_TRY(launch_adj_rescale_step(s->d_scene_of, A.lam_x[c], A.lam_v[c], A.acc, n,
+                                    (uint32_t)S, A.maxbits + par * S, A.maxbits + (1 - par) * S,
+                                    A.e_total, A.e_step, st));
+    par = 1 - par;
+    return FLUID_OK;
+  };
   HIP_TRY(hipEventRecord(ev[0], st));
+  HIP_TRY(hipMemsetAsync(A.maxbits, 0, 2 * S * sizeof(uint32_t), st));
+  HIP_TRY(hipMemsetAsync(A.e_total, 0, S * sizeof(int32_t), st));
   if (n) {
     if (dl_dpos) HIP_TRY(hipMemcpyAsync(A.in_x, dl_dpos, (size_t)n * 12, hipMemcpyDefault, st));
     if (dl_dvel) HIP_TRY(hipMemcpyAsync(A.in_v, dl_dvel, (size_t)n * 12, hipMemcpyDefault, st));
     HIP_TRY(launch_adj_load(s->b.sid, dl_dpos ? A.in_x : nullptr, dl_dvel ? A.in_v : nullptr,
                             A.lam_x[0], A.lam_v[0], n, st));
     HIP_TRY(hipMemsetAsync(A.acc, 0, (size_t)n * kAccStride * sizeof(double), st));
+    const int rc = rescale(0);
+    if (rc) return drop_events(), rc;
   }
   int cur = 0;
   const CkptView live = live_state(s);
@@ -1670,19 +1698,25 @@ int fluid_backward_rollout(fluid_sim *s, const float *dl_dpos, const float *dl_d
     }
     HIP_TRY(hipEventRecord(ev[2 + 3 * seg], st));
     for (int t = (int)len - 1; t >= 0; --t) {
-      const int rc = enqueue_adjoint_step(s, tape_slot(s, (uint32_t)t), A, cur);
+      int rc = enqueue_adjoint_step(s, tape_slot(s, (uint32_t)t), A, cur);
       if (rc) return drop_events(), rc;
       cur = 1 - cur;
+      if (n && (rc = rescale(cur))) return drop_events(), rc;
     }
     HIP_TRY(hipEventRecord(ev[3 + 3 * seg], st));
   }
-  const size_t S = s->sc.size();
   std::vector<uint32_t> slot_first(S + 1, 0);
   for (size_t k = 0; k < S; ++k) slot_first[k] = s->sc[k].first;
   slot_first[S] = n;
   if (S == 1) slot_first[0] = 0;
   std::vector<double> grads(S * kGradParams, 0.0);
+  std::vector<int32_t> e_total(S, 0);
   if (n) {
+    HIP_TRY(hipMemcpyAsync(e_total.data(), A.e_total, S * sizeof(int32_t),
+                           hipMemcpyDeviceToHost, st));
+    if (!log2_scale)  // values: mantissa x 2^E, rounded to f32 once
+      HIP_TRY(launch_adj_apply_scale(s->d_scene_of, A.lam_x[cur], A.lam_v[cur], nullptr, n,
+                                     A.e_total, 1, st));
     const uint32_t *sid0 = nseg ? ckpt_view(s, 0).sid : s->b.sid;
     HIP_TRY(launch_adj_store(sid0, A.lam_x[cur], A.lam_v[cur], A.out_x, A.out_v, n, st));
     if (dl_dpos0)
@@ -1712,8 +1746,34 @@ int fluid_backward_rollout(fluid_sim *s, const float *dl_dpos, const float *dl_d
   R.ms[0] = replay;
   R.ms[1] = reverse;
   R.ms[2] = total;
-  if (dl_dparams) std::memcpy(dl_dparams, grads.data(), grads.size() * sizeof(double));
+  // per scene: viscosity, surface tension, gravity x, y, z; with the scaled
+  // form also stiffness, which no stage of the step reads (its uniform is
+  // copied and never used), so its gradient is exactly zero
+  const int stride = log2_scale ? FLUID_GRAD_PARAMS_ALL : FLUID_GRAD_PARAMS;
+  if (dl_dparams)
+    for (size_t k = 0; k < S; ++k) {
+      for (int q = 0; q < kGradParams; ++q)
+        dl_dparams[k * stride + q] = std::ldexp(grads[k * kGradParams + q], e_total[k]);
+      if (log2_scale) dl_dparams[k * stride + kGradParams] = 0.0;
+    }
+  if (log2_scale) std::memcpy(log2_scale, e_total.data(), S * sizeof(int32_t));
   return check_errors(s);
+}
+
+}  // namespace
+
+extern "C" {
+
+int fluid_backward_rollout(fluid_sim *s, const float *dl_dpos, const float *dl_dvel,
+                           float *dl_dpos0, float *dl_dvel0, double *dl_dparams) {
+  return backward_rollout(s, dl_dpos, dl_dvel, dl_dpos0, dl_dvel0, dl_dparams, nullptr);
+}
+
+int fluid_backward_rollout_scaled(fluid_sim *s, const float *dl_dpos, const float *dl_dvel,
+                                  float *dl_dpos0, float *dl_dvel0, double *dl_dparams,
+                                  int32_t *log2_scale) {
+  if (!log2_scale) return fail(FLUID_ERR_INVALID_ARG, "null log2_scale");
+  return backward_rollout(s, dl_dpos, dl_dvel, dl_dpos0, dl_dvel0, dl_dparams, log2_scale);
 }
 
 int fluid_rollout_timing(const fluid_sim *s, float ms[3]) {

@@ -326,6 +326,32 @@ class SphSimulation3DGrid:
             gv.ctypes.data if gv is not None else None, gp.ctypes.data))
         return gx, gv, gp
 
+    def backward_rollout_scaled(self, dl_dpos=None, dl_dvel=None):
+        """Reverse the last forward_rollout keeping the state gradients as
+        mantissa and exponent (fluid_backward_rollout_scaled).  Returns
+        (mantissa of dL/dpos0 (n,3) float32, mantissa of dL/dvel0,
+        dL/dparams (num_scenes, 6): viscosity, surface tension, gravity x,
+        y, z, stiffness, and log2_scale (num_scenes,) int32): dL/dpos0 =
+        mantissa * 2**log2_scale[scene]."""
+        n = self.num_particles()
+        keep = []
+
+        def src(a):
+            if a is None:
+                return None
+            arr = np.ascontiguousarray(a, dtype=np.float32).reshape(n, 3)
+            keep.append(arr)
+            return arr.ctypes.data
+        gx = np.zeros((n, 3), np.float32)
+        gv = np.zeros((n, 3), np.float32)
+        scenes = max(self._lib.fluid_batch_num_scenes(self._h), 1)
+        gp = np.zeros((scenes, abi.FLUID_GRAD_PARAMS_ALL), np.float64)
+        e = np.zeros(scenes, np.int32)
+        self._chk(self._lib.fluid_backward_rollout_scaled(
+            self._h, src(dl_dpos), src(dl_dvel), gx.ctypes.data, gv.ctypes.data,
+            gp.ctypes.data, e.ctypes.data))
+        return gx, gv, gp, e
+
     def backward_rollout_device(self, dl_dpos_ptr: int, dl_dvel_ptr: int,
                                 out_pos_ptr: int, out_vel_ptr: int) -> np.ndarray:
         """backward_rollout on device buffers (addresses of (n, 3) float32

@@ -502,6 +502,23 @@ int fluid_backward_rollout(fluid_sim *sim, const float *dl_dpos,
                            const float *dl_dvel, float *dl_dpos0,
                            float *dl_dvel0, double *dl_dparams);
 
+/* As fluid_backward_rollout, with the state gradients returned as
+ * mantissa and exponent: dL/d(initial positions) = dl_dpos0 x 2^log2_scale
+ * [scene] (likewise dl_dvel0), log2_scale being one int32 per scene (not
+ * null).  The reverse of a violent flow grows the gradient by 10^1-10^2 per
+ * step, past f32's range within ~16 steps; the carried adjoint is kept as
+ * mantissa x 2^E throughout (fluid_backward_rollout does the same and only
+ * applies 2^E at the end, so its values overflow where the true gradient
+ * exceeds f32's range).  dl_dparams receives FLUID_GRAD_PARAMS_ALL values
+ * (double) per scene: the five of FLUID_GRAD_PARAMS, then stiffness, which
+ * no stage of the step reads (its uniform is copied and never used), so its
+ * gradient is exactly zero. */
+#define FLUID_GRAD_PARAMS_ALL 6
+int fluid_backward_rollout_scaled(fluid_sim *sim, const float *dl_dpos,
+                                  const float *dl_dvel, float *dl_dpos0,
+                                  float *dl_dvel0, double *dl_dparams,
+                                  int32_t *log2_scale);
+
 /* Timing of the last fluid_backward_rollout on the handle's stream (HIP
  * events): ms[0] replays, ms[1] reverse kernels, ms[2] whole call. */
 int fluid_rollout_timing(const fluid_sim *sim, float ms[3]);

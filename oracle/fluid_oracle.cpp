@@ -39,6 +39,22 @@
 
 namespace {
 
+// Branch-decision record (for the gradient checks): every non-smooth
+// decision of the force, solve and integrate stages that comes out "taken"
+// adds a hash of (kind, particle, partner or iteration, detail) to its
+// particle's slot of Oracle::sig.  Two runs whose sums agree took the same
+// branches (up to hash collisions), so a central difference between them
+// differentiates one smooth piece.
+inline uint64_t mix64(uint64_t z) {
+  z += 0x9E3779B97F4A7C15ull;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+inline uint64_t decision(uint32_t kind, uint32_t a, uint32_t b, int64_t c) {
+  return mix64(mix64(((uint64_t)kind << 32) ^ a) ^ ((uint64_t)b << 20) ^ (uint64_t)c);
+}
+
 template <class T> struct V3 {
   T x, y, z;
 };
@@ -141,6 +157,8 @@ template <class T> struct Oracle {
   std::vector<uint32_t> counts, starts, offsets, pci, s2o;
   std::vector<Pred<T>> pa, pb;
   int32_t accum[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  std::vector<uint64_t> sig;  // branch-decision record per original index
+  uint32_t solve_it = 0;      // PCISPH iteration being run (for the record)
 
   // ---- container_common.wgsl:27-54 -------------------------------------
   V3<T> world_to_local(V3<T> pos) const {
@@ -347,6 +365,7 @@ template <class T> struct Oracle {
       V3<T> n_i = P3(sorted[s].normal);
       T npi = near_k * ndi;
       V3<T> ap = mk(T(0), T(0), T(0)), av = ap, ac = ap;
+      uint64_t dec = 0;
       sweep(cl, [&](uint32_t j) {
         V3<T> pos_j = P3(sorted[j].pos);
         V3<T> r_vec = sub(pos_i, pos_j);
@@ -355,6 +374,8 @@ template <class T> struct Oracle {
         T r = std::sqrt(r_sq);
         V3<T> dir;
         T min_dist = h * T(0.1);
+        if (r < min_dist) dec += decision(1, orig, s2o[j], (int64_t)s - (int64_t)j);
+        if (r < h * T(0.5)) dec += decision(3, orig, s2o[j], 0);
         if (r < min_dist) {
           T idx = T((float)s) - T((float)j);
           dir = normalize(mk(std::sin(idx * T(1.0)), std::cos(idx * T(2.0)),
@@ -375,6 +396,7 @@ template <class T> struct Oracle {
         V3<T> v_ij = sub(vel_i, vel_j);
         V3<T> x_ij = sub(pos_i, pos_j);
         T vdx = dot(v_ij, x_ij);
+        if (vdx < T(0)) dec += decision(2, orig, s2o[j], 0);
         if (vdx < T(0)) {
           T mu = h * vdx / (r_sq + T(0.01) * h2);
           T avg = (di + dj) * T(0.5);
@@ -389,6 +411,7 @@ template <class T> struct Oracle {
       });
       ac = muls(ac, st);
       S3(particles[orig].force, add(add(add(ap, av), ac), gravity));
+      sig[orig] += dec;
     }
   }
 
@@ -437,11 +460,13 @@ template <class T> struct Oracle {
           }
         }
       });
+      if (pd - rho0 > T(0)) sig[orig] += decision(4, orig, solve_it, 0);
       T err = std::max(T(0), pd - rho0);
       T new_p = old_p + delta * err;
       T max_a = T(0.25) * h / (dt * dt);
       T amag = dot(ap, ap);
       T max_a2 = max_a * max_a;
+      if (amag > max_a2) sig[orig] += decision(5, orig, solve_it, 0);
       if (amag > max_a2) ap = muls(ap, max_a / std::sqrt(amag));
       V3<T> ov = P3(particles[orig].vel), op = P3(particles[orig].pos);
       V3<T> anp = P3(particles[orig].force);
@@ -457,6 +482,7 @@ template <class T> struct Oracle {
   }
   void solve() {
     for (uint32_t it = 0; it < iters; ++it) {
+      solve_it = it;
       if (it % 2 == 0)
         solve_iter(pa, pb);
       else
@@ -513,6 +539,15 @@ template <class T> struct Oracle {
       T wd = T(8.0);
       V3<T> wa = mk(T(0), T(0), T(0));
       T k = c.wall_stiffness;
+      {  // branch record: wall layers entered, their damping switches
+        const T dw[6] = {lp.x + c.hw, c.hw - lp.x, lp.y + c.hh, c.hh - lp.y, lp.z + c.hd,
+                         c.hd - lp.z};
+        const T into[6] = {-lv.x, lv.x, -lv.y, lv.y, -lv.z, lv.z};
+        for (uint32_t f = 0; f < 6; ++f) {
+          if (dw[f] < bl) sig[i] += decision(6, i, f, 0);
+          if (dw[f] < bl && into[f] > T(0)) sig[i] += decision(7, i, f, 0);
+        }
+      }
       {
         T d = lp.x - (-c.hw);
         if (d < bl) {
@@ -702,6 +737,13 @@ template <class T> struct Oracle {
       pos = add(pos, smul(dt, vel));
 
       V3<T> lpn = world_to_local(pos), lvn = world_dir_to_local(vel);
+      {  // branch record: backstop contacts
+        const T lq[3] = {lpn.x, lpn.y, lpn.z}, hq[3] = {c.hw, c.hh, c.hd};
+        for (uint32_t a = 0; a < 3; ++a) {
+          if (lq[a] < -hq[a]) sig[i] += decision(8, i, 2 * a, 0);
+          if (lq[a] > hq[a]) sig[i] += decision(8, i, 2 * a + 1, 0);
+        }
+      }
       T rest = c.damping;
       if (lpn.x < -c.hw) {
         lpn.x = -c.hw;
@@ -1480,6 +1522,7 @@ void *oracle_create(int fp64, const void *particles, uint32_t n,
     o.offsets.assign(o.cells, 0u);
     o.pci.assign(n, 0u);
     o.s2o.assign(n, 0u);
+    o.sig.assign(n, 0ull);
     o.pa.assign(n, Pred<T>{});
     o.pb.assign(n, Pred<T>{});
   };
@@ -1491,6 +1534,22 @@ void *oracle_create(int fp64, const void *particles, uint32_t n,
 }
 
 void oracle_destroy(void *h) { delete (Handle *)h; }
+
+// The branch-decision record per original index (n values).
+void oracle_decision_record(void *h, uint64_t *out) {
+  Handle *hd = (Handle *)h;
+  const std::vector<uint64_t> &v = hd->fp64 ? hd->d.sig : hd->f.sig;
+  std::copy(v.begin(), v.end(), out);
+}
+
+// Sum of the branch-decision record since creation (see decision()).
+uint64_t oracle_decision_signature(void *h) {
+  Handle *hd = (Handle *)h;
+  const std::vector<uint64_t> &v = hd->fp64 ? hd->d.sig : hd->f.sig;
+  uint64_t sum = 0;
+  for (uint64_t x : v) sum += x;
+  return sum;
+}
 
 void oracle_stage(void *h, uint32_t stage) {
   Handle *hd = (Handle *)h;

@@ -56,6 +56,9 @@ def _open(path: Path):
     lib.oracle_get_pred.argtypes = [P, C.c_int, P]
     lib.oracle_get_accum.argtypes = [P, P]
     lib.oracle_count_pairs.argtypes = [P, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]
+    lib.oracle_decision_signature.restype = C.c_uint64
+    lib.oracle_decision_signature.argtypes = [P]
+    lib.oracle_decision_record.argtypes = [P, P]
     lib.oracle_rest_density_f32.restype = C.c_float
     lib.oracle_rest_density_f32.argtypes = [C.c_float, C.c_float]
     lib.oracle_pcisph_delta_f32.restype = C.c_float
@@ -180,6 +183,19 @@ class Oracle:
     def accum(self) -> np.ndarray:
         out = np.zeros(8, dtype=np.int32)
         self._lib.oracle_get_accum(self._h, out.ctypes.data)
+        return out
+
+    def decision_signature(self) -> int:
+        """Hash-sum of the non-smooth branch decisions taken since creation
+        (close-range switch and its slot difference, viscosity on, Akinci
+        inner piece, positive density error, CFL clamp, wall layer and its
+        damping, backstop): equal for two runs that took the same branches."""
+        return int(self._lib.oracle_decision_signature(self._h))
+
+    def decision_record(self) -> np.ndarray:
+        """The branch-decision record per original index (uint64)."""
+        out = np.zeros(self.n, dtype=np.uint64)
+        self._lib.oracle_decision_record(self._h, out.ctypes.data)
         return out
 
     def count_pairs(self) -> tuple[int, int]:
