@@ -224,6 +224,130 @@ __device__ __forceinline__ void sweep27(const DevUniforms *__restrict__ U,
 }
 
 // ------------------------------------------------------------------------
+// The 27-cell stencil as one concatenated run of candidates: the nine
+// x-rows of sweep27, in its order, end to end.  Lane `sub` of an L-lane
+// group takes candidates sub, sub + L, sub + 2L, ... of the concatenation,
+// so its share runs on across row boundaries: rows shorter than the group
+// leave no lane idle, every trip's four loads are independent of the rows,
+// and a lane's trips number ceil(candidates / 4L) instead of the sum of
+// every row's.  Each row is (first slot, length, index of the first slot in
+// the array the candidates are loaded from); a lane reads the next row only
+// when its share crosses into it.
+struct RowEnt {
+  uint32_t gb, len, lb;
+};
+
+template <int L>
+__device__ __forceinline__ uint32_t group_sum_u32(uint32_t v) {
+#pragma unroll
+  for (int o = L / 2; o >= 1; o >>= 1) v += (uint32_t)__shfl_xor((int)v, o, L);
+  return v;
+}
+
+// Whether any lane of this lane's L-lane group has `p` (all of them call it).
+template <int L>
+__device__ __forceinline__ bool group_any(bool p) {
+  const uint64_t bal = __ballot(p);
+  if constexpr (L >= 64) return bal != 0;
+  const uint32_t lane = threadIdx.x & 63u;
+  return ((bal >> (lane & ~(uint32_t)(L - 1))) & ((1ull << L) - 1ull)) != 0;
+}
+
+// The row table of the 27-cell stencil of cell (cx, cy, cz), written into
+// the group's nine LDS entries `tab` by the group's lanes; every lane of the
+// group must call it.  Returns the group's candidate count.
+template <int L>
+__device__ __forceinline__ uint32_t rows27(const DevUniforms *__restrict__ U,
+                                           const uint32_t *__restrict__ counts,
+                                           const uint32_t *__restrict__ starts, int cx, int cy,
+                                           int cz, uint32_t sub, RowEnt *tab) {
+  const int gx = (int)U->gx, gy = (int)U->gy, gz = (int)U->gz;
+  cx = min(max(cx, -2), gx + 1);
+  cy = min(max(cy, -2), gy + 1);
+  cz = min(max(cz, -2), gz + 1);
+  const int x0 = max(cx - 1, 0), x1 = min(cx + 1, gx - 1);
+  uint32_t mine = 0;
+  for (uint32_t q = sub; q < 9u; q += L) {
+    // row q: dz = q / 3 - 1 (outer), dy = q % 3 - 1, the reference's order
+    const int z = cz + (int)(q / 3u) - 1, y = cy + (int)(q % 3u) - 1;
+    uint32_t gb = 0, len = 0;
+    if (x0 <= x1 && z >= 0 && z < gz && y >= 0 && y < gy) {
+      const uint32_t row = U->cell_offset + (uint32_t)y * U->gx + (uint32_t)z * U->gxy;
+      const uint32_t c0 = row + (uint32_t)x0, c1 = row + (uint32_t)x1;
+      gb = starts[c0] - counts[c0];
+      len = starts[c1] - gb;
+    }
+    tab[q] = RowEnt{gb, len, gb};
+    mine += len;
+  }
+  // the group's lanes read each other's entries (one wave: in order)
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  return group_sum_u32<L>(mine);
+}
+
+// Walk a group's concatenated rows (row(r) gives row r, total their summed
+// length).  pts is indexed by RowEnt::lb + offset; `safe` is any valid index
+// of pts, loaded in place of the candidates past the end.
+//   SYNC = false: each lane walks its own share and f(j, p_j, true) sees
+//                 real candidates only.
+//   SYNC = true:  the group's lanes make the same ceil(total / 4L) trips,
+//                 candidates past the end passed with valid = false, so
+//                 group-wide ballots inside f(j, p_j, valid) see every lane.
+template <int L, bool SYNC, class T, class Row, class F>
+__device__ __forceinline__ void walk_rows(Row &&row, uint32_t total, const T *__restrict__ pts,
+                                          uint32_t sub, uint32_t safe, F &&f) {
+  uint32_t r = 0, o = sub;
+  RowEnt e = row(0u);
+  auto settle = [&]() {
+    while (r < 9u && o >= e.len) {
+      o -= e.len;
+      if (++r < 9u) e = row(r);
+    }
+  };
+  settle();
+  if (SYNC) {
+    for (uint32_t base = 0; base < total; base += 4u * L) {
+      uint32_t j[4], li[4];
+      bool v[4];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        v[q] = r < 9u;
+        j[q] = e.gb + o;
+        li[q] = v[q] ? e.lb + o : safe;
+        o += L;
+        settle();
+      }
+      T p[4];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) p[q] = pts[li[q]];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) f(j[q], p[q], v[q]);
+    }
+  } else {
+    while (r < 9u) {
+      uint32_t j[4], li[4];
+      bool v[4];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        v[q] = r < 9u;
+        j[q] = e.gb + o;
+        li[q] = v[q] ? e.lb + o : safe;
+        o += L;
+        settle();
+      }
+      T p[4];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) p[q] = pts[li[q]];
+#pragma unroll
+      for (int q = 0; q < 4; ++q)
+        if (v[q]) f(j[q], p[q], true);
+    }
+  }
+}
+
+// ------------------------------------------------------------------------
 // Cell tiles.  A tile is a run of up to w (<= kTileWMax) cells along x of
 // one grid row (y, z) of one scene.  Its particles' 27-cell stencils cover
 // nine x-rows (y-1..y+1, z-1..z+1) of cells xa = x0-1 .. xb = x1+1, each a
@@ -344,6 +468,31 @@ __device__ __forceinline__ void tile_sweep(const TileShared &S, const T *pts, bo
     }
   }
 }
+
+// Row r of the 27-cell stencil of a particle in the tile with cell x
+// coordinate cx, for walk_rows: slots from the tile's row table, loaded from
+// the staged copy (staged) or the global array.
+struct TileRows {
+  const TileShared &S;
+  int lo, hi;
+  bool staged;
+  __device__ __forceinline__ TileRows(const TileShared &s_, int cx, bool st)
+      : S(s_), lo(max(cx - 1, s_.xa) - s_.xa), hi(min(cx + 1, s_.xb) - s_.xa + 1), staged(st) {}
+  __device__ __forceinline__ RowEnt operator()(uint32_t r) const {
+    const uint32_t b0 = S.goff[r][lo], b1 = S.goff[r][hi];
+    return RowEnt{b0, b1 - b0, staged ? b0 + S.rowl[r] - S.goff[r][0] : b0};
+  }
+  __device__ __forceinline__ uint32_t total() const {
+    uint32_t t = 0;
+#pragma unroll
+    for (int r = 0; r < 9; ++r) t += S.goff[r][hi] - S.goff[r][lo];
+    return t;
+  }
+  // the load index of slot s, one of the tile's own particles (row 4)
+  __device__ __forceinline__ uint32_t own(uint32_t s) const {
+    return staged ? s + S.rowl[4] - S.goff[4][0] : s;
+  }
+};
 
 // Walk this lane's share of particle s's in-range list, or, if the list
 // overflowed, the lane's candidate subset again with the same test; f(j, p_j)

@@ -352,6 +352,7 @@ __device__ __forceinline__ void density_particle(const DevUniforms *__restrict__
 
 template <int L, bool B>
 __global__ __launch_bounds__(256) void k_density(Uni un, Bufs b, uint32_t n, uint32_t xcd) {
+  __shared__ RowEnt rows[256 / L][9];
   const uint32_t g = xcd_block(xcd) * blockDim.x + threadIdx.x;
   const uint32_t s = g / L, sub = g % L;
   if (s >= n) return;
@@ -360,8 +361,10 @@ __global__ __launch_bounds__(256) void k_density(Uni un, Bufs b, uint32_t n, uin
   const int cx = cell_coord(pi.x, U->ox, U->inv_cell);
   const int cy = cell_coord(pi.y, U->oy, U->inv_cell);
   const int cz = cell_coord(pi.z, U->oz, U->inv_cell);
+  RowEnt *tab = rows[threadIdx.x / L];
+  const uint32_t total = rows27<L>(U, b.counts, b.starts, cx, cy, cz, sub, tab);
   density_particle<L>(U, b, n, s, sub, pi, [&](auto &&f) {
-    sweep27<L, true>(U, b.counts, b.starts, b.spos, cx, cy, cz, sub, f);
+    walk_rows<L, true>([&](uint32_t r) { return tab[r]; }, total, b.spos, sub, s, f);
   });
 }
 
@@ -374,6 +377,7 @@ template <int L, bool B>
 __global__ __launch_bounds__(256) void k_density_t(Uni un, Bufs b, uint32_t n) {
   extern __shared__ float4 stage[];
   __shared__ TileShared S;
+  __shared__ RowEnt rows[256 / L][9];
   const uint32_t count = *b.tiles.count;
   for (uint32_t t = blockIdx.x; t < count; t += gridDim.x) {
     const uint4 e = b.tiles.list[t];
@@ -391,12 +395,16 @@ __global__ __launch_bounds__(256) void k_density_t(Uni un, Bufs b, uint32_t n) {
       const int cy = cell_coord(pi.y, U->oy, U->inv_cell);
       const int cz = cell_coord(pi.z, U->oz, U->inv_cell);
       if (staged && tile_owns(S, U, cx, cy, cz)) {
+        const TileRows tr(S, cx, true);
+        const uint32_t total = tr.total();
         density_particle<L>(U, b, n, s, sub, pi, [&](auto &&f) {
-          tile_sweep<L, true>(S, stage, true, cx, sub, f);
+          walk_rows<L, true>(tr, total, stage, sub, tr.own(s), f);
         });
       } else {
+        RowEnt *tab = rows[threadIdx.x / L];
+        const uint32_t total = rows27<L>(U, b.counts, b.starts, cx, cy, cz, sub, tab);
         density_particle<L>(U, b, n, s, sub, pi, [&](auto &&f) {
-          sweep27<L, true>(U, b.counts, b.starts, b.spos, cx, cy, cz, sub, f);
+          walk_rows<L, true>([&](uint32_t r) { return tab[r]; }, total, b.spos, sub, s, f);
         });
       }
     }
@@ -710,10 +718,25 @@ __device__ __forceinline__ void solve_particle(const DevUniforms *__restrict__ U
   out.v[s] = make_float4(vx, vy, vz, 0.f);
 }
 
+// The row table of a solve sweep, built (by the whole group) only when some
+// lane of the group sweeps: in the first iteration, and in a later one for
+// a lane whose skin list (list lane sub % LS) overflowed.
+template <int L, int LS>
+__device__ __forceinline__ uint32_t solve_rows(const DevUniforms *__restrict__ U, const Bufs &b,
+                                               uint32_t s, uint32_t sub, int cx, int cy, int cz,
+                                               uint32_t first_iter, uint32_t use_list,
+                                               RowEnt *tab) {
+  const bool need = first_iter || !use_list ||
+                    b.scnt[(size_t)s * LS + sub % LS] == kSolveListOverflow;
+  if (!group_any<L>(need)) return 0u;
+  return rows27<L>(U, b.counts, b.starts, cx, cy, cz, sub, tab);
+}
+
 template <int L, bool B>
 __global__ __launch_bounds__(256) void k_solve(Uni un, Bufs b, uint32_t n, PredBuf in,
                                                PredBuf out, float4 *ap_out, uint32_t xcd,
                                                uint32_t first_iter, uint32_t use_list) {
+  __shared__ RowEnt rows[256 / L][9];
   const uint32_t g = xcd_block(xcd) * blockDim.x + threadIdx.x;
   const uint32_t s = g / L, sub = g % L;
   if (s >= n) return;
@@ -723,8 +746,11 @@ __global__ __launch_bounds__(256) void k_solve(Uni un, Bufs b, uint32_t n, PredB
   const int cx = cell_coord(p0.x, U->ox, U->inv_cell);
   const int cy = cell_coord(p0.y, U->oy, U->inv_cell);
   const int cz = cell_coord(p0.z, U->oz, U->inv_cell);
+  RowEnt *tab = rows[threadIdx.x / L];
+  const uint32_t total =
+      solve_rows<L, L>(U, b, s, sub, cx, cy, cz, first_iter, use_list, tab);
   solve_particle<L>(U, b, n, in, out, ap_out, first_iter, use_list, s, sub, xi, p0, [&](auto &&f) {
-    sweep27<L, false>(U, b.counts, b.starts, in.xq, cx, cy, cz, sub, f);
+    walk_rows<L, false>([&](uint32_t r) { return tab[r]; }, total, in.xq, sub, s, f);
   });
 }
 
@@ -733,6 +759,7 @@ __global__ __launch_bounds__(256) void k_solve(Uni un, Bufs b, uint32_t n, PredB
 template <int L, int LS, bool B>
 __global__ __launch_bounds__(256) void k_solve_w(Uni un, Bufs b, uint32_t n, PredBuf in,
                                                  PredBuf out, float4 *ap_out, uint32_t xcd) {
+  __shared__ RowEnt rows[256 / L][9];
   const uint32_t g = xcd_block(xcd) * blockDim.x + threadIdx.x;
   const uint32_t s = g / L, sub = g % L;
   if (s >= n) return;
@@ -742,8 +769,10 @@ __global__ __launch_bounds__(256) void k_solve_w(Uni un, Bufs b, uint32_t n, Pre
   const int cx = cell_coord(p0.x, U->ox, U->inv_cell);
   const int cy = cell_coord(p0.y, U->oy, U->inv_cell);
   const int cz = cell_coord(p0.z, U->oz, U->inv_cell);
+  RowEnt *tab = rows[threadIdx.x / L];
+  const uint32_t total = solve_rows<L, LS>(U, b, s, sub, cx, cy, cz, 0u, 1u, tab);
   auto sweep = [&](auto &&f) {
-    sweep27<L, false>(U, b.counts, b.starts, in.xq, cx, cy, cz, sub, f);
+    walk_rows<L, false>([&](uint32_t r) { return tab[r]; }, total, in.xq, sub, s, f);
   };
   solve_particle<L, decltype(sweep) &, LS>(U, b, n, in, out, ap_out, 0u, 1u, s, sub, xi, p0,
                                            sweep);
@@ -754,6 +783,7 @@ __global__ __launch_bounds__(256) void k_solve_w(Uni un, Bufs b, uint32_t n, Pre
 template <int L, int LS, bool B>
 __global__ __launch_bounds__(256) void k_solve_fw(Uni un, Bufs b, uint32_t n, PredBuf in,
                                                   PredBuf out, float4 *ap_out, uint32_t xcd) {
+  __shared__ RowEnt rows[256 / L][9];
   const uint32_t g = xcd_block(xcd) * blockDim.x + threadIdx.x;
   const uint32_t s = g / L, sub = g % L;
   if (s >= n) return;
@@ -763,8 +793,10 @@ __global__ __launch_bounds__(256) void k_solve_fw(Uni un, Bufs b, uint32_t n, Pr
   const int cx = cell_coord(p0.x, U->ox, U->inv_cell);
   const int cy = cell_coord(p0.y, U->oy, U->inv_cell);
   const int cz = cell_coord(p0.z, U->oz, U->inv_cell);
+  RowEnt *tab = rows[threadIdx.x / L];
+  const uint32_t total = rows27<L>(U, b.counts, b.starts, cx, cy, cz, sub, tab);
   auto sweep = [&](auto &&f) {
-    sweep27<L, true>(U, b.counts, b.starts, in.xq, cx, cy, cz, sub, f);
+    walk_rows<L, true>([&](uint32_t r) { return tab[r]; }, total, in.xq, sub, s, f);
   };
   solve_particle<L, decltype(sweep) &, LS>(U, b, n, in, out, ap_out, 1u, 1u, s, sub, xi, p0,
                                            sweep);
@@ -778,6 +810,7 @@ __global__ __launch_bounds__(256) void k_solve_t(Uni un, Bufs b, uint32_t n, Pre
                                                  uint32_t first_iter, uint32_t use_list) {
   extern __shared__ float4 stage[];
   __shared__ TileShared S;
+  __shared__ RowEnt rows[256 / L][9];
   const uint32_t count = *b.tiles.count;
   for (uint32_t t = blockIdx.x; t < count; t += gridDim.x) {
     const uint4 e = b.tiles.list[t];
@@ -796,12 +829,19 @@ __global__ __launch_bounds__(256) void k_solve_t(Uni un, Bufs b, uint32_t n, Pre
       const int cy = cell_coord(p0.y, U->oy, U->inv_cell);
       const int cz = cell_coord(p0.z, U->oz, U->inv_cell);
       if (staged && tile_owns(S, U, cx, cy, cz)) {
-        solve_particle<L>(U, b, n, in, out, ap_out, first_iter, use_list, s, sub, xi, p0, [&](auto &&f) {
-          tile_sweep<L, false>(S, stage, true, cx, sub, f);
+        const TileRows tr(S, cx, true);
+        const uint32_t total = tr.total();
+        solve_particle<L>(U, b, n, in, out, ap_out, first_iter, use_list, s, sub, xi, p0,
+                          [&](auto &&f) {
+          walk_rows<L, false>(tr, total, stage, sub, tr.own(s), f);
         });
       } else {
-        solve_particle<L>(U, b, n, in, out, ap_out, first_iter, use_list, s, sub, xi, p0, [&](auto &&f) {
-          sweep27<L, false>(U, b.counts, b.starts, in.xq, cx, cy, cz, sub, f);
+        RowEnt *tab = rows[threadIdx.x / L];
+        const uint32_t total =
+            solve_rows<L, L>(U, b, s, sub, cx, cy, cz, first_iter, use_list, tab);
+        solve_particle<L>(U, b, n, in, out, ap_out, first_iter, use_list, s, sub, xi, p0,
+                          [&](auto &&f) {
+          walk_rows<L, false>([&](uint32_t r) { return tab[r]; }, total, in.xq, sub, s, f);
         });
       }
     }
