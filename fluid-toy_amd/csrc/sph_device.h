@@ -237,6 +237,12 @@ struct RowEnt {
   uint32_t gb, len, lb;
 };
 
+// Candidates loaded together per trip of walk_rows.
+#ifndef FLUID_WALK_LOADS
+#define FLUID_WALK_LOADS 4
+#endif
+constexpr int kWalkLoads = FLUID_WALK_LOADS;
+
 template <int L>
 __device__ __forceinline__ uint32_t group_sum_u32(uint32_t v) {
 #pragma unroll
@@ -295,53 +301,65 @@ __device__ __forceinline__ uint32_t rows27(const DevUniforms *__restrict__ U,
 //   SYNC = true:  the group's lanes make the same ceil(total / 4L) trips,
 //                 candidates past the end passed with valid = false, so
 //                 group-wide ballots inside f(j, p_j, valid) see every lane.
-template <int L, bool SYNC, class T, class Row, class F>
+template <int L, bool SYNC, class T, class Row, class F, int U = kWalkLoads>
 __device__ __forceinline__ void walk_rows(Row &&row, uint32_t total, const T *__restrict__ pts,
                                           uint32_t sub, uint32_t safe, F &&f) {
   uint32_t r = 0, o = sub;
   RowEnt e = row(0u);
+  // move (r, o) on to the row holding offset o (past the last row: r = 9,
+  // e.len = 0); a loop, not unrolled: crossings are rare
   auto settle = [&]() {
+#pragma clang loop unroll(disable)
     while (r < 9u && o >= e.len) {
       o -= e.len;
-      if (++r < 9u) e = row(r);
+      ++r;
+      e = r < 9u ? row(r) : RowEnt{0u, 0u, 0u};
     }
   };
   settle();
-  if (SYNC) {
-    for (uint32_t base = 0; base < total; base += 4u * L) {
-      uint32_t j[4], li[4];
-      bool v[4];
+  auto trip = [&](uint32_t (&j)[U], uint32_t (&li)[U], bool (&v)[U]) {
+    if (o + (uint32_t)(U - 1) * L < e.len) {
+      // the whole trip lies in the current row
 #pragma unroll
-      for (int q = 0; q < 4; ++q) {
+      for (int q = 0; q < U; ++q) {
+        v[q] = true;
+        j[q] = e.gb + o + (uint32_t)q * L;
+        li[q] = e.lb + o + (uint32_t)q * L;
+      }
+      o += (uint32_t)U * L;
+      settle();
+    } else {
+#pragma unroll
+      for (int q = 0; q < U; ++q) {
         v[q] = r < 9u;
         j[q] = e.gb + o;
         li[q] = v[q] ? e.lb + o : safe;
         o += L;
         settle();
       }
-      T p[4];
+    }
+  };
+  if (SYNC) {
+    for (uint32_t base = 0; base < total; base += (uint32_t)U * L) {
+      uint32_t j[U], li[U];
+      bool v[U];
+      trip(j, li, v);
+      T p[U];
 #pragma unroll
-      for (int q = 0; q < 4; ++q) p[q] = pts[li[q]];
+      for (int q = 0; q < U; ++q) p[q] = pts[li[q]];
 #pragma unroll
-      for (int q = 0; q < 4; ++q) f(j[q], p[q], v[q]);
+      for (int q = 0; q < U; ++q) f(j[q], p[q], v[q]);
     }
   } else {
     while (r < 9u) {
-      uint32_t j[4], li[4];
-      bool v[4];
+      uint32_t j[U], li[U];
+      bool v[U];
+      trip(j, li, v);
+      T p[U];
 #pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        v[q] = r < 9u;
-        j[q] = e.gb + o;
-        li[q] = v[q] ? e.lb + o : safe;
-        o += L;
-        settle();
-      }
-      T p[4];
+      for (int q = 0; q < U; ++q) p[q] = pts[li[q]];
 #pragma unroll
-      for (int q = 0; q < 4; ++q) p[q] = pts[li[q]];
-#pragma unroll
-      for (int q = 0; q < 4; ++q)
+      for (int q = 0; q < U; ++q)
         if (v[q]) f(j[q], p[q], true);
     }
   }
@@ -541,10 +559,18 @@ __device__ __forceinline__ void for_each_neighbour(const DevUniforms *__restrict
         if (!more) break;
       }
     }
-    for (; k < cnt; ++k) {
-      const uint32_t j0 = nbr[0];
-      nbr += kStep;
-      f(j0, srec[3 * (size_t)j0]);
+    // the remaining (fewer than U) pairs as one predicated chunk: their list
+    // entries, then their records, each in one round trip
+    if (k < cnt) {
+      uint32_t jt[U];
+#pragma unroll
+      for (int q = 0; q < U; ++q) jt[q] = k + q < cnt ? nbr[q * kStep] : s;
+      float4 p[U];
+#pragma unroll
+      for (int q = 0; q < U; ++q) p[q] = srec[3 * (size_t)jt[q]];
+#pragma unroll
+      for (int q = 0; q < U; ++q)
+        if (k + q < cnt) f(jt[q], p[q]);
     }
     return;
   }

@@ -684,9 +684,16 @@ __device__ __forceinline__ void solve_particle(const DevUniforms *__restrict__ U
         body(j2, x2, true);
         body(j3, x3, true);
       }
-      for (; k < cnt; ++k, p += kStep) {
-        const uint32_t j0 = p[0];
-        body(j0, xq[j0], true);
+      if (k < cnt) {  // the last (fewer than four) entries as one predicated trip
+        uint32_t jt[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) jt[q] = k + q < cnt ? p[q * kStep] : s;
+        float4 xt[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) xt[q] = xq[jt[q]];
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+          if (k + q < cnt) body(jt[q], xt[q], true);
       }
     } else {
       sweep(body);
