@@ -39,9 +39,14 @@ STAGE_BYTES = {
     "solve": (92, 8), "finalize": (52, 0), "integrate": (48, 0),
 }
 # Algorithmic flops per candidate pair (distance test) and per in-range pair,
-# counted from the WGSL bodies (sqrt and divide as one flop each).
+# counted from the WGSL bodies (sqrt and divide as one flop each).  Only the
+# passes that test candidates are charged the distance test: density, and the
+# first PCISPH iteration (one of `iters` solve launches).  XSPH and force walk
+# the density pass's in-range lists, and the later PCISPH iterations walk the
+# first one's skin lists, testing only the listed pairs (not charged).
 FLOPS_CAND = 8
 FLOPS_INRANGE = {"density": 24, "xsph": 16, "force": 68, "solve": 30}
+CAND_LAUNCHES = {"density": 1, "solve": 1}
 
 
 def stage_launches(stage: str, iters: int) -> int:
@@ -93,7 +98,10 @@ def roofline_for(stages, n, cells, iters, cand, inr, lanes=None):
         entry = {"ms": round(ms, 5), "launches": launches, "alg_bytes_per_launch": bytes_per_launch,
                  "hbm_gbs": round(gbs, 1), "hbm_frac": round(gbs / HBM_PEAK_GBS, 4)}
         if name in FLOPS_INRANGE:
-            flops = FLOPS_CAND * cand + FLOPS_INRANGE[name] * inr
+            # per launch: the in-range work, plus the distance tests of the
+            # launches that sweep candidates, averaged over the stage's launches
+            flops = int(FLOPS_INRANGE[name] * inr +
+                        FLOPS_CAND * cand * CAND_LAUNCHES.get(name, 0) / launches)
             tf = flops / t_launch / 1e12 if t_launch > 0 else 0.0
             entry.update({"alg_flops_per_launch": flops, "tflops": round(tf, 2),
                           "valu_frac": round(tf / FP32_PEAK_TFLOPS, 4)})
@@ -526,6 +534,9 @@ def main() -> int:
         ms1, st1, c1, i1, cells1, it1 = run_gpu(pkg, big, 5, k1, min(args.stage_steps, 10))
         roof1, _ = roofline_for(st1, big.n, cells1, it1, c1, i1)
         line[name] = {"particles": big.n, "steps": k1, "grid_cells": cells1,
+                      "grid": "the reference's dense grid (cell = h, ceil(2 extent / h) + 2 "
+                              "cells per axis; its counts and starts are part of the readback "
+                              "surface), not a 2^22 / 2^24-cell hash",
                       "value": round(big.n * k1 / (ms1 / 1e3), 1),
                       "ms_per_step": round(ms1 / k1, 4),
                       "lanes_per_particle": list(run_gpu.lanes),
