@@ -156,17 +156,18 @@ __device__ __forceinline__ uint32_t xcd_block(uint32_t remap) {
 // x-rows: cells (x-1..x+1, y, z) are consecutive cell indices, so their
 // particles form one contiguous sorted range.  The visit order (dz, dy, dx,
 // then slot) is the reference's.  Lane `sub` of an L-lane group takes every
-// L-th candidate of each row, four gathers in flight per trip.
+// L-th candidate of each row, four gathers in flight per trip (UN with SYNC
+// = false).
 //   SYNC = false: each lane stops at its own last candidate and f(j, p_j,
 //                 true) only sees real candidates.
 //   SYNC = true:  all L lanes of a particle make the same trips, the tail
 //                 predicated by `valid`, so group-wide ballots inside
 //                 f(j, p_j, valid) see every lane of the group.
-template <int L, bool SYNC, class F>
+template <int L, bool SYNC, int UN = 4, class T, class F>
 __device__ __forceinline__ void sweep27(const DevUniforms *__restrict__ U,
                                         const uint32_t *__restrict__ counts,
                                         const uint32_t *__restrict__ starts,
-                                        const float4 *__restrict__ pts,
+                                        const T *__restrict__ pts,
                                         int cx, int cy, int cz, uint32_t sub,
                                         F &&f) {
   const int gx = (int)U->gx, gy = (int)U->gy, gz = (int)U->gz;
@@ -189,10 +190,10 @@ __device__ __forceinline__ void sweep27(const DevUniforms *__restrict__ U,
         if (begin >= end) continue;
         const uint32_t last = end - 1;
         uint32_t base = begin;
-        const float4 *p = pts + begin + sub;
+        const T *p = pts + begin + sub;
         for (; base + 4 * L <= end; base += 4 * L, p += 4 * L) {
           const uint32_t j0 = base + sub;
-          const float4 p0 = p[0], p1 = p[L], p2 = p[2 * L], p3 = p[3 * L];
+          const T p0 = p[0], p1 = p[L], p2 = p[2 * L], p3 = p[3 * L];
           f(j0, p0, true);
           f(j0 + L, p1, true);
           f(j0 + 2 * L, p2, true);
@@ -200,8 +201,8 @@ __device__ __forceinline__ void sweep27(const DevUniforms *__restrict__ U,
         }
         if (base < end) {
           const uint32_t j0 = base + sub, j1 = j0 + L, j2 = j1 + L, j3 = j2 + L;
-          const float4 p0 = pts[min(j0, last)], p1 = pts[min(j1, last)];
-          const float4 p2 = pts[min(j2, last)], p3 = pts[min(j3, last)];
+          const T p0 = pts[min(j0, last)], p1 = pts[min(j1, last)];
+          const T p2 = pts[min(j2, last)], p3 = pts[min(j3, last)];
           f(j0, p0, j0 < end);
           f(j1, p1, j1 < end);
           f(j2, p2, j2 < end);
@@ -209,13 +210,13 @@ __device__ __forceinline__ void sweep27(const DevUniforms *__restrict__ U,
         }
       } else {
         uint32_t j = begin + sub;
-        const float4 *p = pts + j;
-        for (; j + 3 * L < end; j += 4 * L, p += 4 * L) {
-          const float4 p0 = p[0], p1 = p[L], p2 = p[2 * L], p3 = p[3 * L];
-          f(j, p0, true);
-          f(j + L, p1, true);
-          f(j + 2 * L, p2, true);
-          f(j + 3 * L, p3, true);
+        const T *p = pts + j;
+        for (; j + (UN - 1) * L < end; j += UN * L, p += UN * L) {
+          T c[UN];
+#pragma unroll
+          for (int q = 0; q < UN; ++q) c[q] = p[q * L];
+#pragma unroll
+          for (int q = 0; q < UN; ++q) f(j + q * L, c[q], true);
         }
         for (; j < end; j += L, p += L) f(j, p[0], true);
       }
@@ -512,20 +513,43 @@ struct TileRows {
   }
 };
 
+// The neighbour record the density pass writes for slot j: srec[3 j + k]
+// is position and rho / gamma (k = 0), velocity and near density (1), the
+// scaled normal and 1 / max(near density, 1) (2).
+struct SlotRec {
+  float4 r[3];
+};
+
+// The first NR parts of a neighbour record, as the list walks hand it on.
+template <int NR>
+struct NbrRec {
+  float4 r[NR];
+};
+
 // Walk this lane's share of particle s's in-range list, or, if the list
-// overflowed, the lane's candidate subset again with the same test; f(j, p_j)
-// sees every pair once.  The list is written for LW = 8 lanes; with L = 16 or
-// 32 walkers, M = L / 8 of them share each list lane, taking interleaved
-// depths.  Full chunks of U pairs per trip, the next chunk's list entries
-// loaded a trip ahead so only the record gathers wait on memory.
-template <int L, int U, class F>
+// overflowed, the lane's candidate subset again with the same test;
+// f(j, rec) sees every pair once, with the first NR parts of j's record.
+// The list is written for LW = 8 lanes; with L = 16 or 32 walkers, M = L / 8
+// of them share each list lane, taking interleaved depths.  Full chunks of
+// U pairs per trip, the next chunk's list entries loaded a trip ahead so
+// only the record gathers wait on memory.  Every record of a trip is loaded
+// before any pair is summed: a load inside the per-pair test of the sweep
+// would wait on memory once per pair.
+template <int L, int U, int NR, class F>
 __device__ __forceinline__ void for_each_neighbour(const DevUniforms *__restrict__ Un,
                                                    const Bufs &b, uint32_t n, uint32_t s,
                                                    uint32_t sub, float4 pi, F &&f) {
   static_assert(L % kListLanes == 0, "walkers are a multiple of the list lanes");
+  static_assert(NR >= 1 && NR <= 3, "a record has three parts");
   constexpr uint32_t LW = kListLanes, M = L / LW;
   const uint32_t total = b.ncnt[s];
   const float4 *__restrict__ srec = b.srec;
+  auto load = [&](uint32_t j) {
+    NbrRec<NR> q;
+#pragma unroll
+    for (int k = 0; k < NR; ++k) q.r[k] = srec[3 * (size_t)j + k];
+    return q;
+  };
   if (total <= kListCapacity) {
     const uint32_t kl = sub % LW, d0 = sub / LW;
     const uint32_t nd = total > kl ? (total - 1 - kl) / LW + 1 : 0;  // depths of list lane kl
@@ -551,9 +575,9 @@ __device__ __forceinline__ void for_each_neighbour(const DevUniforms *__restrict
           nbr += U * kStep;
           k += U;
         }
-        float4 p[U];
+        NbrRec<NR> p[U];
 #pragma unroll
-        for (int q = 0; q < U; ++q) p[q] = srec[3 * (size_t)jc[q]];
+        for (int q = 0; q < U; ++q) p[q] = load(jc[q]);
 #pragma unroll
         for (int q = 0; q < U; ++q) f(jc[q], p[q]);
         if (!more) break;
@@ -565,24 +589,39 @@ __device__ __forceinline__ void for_each_neighbour(const DevUniforms *__restrict
       uint32_t jt[U];
 #pragma unroll
       for (int q = 0; q < U; ++q) jt[q] = k + q < cnt ? nbr[q * kStep] : s;
-      float4 p[U];
+      NbrRec<NR> p[U];
 #pragma unroll
-      for (int q = 0; q < U; ++q) p[q] = srec[3 * (size_t)jt[q]];
+      for (int q = 0; q < U; ++q) p[q] = load(jt[q]);
 #pragma unroll
       for (int q = 0; q < U; ++q)
         if (k + q < cnt) f(jt[q], p[q]);
     }
     return;
   }
+  // overflow: sweep the records themselves (part 0 is the position the
+  // density pass tested), the parts a pair needs loaded with it
   const float h2 = Un->h2;
   const int cx = cell_coord(pi.x, Un->ox, Un->inv_cell);
   const int cy = cell_coord(pi.y, Un->oy, Un->inv_cell);
   const int cz = cell_coord(pi.z, Un->oz, Un->inv_cell);
-  sweep27<L, false>(Un, b.counts, b.starts, b.spos, cx, cy, cz, sub,
-                    [&](uint32_t j, float4 pj, bool) {
+  const SlotRec *__restrict__ recs = reinterpret_cast<const SlotRec *>(srec);
+  // candidates per trip: as many parts in flight as four positions and
+  // velocities (two whole records with all three parts)
+  constexpr int UN = NR == 3 ? 2 : 4;
+  sweep27<L, false, UN>(Un, b.counts, b.starts, recs, cx, cy, cz, sub,
+                        [&](uint32_t j, const SlotRec &c, bool) {
+    NbrRec<NR> q;
+#pragma unroll
+    for (int k = 0; k < NR; ++k) {
+      // in registers before the test: the compiler would otherwise move the
+      // load of the parts only the pair uses behind it
+      q.r[k] = c.r[k];
+      asm volatile("" : "+v"(q.r[k].x), "+v"(q.r[k].y), "+v"(q.r[k].z), "+v"(q.r[k].w));
+    }
+    const float4 pj = q.r[0];
     const float rx = pi.x - pj.x, ry = pi.y - pj.y, rz = pi.z - pj.z;
     const float r2 = dot3(rx, ry, rz, rx, ry, rz);
-    if (r2 < h2 && r2 > 1e-12f) f(j, srec[3 * (size_t)j]);
+    if (r2 < h2 && r2 > 1e-12f) f(j, q);
   });
 }
 

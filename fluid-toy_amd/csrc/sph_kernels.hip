@@ -425,11 +425,10 @@ __device__ __forceinline__ float4 xsph_particle(const DevUniforms *__restrict__ 
   const float4 pi = b.spos[s];
   const float h2 = U->h2, mass = U->mass, ps = U->poly6_scale;
   float cxv = 0.f, cyv = 0.f, czv = 0.f;
-  const float4 *__restrict__ srec = b.srec;
-  for_each_neighbour<L, 4>(U, b, n, s, sub, pi, [&](uint32_t j, float4 pj) {
+  for_each_neighbour<L, 4, 2>(U, b, n, s, sub, pi, [&](uint32_t, const NbrRec<2> &q) {
+    const float4 pj = q.r[0], vj = q.r[1];
     const float rx = pi.x - pj.x, ry = pi.y - pj.y, rz = pi.z - pj.z;
     const float r2 = dot3(rx, ry, rz, rx, ry, rz);
-    const float4 vj = srec[3 * (size_t)j + 1];
     const float d = h2 - r2;
     const float w = ps * d * d * d;
     const float f = mass * __builtin_amdgcn_rcpf(max1(pj.w)) * w;
@@ -484,12 +483,10 @@ __device__ __forceinline__ void force_particle(const DevUniforms *__restrict__ U
   const float inv_md = 1.0f / md;
   // one accumulator: a = sum (sp + st mk coh) dir - st mk (n_i - n_j)
   float fx = 0.f, fy = 0.f, fz = 0.f;
-  const float4 *__restrict__ srec = b.srec;
-  for_each_neighbour<L, 2>(U, b, n, s, sub, pi, [&](uint32_t j, float4 pj) {
+  for_each_neighbour<L, 2, 3>(U, b, n, s, sub, pi, [&](uint32_t j, const NbrRec<3> &q) {
+    const float4 pj = q.r[0], vj = q.r[1], nj = q.r[2];
     const float rx = pi.x - pj.x, ry = pi.y - pj.y, rz = pi.z - pj.z;
     const float r2 = dot3(rx, ry, rz, rx, ry, rz);
-    const float4 vj = srec[3 * (size_t)j + 1];
-    const float4 nj = srec[3 * (size_t)j + 2];
     const bool close = r2 < close_r2;
     const float il = __builtin_amdgcn_rsqf(r2);  // 1 / r
     const float r = r2 * il;

@@ -215,8 +215,9 @@ __global__ __launch_bounds__(256) void k_spray_potentials(Uni un, Bufs b, SprayB
   if (active) {
     const float inv_h = 1.0f / h, mnorm = mass * norm;
     // the list holds exactly the pairs the WGSL loop keeps, 1e-12 < r^2 < h^2
-    for_each_neighbour<kEmitLanes, 2>(U, b, n, s, sub, r0, [&](uint32_t j, float4 pj) {
-      const float4 vj = b.srec[3 * (size_t)j + 1];
+    for_each_neighbour<kEmitLanes, 2, 2>(U, b, n, s, sub, r0,
+                                         [&](uint32_t j, const NbrRec<2> &q) {
+      const float4 pj = q.r[0], vj = q.r[1];
       const float rx = r0.x - pj.x, ry = r0.y - pj.y, rz = r0.z - pj.z;
       const float rsq = dot3(rx, ry, rz, rx, ry, rz);
       const float il = __builtin_amdgcn_rsqf(rsq);

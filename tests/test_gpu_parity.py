@@ -285,6 +285,34 @@ def test_pair_test_lower_bound(pkg, orc):
     sim.close()
 
 
+@pytest.mark.parametrize("wide", [0, 4, 8])
+def test_list_overflow_cluster(pkg, orc, wide):
+    """1,500 particles packed into a ball of radius 0.03 (< h / 2) inside the
+    8k block, the collapsed clusters of the 1M and 4M blocks in small: each
+    has more in-range neighbours than a list holds (1024), so XSPH and force
+    sweep its candidates again (with the 8-lane walk and with 16 or 32
+    walkers); the step must still match the oracle."""
+    scene = pkg.scenes.dam_break_8k()
+    parts = scene.particles.copy()
+    rng = np.random.default_rng(7)
+    k = 1500
+    centre = parts[:, 0:3].mean(axis=0)
+    d = rng.normal(size=(k, 3))
+    d *= (0.03 * rng.uniform(size=(k, 1)) ** (1 / 3)) / np.linalg.norm(d, axis=1, keepdims=True)
+    parts[:k, 0:3] = (centre + d).astype(np.float32)
+    scene.particles = parts
+    sim, o = make_pair(pkg, orc, scene)
+    if wide:
+        wide_walk(pkg, sim, wide)
+    print(f"case: list overflow cluster, walk flag={wide}", flush=True)
+    one_step_compare(sim, o, "cluster")
+    # the step's sorted state and grid: the cluster overflowed its lists
+    st = sim.pair_stats()
+    print(f"  max in-range {st['max_in_range']}", flush=True)
+    assert st["max_in_range"] > 1024
+    sim.close()
+
+
 @pytest.mark.parametrize("mode", [0, 1, 2, 3, 4])
 def test_mouse_force_modes(pkg, orc, mode):
     scene = pkg.scenes.dam_break_8k()
