@@ -168,7 +168,7 @@ constexpr uint32_t kScanBigGrid = 1u << 20;
 constexpr uint32_t kListCapacity = 1024;
 // Solve candidates kept per particle for the later iterations (split over
 // its lanes; a lane that overflows sweeps its cells instead).
-constexpr uint32_t kSolveListCapacity = 512;
+constexpr uint32_t kSolveListCapacity = 1024;
 // Most lanes per particle any sweep runs with (per-lane counts are sized by it).
 constexpr uint32_t kMaxLanes = 16;
 // Walker lanes the density pass writes the in-range lists for (XSPH and force
