@@ -53,6 +53,27 @@ def stage_launches(stage: str, iters: int) -> int:
     return iters if stage == "solve" else 1
 
 
+def step_kernels(pkg, sim, lanes, reps: int):
+    """The density kernel and the XSPH + force kernel(s) of the timed step,
+    each re-run `reps` times back to back on the last step's state between
+    one HIP event pair (fluid_time_range): {key: (kernel names, stages, ms
+    per launch)}."""
+    A = pkg.abi
+    fusion = sim.fusion()
+    tiled = sim.tile_config()[0]
+    dens = f"k_density_t<{lanes[0]},false>" if tiled else f"k_density<{lanes[0]},false>"
+    out = {"density": ((dens,), ("density",), sim.time_range("density", "density", reps))}
+    w = 16 if fusion & A.FLUID_WALK_16 else 32 if fusion & A.FLUID_WALK_32 else 8
+    if fusion & A.FLUID_FUSE_XSPH_FORCE:
+        out["xsph+force"] = ((f"k_xsph_force<{w},false>",), ("xsph", "force"),
+                             sim.time_range("xsph", "force", reps))
+    else:
+        out["xsph"] = ((f"k_xsph<{w},false>",), ("xsph",), sim.time_range("xsph", "xsph", reps))
+        out["force"] = ((f"k_force<{w},false>",), ("force",),
+                        sim.time_range("force", "force", reps))
+    return out
+
+
 def run_gpu(pkg, scene, warmup: int, steps: int, stage_steps: int):
     sim = pkg.make_sim(scene)
     sim.step(warmup)
@@ -60,6 +81,7 @@ def run_gpu(pkg, scene, warmup: int, steps: int, stage_steps: int):
     lanes = sim.autotune(2)  # does not advance the state
     ms = sim.time_steps(0, steps)
     stages = sim.time_stages(stage_steps)
+    run_gpu.kernels = step_kernels(pkg, sim, lanes, 200 if scene.n <= 100_000 else 10)
     cand, inr = sim.count_pairs()
     cells = sim.grid_params().total_cells
     iters = sim.pcisph_iterations
@@ -87,7 +109,21 @@ def profiled_traffic(kernel: str):
     return None
 
 
-def roofline_for(stages, n, cells, iters, cand, inr, lanes=None):
+def kernel_entry(stage_names, n, cells, cand, inr, ms):
+    """Algorithmic bytes and flops of one launch of a kernel that does
+    `stage_names` (the stage models summed), and the rates at `ms`."""
+    bytes_per_launch = sum(STAGE_BYTES[k][0] * n + STAGE_BYTES[k][1] * cells for k in stage_names)
+    flops = int(sum(FLOPS_INRANGE[k] * inr + FLOPS_CAND * cand * CAND_LAUNCHES.get(k, 0)
+                    for k in stage_names))
+    t = ms / 1e3
+    gbs = bytes_per_launch / t / 1e9 if t > 0 else 0.0
+    tf = flops / t / 1e12 if t > 0 else 0.0
+    return {"ms": round(ms, 5), "alg_bytes_per_launch": bytes_per_launch, "hbm_gbs": round(gbs, 1),
+            "hbm_frac": round(gbs / HBM_PEAK_GBS, 4), "alg_flops_per_launch": flops,
+            "tflops": round(tf, 2), "valu_frac": round(tf / FP32_PEAK_TFLOPS, 4)}
+
+
+def roofline_for(stages, n, cells, iters, cand, inr, lanes=None, kernels=None):
     per_stage = {}
     for name, ms in stages.items():
         launches = stage_launches(name, iters)
@@ -106,6 +142,23 @@ def roofline_for(stages, n, cells, iters, cand, inr, lanes=None):
             entry.update({"alg_flops_per_launch": flops, "tflops": round(tf, 2),
                           "valu_frac": round(tf / FP32_PEAK_TFLOPS, 4)})
         per_stage[name] = entry
+    if kernels:
+        # the dominant kernel of the step: the longest of density and XSPH +
+        # force, each timed over back-to-back launches
+        ks = {key: (names, kernel_entry(st, n, cells, cand, inr, ms))
+              for key, (names, st, ms) in kernels.items()}
+        dom = max(ks, key=lambda k: ks[k][1]["ms"])
+        names, d = ks[dom]
+        traffic = profiled_traffic(names[0]) if lanes is not None else None
+        roof = {"bound": "hbm", "kernel": names[0], "stages": dom,
+                "achieved": d["hbm_gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": d["hbm_frac"], "traffic": traffic, "launch_ms": d["ms"],
+                "alg_bytes_per_launch": d["alg_bytes_per_launch"],
+                "valu": {"achieved": d["tflops"], "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+                         "frac": d["valu_frac"],
+                         "alg_flops_per_launch": d["alg_flops_per_launch"]},
+                "kernels": {k: dict(v[1], kernel=v[0][0]) for k, v in ks.items()}}
+        return roof, per_stage
     dom = max(stages, key=lambda k: stages[k] / stage_launches(k, iters))
     d = per_stage[dom]
     traffic = None
@@ -499,7 +552,8 @@ def main() -> int:
                                                   args.stage_steps)
     start50 = run_gpu.start_state
     value = scene.n * args.steps / (ms / 1e3)
-    roof, per_stage = roofline_for(stages, scene.n, cells, iters, cand, inr, run_gpu.lanes)
+    roof, per_stage = roofline_for(stages, scene.n, cells, iters, cand, inr, run_gpu.lanes,
+                                   run_gpu.kernels)
     line = {
         "metric": "particle-updates/s at 50k & 1M; batched vs plain speedup; adjoint/fwd cost",
         "value": round(value, 1),
@@ -532,7 +586,7 @@ def main() -> int:
         big = make()
         k1 = max(10, min(100, args.steps // 10))
         ms1, st1, c1, i1, cells1, it1 = run_gpu(pkg, big, 5, k1, min(args.stage_steps, 10))
-        roof1, _ = roofline_for(st1, big.n, cells1, it1, c1, i1)
+        roof1, _ = roofline_for(st1, big.n, cells1, it1, c1, i1, None, run_gpu.kernels)
         line[name] = {"particles": big.n, "steps": k1, "grid_cells": cells1,
                       "grid": "the reference's dense grid (cell = h, ceil(2 extent / h) + 2 "
                               "cells per axis; its counts and starts are part of the readback "

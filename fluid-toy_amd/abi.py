@@ -258,6 +258,7 @@ PROTOTYPES = {
     "fluid_stream": (P, [P]),
     "fluid_time_steps": (C.c_int, [P, u32, u32, C.POINTER(C.c_float)]),
     "fluid_time_stages": (C.c_int, [P, u32, C.POINTER(C.c_float)]),
+    "fluid_time_range": (C.c_int, [P, u32, u32, u32, C.POINTER(C.c_float)]),
     "fluid_count_pairs": (C.c_int, [P, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
     "fluid_pair_stats": (C.c_int, [P, C.POINTER(C.c_uint64)]),
     "fluid_batch_create": (C.c_int, [C.POINTER(FluidSceneDesc), u32, u32, C.c_int, C.POINTER(P)]),

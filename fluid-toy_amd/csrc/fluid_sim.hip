@@ -110,6 +110,7 @@ struct fluid_sim {
   uint32_t num_cu = 256;
   uint32_t *d_scene_first = nullptr;  // first tile id of each scene (+ total)
   int cursor = -1;  // last completed stage of a partial step, -1 = none
+  bool sorted_ready = false;  // a grid build and reorder have been enqueued
   GpuRigidBodyAccum last_accum{};
   SweepCfg cfg{8, 8, 0};
   bool tuned = false;  // cfg chosen by fluid_set_sweep_lanes / fluid_autotune
@@ -222,6 +223,7 @@ int enqueue_range(fluid_sim *s, uint32_t first, uint32_t last) {
         break;
       case FLUID_STAGE_REORDER:
         HIP_TRY(launch_reorder(u, b, n, st));
+        s->sorted_ready = true;
         break;
       case FLUID_STAGE_DENSITY:
         HIP_TRY(launch_density(u, b, n, s->cfg, st));
@@ -1306,6 +1308,59 @@ int fluid_time_stages(fluid_sim *s, uint32_t steps, float *ms_per_stage) {
       ms_per_stage[st] += ms / (float)steps;
     }
   for (auto &e : ev) (void)hipEventDestroy(e);
+  return check_errors(s);
+}
+
+int fluid_time_range(fluid_sim *s, uint32_t first, uint32_t last, uint32_t reps,
+                     float *ms_per_rep) {
+  if (!s || !ms_per_rep) return fail(FLUID_ERR_INVALID_ARG, "null argument");
+  if (first < FLUID_STAGE_DENSITY || last > FLUID_STAGE_PREDICT || first > last || reps == 0)
+    return fail(FLUID_ERR_INVALID_ARG, "time range: stages within DENSITY..PREDICT, reps > 0");
+  if (s->cursor != -1) return fail(FLUID_ERR_STATE, "a partial step is in progress");
+  if (!s->sorted_ready) return fail(FLUID_ERR_STATE, "no step to re-run stages of");
+  HIP_TRY(hipSetDevice(s->device));
+  const size_t n = s->n;
+  *ms_per_rep = 0.f;
+  if (n == 0) return FLUID_OK;
+  // velocity, force and the first predicted buffer are rewritten by the
+  // range: kept and put back
+  float4 *keep = nullptr;
+  HIP_TRY(hipMalloc(&keep, 4 * n * sizeof(float4) + n * sizeof(float2)));
+  float2 *keep_pr = reinterpret_cast<float2 *>(keep + 4 * n);
+  const Bufs &b = s->b;
+  float4 *const live[4] = {b.vel, b.frc, b.pred[0].xq, b.pred[0].v};
+  hipError_t e = hipSuccess;
+  for (int k = 0; k < 4 && e == hipSuccess; ++k)
+    e = hipMemcpyAsync(keep + k * n, live[k], n * sizeof(float4), hipMemcpyDeviceToDevice,
+                       s->stream);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(keep_pr, b.pred[0].pr, n * sizeof(float2), hipMemcpyDeviceToDevice,
+                       s->stream);
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (e == hipSuccess) e = hipEventCreate(&e0);
+  if (e == hipSuccess) e = hipEventCreate(&e1);
+  int rc = FLUID_OK;
+  if (e == hipSuccess) e = hipEventRecord(e0, s->stream);
+  for (uint32_t r = 0; r < reps && e == hipSuccess && rc == FLUID_OK; ++r)
+    rc = enqueue_range(s, first, last);
+  if (e == hipSuccess && rc == FLUID_OK) e = hipEventRecord(e1, s->stream);
+  if (e == hipSuccess && rc == FLUID_OK) e = hipEventSynchronize(e1);
+  float ms = 0.f;
+  if (e == hipSuccess && rc == FLUID_OK) e = hipEventElapsedTime(&ms, e0, e1);
+  for (int k = 0; k < 4; ++k)
+    (void)hipMemcpyAsync(live[k], keep + k * n, n * sizeof(float4), hipMemcpyDeviceToDevice,
+                         s->stream);
+  (void)hipMemcpyAsync(b.pred[0].pr, keep_pr, n * sizeof(float2), hipMemcpyDeviceToDevice,
+                       s->stream);
+  const hipError_t es = hipStreamSynchronize(s->stream);
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  (void)hipFree(keep);
+  if (rc) return rc;
+  if (e != hipSuccess || es != hipSuccess)
+    return fail(FLUID_ERR_HIP, std::string("time range: ") +
+                                   hipGetErrorString(e != hipSuccess ? e : es));
+  *ms_per_rep = ms / (float)reps;
   return check_errors(s);
 }
 

@@ -244,6 +244,14 @@ class SphSimulation3DGrid:
         self._chk(self._lib.fluid_time_steps(self._h, warmup, steps, C.byref(ms)))
         return ms.value
 
+    def time_range(self, first: str, last: str, reps: int) -> float:
+        """Mean ms of one re-run of stages first..last (names of abi.STAGES,
+        within density..predict) of the last step (fluid_time_range)."""
+        ms = C.c_float()
+        self._chk(self._lib.fluid_time_range(self._h, abi.STAGE[first], abi.STAGE[last], reps,
+                                             C.byref(ms)))
+        return ms.value
+
     def time_stages(self, steps: int) -> dict:
         arr = (C.c_float * len(abi.STAGES))()
         self._chk(self._lib.fluid_time_stages(self._h, steps, arr))

@@ -394,6 +394,20 @@ int fluid_time_steps(fluid_sim *sim, uint32_t warmup, uint32_t steps,
  * iterations). */
 int fluid_time_stages(fluid_sim *sim, uint32_t steps, float *ms_per_stage);
 
+/* Re-run stages [first, last] (within FLUID_STAGE_DENSITY ..
+ * FLUID_STAGE_PREDICT) of the last step `reps` times back to back between
+ * one HIP event pair on the handle's stream, launched as a full step would
+ * launch them (XSPH + force as one kernel under FLUID_FUSE_XSPH_FORCE);
+ * *ms_per_rep receives the mean time of one repetition.  These stages read
+ * the step's grid and sorted state and rewrite what they compute, so every
+ * repetition does the same work; the velocity, force and predicted state
+ * they overwrite are put back afterwards.  FLUID_ERR_STATE before the
+ * first step.  No counterpart in the reference: it times a kernel's
+ * back-to-back launches (as a graph replay runs them), for the roofline of
+ * the step's dominant kernel. */
+int fluid_time_range(fluid_sim *sim, uint32_t first, uint32_t last, uint32_t reps,
+                     float *ms_per_rep);
+
 /* Launch configuration of the neighbour sweeps, cfg[3]:
  *   cfg[0] lanes cooperating on one particle in the density sweep: 1, 2,
  *          4, 8 or 16 (it writes the in-range list for 8 walker lanes
