@@ -90,15 +90,13 @@ def run_gpu(pkg, scene, warmup: int, steps: int, stage_steps: int):
     return ms, stages, cand, inr, cells, iters
 
 
-TRAFFIC_PROFILE = ROOT / "profiles" / "pmc_traffic_dam_break_50k.md"
-
-
-def profiled_traffic(kernel: str):
+def profiled_traffic(kernel: str, workload: str = "dam_break_50k"):
     """HBM bytes per dispatch of `kernel` (e.g. "k_force<8,false>") from the
-    committed FETCH_SIZE / WRITE_SIZE summary (copy-calibrated), or None."""
+    committed FETCH_SIZE / WRITE_SIZE summary of `workload`
+    (profiles/pmc_traffic_<workload>.md, copy-calibrated), or None."""
     try:
         col = None
-        for line in TRAFFIC_PROFILE.read_text().splitlines():
+        for line in (ROOT / "profiles" / f"pmc_traffic_{workload}.md").read_text().splitlines():
             cols = [c.strip() for c in line.strip().strip("|").split("|")]
             if cols and cols[0] == "kernel" and "HBM bytes" in cols:
                 col = cols.index("HBM bytes")
@@ -123,7 +121,8 @@ def kernel_entry(stage_names, n, cells, cand, inr, ms):
             "tflops": round(tf, 2), "valu_frac": round(tf / FP32_PEAK_TFLOPS, 4)}
 
 
-def roofline_for(stages, n, cells, iters, cand, inr, lanes=None, kernels=None):
+def roofline_for(stages, n, cells, iters, cand, inr, lanes=None, kernels=None,
+                 workload="dam_break_50k"):
     per_stage = {}
     for name, ms in stages.items():
         launches = stage_launches(name, iters)
@@ -149,7 +148,7 @@ def roofline_for(stages, n, cells, iters, cand, inr, lanes=None, kernels=None):
               for key, (names, st, ms) in kernels.items()}
         dom = max(ks, key=lambda k: ks[k][1]["ms"])
         names, d = ks[dom]
-        traffic = profiled_traffic(names[0]) if lanes is not None else None
+        traffic = profiled_traffic(names[0], workload) if lanes is not None else None
         roof = {"bound": "hbm", "kernel": names[0], "stages": dom,
                 "achieved": d["hbm_gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": d["hbm_frac"], "traffic": traffic, "launch_ms": d["ms"],
@@ -586,7 +585,8 @@ def main() -> int:
         big = make()
         k1 = max(10, min(100, args.steps // 10))
         ms1, st1, c1, i1, cells1, it1 = run_gpu(pkg, big, 5, k1, min(args.stage_steps, 10))
-        roof1, _ = roofline_for(st1, big.n, cells1, it1, c1, i1, None, run_gpu.kernels)
+        roof1, _ = roofline_for(st1, big.n, cells1, it1, c1, i1, run_gpu.lanes, run_gpu.kernels,
+                                name)
         line[name] = {"particles": big.n, "steps": k1, "grid_cells": cells1,
                       "grid": "the reference's dense grid (cell = h, ceil(2 extent / h) + 2 "
                               "cells per axis; its counts and starts are part of the readback "
