@@ -143,7 +143,8 @@ struct Bufs {
   uint32_t *slist, *scnt;  // per-lane candidate lists of the later PCISPH iterations
   uint32_t *sid;
   uint32_t *gmap;  // new slot -> previous slot of the last reorder, or null
-  uint32_t *counts, *starts;
+  uint32_t *counts, *starts;  // the step's cell counts (as the scan publishes them), inclusive starts
+  uint32_t *cwork;            // the build's per-cell counters, zeroed again by the scan
   uint32_t *cell_of, *slot_of;
   uint2 *tmp;
   PredBuf pred[2];
@@ -236,7 +237,7 @@ hipError_t launch_finalize(const Uni &u, const Bufs &b, uint32_t n,
                            const PredBuf &fin, hipStream_t st);
 // fin_v non-null: apply pcisph_finalize from that predicted velocity first.
 hipError_t launch_integrate(const Uni &u, const Bufs &b, uint32_t n,
-                            const float4 *fin_v, hipStream_t st);
+                            const float4 *fin_v, uint32_t ntiles, hipStream_t st);
 hipError_t launch_count_pairs(const Uni &u, const Bufs &b, uint32_t n,
                               unsigned long long *out4, hipStream_t st);
 

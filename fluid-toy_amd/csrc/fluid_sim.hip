@@ -111,6 +111,12 @@ struct fluid_sim {
   uint32_t *d_scene_first = nullptr;  // first tile id of each scene (+ total)
   int cursor = -1;  // last completed stage of a partial step, -1 = none
   bool sorted_ready = false;  // a grid build and reorder have been enqueued
+  // What the grid clear would clear, as far as enqueued work goes: the
+  // build's counters (zeroed by every scan) and the scan's tile states
+  // (reset by every integrate pass).  A full-step range launches no clear
+  // when both are clear; the captured step records whether it has one.
+  bool cwork_zero = true, scan_reset = true;
+  bool graph_has_clear = false;
   GpuRigidBodyAccum last_accum{};
   SweepCfg cfg{8, 8, 0};
   bool tuned = false;  // cfg chosen by fluid_set_sweep_lanes / fluid_autotune
@@ -213,13 +219,21 @@ int enqueue_range(fluid_sim *s, uint32_t first, uint32_t last) {
   for (uint32_t stage = first; stage <= last; ++stage) {
     switch (stage) {
       case FLUID_STAGE_CLEAR:
-        HIP_TRY(launch_clear(b, s->total_cells, s->ntiles, st));
+        // skipped only where the same range rebuilds and republishes the
+        // counts; on its own the stage clears them, as the reference does
+        if (!(last >= FLUID_STAGE_SCAN && s->cwork_zero && s->scan_reset)) {
+          HIP_TRY(launch_clear(b, s->total_cells, s->ntiles, st));
+          s->cwork_zero = s->scan_reset = true;
+        }
         break;
       case FLUID_STAGE_BUILD:
         HIP_TRY(launch_build(u, b, n, st));
+        s->cwork_zero = false;
         break;
       case FLUID_STAGE_SCAN:
         HIP_TRY(launch_scan(u, b, s->total_cells, s->cfg, st));
+        s->cwork_zero = true;
+        s->scan_reset = false;
         break;
       case FLUID_STAGE_REORDER:
         HIP_TRY(launch_reorder(u, b, n, st));
@@ -249,7 +263,8 @@ int enqueue_range(fluid_sim *s, uint32_t first, uint32_t last) {
         if (!fuse_fi) HIP_TRY(launch_finalize(u, b, n, b.pred[fin_buf], st));
         break;
       case FLUID_STAGE_INTEGRATE:
-        HIP_TRY(launch_integrate(u, b, n, fuse_fi ? b.pred[fin_buf].v : nullptr, st));
+        HIP_TRY(launch_integrate(u, b, n, fuse_fi ? b.pred[fin_buf].v : nullptr, s->ntiles, st));
+        s->scan_reset = true;
         break;
       default:
         return fail(FLUID_ERR_INVALID_ARG, "unknown stage");
@@ -278,9 +293,14 @@ int ensure_graph(fluid_sim *s) {
     return FLUID_OK;
   drop_graph(s);
   hipGraph_t g = nullptr;
+  // capturing runs nothing: the grid's state stays what it was
+  const bool cz = s->cwork_zero, sr = s->scan_reset;
+  s->graph_has_clear = !(cz && sr);
   HIP_TRY(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
   const int rc = enqueue_step(s);
   const hipError_t e = hipStreamEndCapture(s->stream, &g);
+  s->cwork_zero = cz;
+  s->scan_reset = sr;
   if (rc) {
     if (g) (void)hipGraphDestroy(g);
     return rc;
@@ -305,6 +325,17 @@ int ensure_graph(fluid_sim *s) {
   s->graph_wide = s->cfg.wide_walk;
   s->graph_solve_walk = s->cfg.solve_walk;
   s->graph_solve_first = s->cfg.solve_first;
+  return FLUID_OK;
+}
+
+// One graph-launched step.  A step captured without the grid clear (the
+// grid was clear then) gets one first if other work has since left the
+// build's counters or the scan's tile states set.
+int replay(fluid_sim *s) {
+  if (!s->graph_has_clear && !(s->cwork_zero && s->scan_reset))
+    HIP_TRY(launch_clear(s->b, s->total_cells, s->ntiles, s->stream));
+  HIP_TRY(hipGraphLaunch(s->graph, s->stream));
+  s->cwork_zero = s->scan_reset = true;
   return FLUID_OK;
 }
 
@@ -570,7 +601,7 @@ int enqueue_tape_step(fluid_sim *s, const TapeSlot &T, float4 *scratch_v) {
     const PredBuf out{T.xq[k + 1], T.pr[k + 1], k + 1 == iters ? T.vK : scratch_v};
     HIP_TRY(launch_solve(u, mb, n, in, out, k >= 1 ? T.ap[k] : nullptr, c, k == 0, st));
   }
-  HIP_TRY(launch_integrate(u, rb, n, T.vK, st));
+  HIP_TRY(launch_integrate(u, rb, n, T.vK, s->ntiles, st));
   return FLUID_OK;
 }
 
@@ -738,6 +769,7 @@ static int create_handle(int device, std::vector<SceneState> scenes,
   ALLOC(scnt, uint32_t *, (size_t)kMaxLanes * N);  // one count per solve lane
   ALLOC(sid, uint32_t *, N);
   ALLOC(counts, uint32_t *, C);
+  ALLOC(cwork, uint32_t *, C);
   ALLOC(starts, uint32_t *, C);
   ALLOC(cell_of, uint32_t *, N);
   ALLOC(slot_of, uint32_t *, N);
@@ -926,7 +958,10 @@ int fluid_step_n(fluid_sim *s, uint32_t steps) {
   int rc = ensure_graph(s);
   if (rc) return rc;
   s->roll.valid = false;
-  for (uint32_t k = 0; k < steps; ++k) HIP_TRY(hipGraphLaunch(s->graph, s->stream));
+  for (uint32_t k = 0; k < steps; ++k) {
+    rc = replay(s);
+    if (rc) return rc;
+  }
   return FLUID_OK;
 }
 
@@ -1188,7 +1223,8 @@ static int read_u32(fluid_sim *s, const uint32_t *src, size_t count, uint32_t *o
 
 int fluid_read_cell_counts(fluid_sim *s, uint32_t *out, uint32_t cap) {
   if (!s) return fail(FLUID_ERR_INVALID_ARG, "null handle");
-  return read_u32(s, s->b.counts, s->total_cells, out, cap);
+  // between a build and its scan the counts are still the build's counters
+  return read_u32(s, s->cwork_zero ? s->b.counts : s->b.cwork, s->total_cells, out, cap);
 }
 
 int fluid_read_cell_starts(fluid_sim *s, uint32_t *out, uint32_t cap) {
@@ -1272,8 +1308,15 @@ int fluid_time_steps(fluid_sim *s, uint32_t warmup, uint32_t steps, float *ms_to
   HIP_TRY(hipEventCreate(&e0));
   HIP_TRY(hipEventCreate(&e1));
   HIP_TRY(hipStreamSynchronize(s->stream));
+  if (!s->graph_has_clear && !(s->cwork_zero && s->scan_reset)) {
+    HIP_TRY(launch_clear(s->b, s->total_cells, s->ntiles, s->stream));
+    s->cwork_zero = s->scan_reset = true;
+  }
   HIP_TRY(hipEventRecord(e0, s->stream));
-  for (uint32_t k = 0; k < steps; ++k) HIP_TRY(hipGraphLaunch(s->graph, s->stream));
+  for (uint32_t k = 0; k < steps; ++k) {
+    rc = replay(s);
+    if (rc) return rc;
+  }
   HIP_TRY(hipEventRecord(e1, s->stream));
   HIP_TRY(hipEventSynchronize(e1));
   HIP_TRY(hipEventElapsedTime(ms_total, e0, e1));
@@ -1678,7 +1721,8 @@ int fluid_forward_rollout(fluid_sim *s, uint32_t steps, uint32_t checkpoint_ever
   const CkptView live = live_state(s);
   for (uint32_t t = 0; t < steps; ++t) {
     if (t % every == 0) HIP_TRY(copy_state(s, ckpt_view(s, t / every), live, s->n));
-    HIP_TRY(hipGraphLaunch(s->graph, s->stream));
+    rc = replay(s);
+    if (rc) return rc;
   }
   HIP_TRY(copy_state(s, fin_view(s), live, s->n));
   rc = check_errors(s);

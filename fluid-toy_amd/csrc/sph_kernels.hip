@@ -28,7 +28,12 @@ constexpr uint32_t kSpinCap = 1u << 24;
 // Stage 0 — grid_clear.wgsl:23-29 and clear_buffer(cell_offsets)
 // (sph_3d_grid.rs:1281).  The offsets buffer does not exist here (build
 // hands out the in-cell slots); the scan's tile states are cleared instead.
+// The build counts into `work`, which the scan zeroes again as it publishes
+// the counts, and the integrate pass resets the scan's tile states: a full
+// step that follows a full step finds all of it clear and launches no
+// clear (the step's range decides, fluid_sim.hip).
 __global__ __launch_bounds__(256) void k_clear(uint32_t *__restrict__ counts,
+                                               uint32_t *__restrict__ work,
                                                uint32_t cells,
                                                unsigned long long *status,
                                                uint32_t ntiles,
@@ -36,8 +41,14 @@ __global__ __launch_bounds__(256) void k_clear(uint32_t *__restrict__ counts,
                                                uint32_t *cell_tile_count) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   const uint32_t q = cells / 4;
-  if (i < q) reinterpret_cast<uint4 *>(counts)[i] = make_uint4(0, 0, 0, 0);
-  if (i < cells - 4 * q) counts[4 * q + i] = 0u;
+  if (i < q) {
+    reinterpret_cast<uint4 *>(counts)[i] = make_uint4(0, 0, 0, 0);
+    reinterpret_cast<uint4 *>(work)[i] = make_uint4(0, 0, 0, 0);
+  }
+  if (i < cells - 4 * q) {
+    counts[4 * q + i] = 0u;
+    work[4 * q + i] = 0u;
+  }
   if (i < ntiles) status[i] = 0ull;
   if (i == 0) {
     *tile_counter = 0u;
@@ -74,7 +85,7 @@ __global__ __launch_bounds__(256) void k_build(Uni un, Bufs b, uint32_t n) {
   if (head) {
     const unsigned long long above = ends & ~((2ull << lane) - 1ull);
     const int next = above ? (__ffsll((long long)above) - 1) : 64;
-    base = atomicAdd(&b.counts[ci], (uint32_t)(next - lane));
+    base = atomicAdd(&b.cwork[ci], (uint32_t)(next - lane));
   }
   const unsigned long long below = heads & ((2ull << lane) - 1ull);
   const int hl = below ? 63 - __clzll((long long)below) : lane;
@@ -99,16 +110,19 @@ __global__ __launch_bounds__(256) void k_scan(Bufs b, uint32_t cells) {
   __syncthreads();
   const uint32_t tile = s_tile;
   const uint32_t base = tile * TILE + threadIdx.x * ITEMS;
+  // counts from b.cwork; with b.counts set they are published there and
+  // b.cwork is zeroed for the next build (after the look-back, with the
+  // starts: stores before it would delay the tile's aggregate)
   uint32_t v[ITEMS];
   if (base + ITEMS <= cells) {
 #pragma unroll
     for (uint32_t q = 0; q < ITEMS / 4; ++q) {
-      const uint4 x = *reinterpret_cast<const uint4 *>(b.counts + base + 4 * q);
+      const uint4 x = *reinterpret_cast<const uint4 *>(b.cwork + base + 4 * q);
       v[4 * q] = x.x; v[4 * q + 1] = x.y; v[4 * q + 2] = x.z; v[4 * q + 3] = x.w;
     }
   } else {
 #pragma unroll
-    for (uint32_t q = 0; q < ITEMS; ++q) v[q] = base + q < cells ? b.counts[base + q] : 0u;
+    for (uint32_t q = 0; q < ITEMS; ++q) v[q] = base + q < cells ? b.cwork[base + q] : 0u;
   }
 #pragma unroll
   for (uint32_t q = 1; q < ITEMS; ++q) v[q] += v[q - 1];
@@ -168,10 +182,26 @@ __global__ __launch_bounds__(256) void k_scan(Bufs b, uint32_t cells) {
     for (uint32_t q = 0; q < ITEMS / 4; ++q)
       *reinterpret_cast<uint4 *>(b.starts + base + 4 * q) =
           make_uint4(off + v[4 * q], off + v[4 * q + 1], off + v[4 * q + 2], off + v[4 * q + 3]);
+    if (b.counts) {
+#pragma unroll
+      for (uint32_t q = 0; q < ITEMS / 4; ++q) {
+        const uint32_t p = q ? v[4 * q - 1] : 0u;  // the counts from the running sums
+        *reinterpret_cast<uint4 *>(b.counts + base + 4 * q) =
+            make_uint4(v[4 * q] - p, v[4 * q + 1] - v[4 * q], v[4 * q + 2] - v[4 * q + 1],
+                       v[4 * q + 3] - v[4 * q + 2]);
+        *reinterpret_cast<uint4 *>(b.cwork + base + 4 * q) = make_uint4(0, 0, 0, 0);
+      }
+    }
   } else {
 #pragma unroll
     for (uint32_t q = 0; q < ITEMS; ++q)
-      if (base + q < cells) b.starts[base + q] = off + v[q];
+      if (base + q < cells) {
+        b.starts[base + q] = off + v[q];
+        if (b.counts) {
+          b.counts[base + q] = v[q] - (q ? v[q - 1] : 0u);
+          b.cwork[base + q] = 0u;
+        }
+      }
   }
 }
 
@@ -899,8 +929,16 @@ __device__ float sdf_sample(const float *sdf, uint32_t res, float ux, float uy,
 
 template <bool B>
 __global__ __launch_bounds__(256) void k_integrate(Uni un, Bufs b, uint32_t n,
-                                                   const float4 *__restrict__ fin_v) {
+                                                   const float4 *__restrict__ fin_v,
+                                                   uint32_t ntiles) {
   const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+  // the scan's tile states and the tile-item count, cleared for the next
+  // step's scan (every reader of them ran before this pass)
+  if (s < ntiles) b.tile_status[s] = 0ull;
+  if (s == ntiles) {
+    *b.tile_counter = 0u;
+    if (b.tiles.count) *b.tiles.count = 0u;
+  }
   if (s >= n) return;
   const DevUniforms *__restrict__ U = uni_at<B>(un, s);
   const float4 p4 = b.spos[s];
@@ -1129,7 +1167,7 @@ inline uint32_t blocks_for(uint64_t threads, uint32_t per) {
 
 hipError_t launch_clear(const Bufs &b, uint32_t cells, uint32_t ntiles, hipStream_t st) {
   const uint32_t work = std::max(cells / 4 + 4, ntiles + 1);
-  hipLaunchKernelGGL(k_clear, dim3(blocks_for(work, 256)), dim3(256), 0, st, b.counts,
+  hipLaunchKernelGGL(k_clear, dim3(blocks_for(work, 256)), dim3(256), 0, st, b.counts, b.cwork,
                      cells, b.tile_status, ntiles, b.tile_counter, b.tiles.count);
   return hipGetLastError();
 }
@@ -1340,9 +1378,8 @@ hipError_t launch_finalize(const Uni &, const Bufs &b, uint32_t n, const PredBuf
 }
 
 hipError_t launch_integrate(const Uni &u, const Bufs &b, uint32_t n, const float4 *fin_v,
-                            hipStream_t st) {
-  if (n == 0) return hipSuccess;
-  FLUID_LAUNCH_B(k_integrate, blocks_for(n, 256), u, b, n, fin_v);
+                            uint32_t ntiles, hipStream_t st) {
+  FLUID_LAUNCH_B(k_integrate, blocks_for(std::max(n, ntiles + 1), 256), u, b, n, fin_v, ntiles);
   return hipGetLastError();
 }
 

@@ -527,7 +527,8 @@ hipError_t launch_spray_emit(const Uni &u, const Bufs &b, const SprayBufs &sb, u
   hipLaunchKernelGGL(k_spray_potentials, dim3(grid), dim3(256), 0, st, u, b, sb, n, p, nclear);
   if (n == 0) return hipGetLastError();
   Bufs sc = b;
-  sc.counts = sb.emit_n;
+  sc.cwork = sb.emit_n;  // scanned in place: no published copy, not zeroed
+  sc.counts = nullptr;
   sc.starts = sb.emit_incl;
   sc.tile_status = sb.tile_status;
   sc.tile_counter = sb.tile_counter;
