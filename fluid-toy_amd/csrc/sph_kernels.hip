@@ -443,6 +443,15 @@ __global__ __launch_bounds__(256) void k_density_t(Uni un, Bufs b, uint32_t n) {
 }
 
 // ------------------------------------------------------------------------
+// Pairs per trip of the XSPH and force list walks (their records' loads in
+// flight together).
+#ifndef FLUID_XSPH_LOADS
+#define FLUID_XSPH_LOADS 4
+#endif
+#ifndef FLUID_FORCE_LOADS
+#define FLUID_FORCE_LOADS 2
+#endif
+
 // Stage 5 — xsph.wgsl:81-138: the particle's XSPH-corrected velocity w_i
 // (in every lane of its group).  With epsilon <= 0 the reference leaves the
 // velocity untouched, i.e. equal to the sorted copy.
@@ -455,7 +464,7 @@ __device__ __forceinline__ float4 xsph_particle(const DevUniforms *__restrict__ 
   const float4 pi = b.spos[s];
   const float h2 = U->h2, mass = U->mass, ps = U->poly6_scale;
   float cxv = 0.f, cyv = 0.f, czv = 0.f;
-  for_each_neighbour<L, 4, 2>(U, b, n, s, sub, pi, [&](uint32_t, const NbrRec<2> &q) {
+  for_each_neighbour<L, FLUID_XSPH_LOADS, 2>(U, b, n, s, sub, pi, [&](uint32_t, const NbrRec<2> &q) {
     const float4 pj = q.r[0], vj = q.r[1];
     const float rx = pi.x - pj.x, ry = pi.y - pj.y, rz = pi.z - pj.z;
     const float r2 = dot3(rx, ry, rz, rx, ry, rz);
@@ -513,7 +522,7 @@ __device__ __forceinline__ void force_particle(const DevUniforms *__restrict__ U
   const float inv_md = 1.0f / md;
   // one accumulator: a = sum (sp + st mk coh) dir - st mk (n_i - n_j)
   float fx = 0.f, fy = 0.f, fz = 0.f;
-  for_each_neighbour<L, 2, 3>(U, b, n, s, sub, pi, [&](uint32_t j, const NbrRec<3> &q) {
+  for_each_neighbour<L, FLUID_FORCE_LOADS, 3>(U, b, n, s, sub, pi, [&](uint32_t j, const NbrRec<3> &q) {
     const float4 pj = q.r[0], vj = q.r[1], nj = q.r[2];
     const float rx = pi.x - pj.x, ry = pi.y - pj.y, rz = pi.z - pj.z;
     const float r2 = dot3(rx, ry, rz, rx, ry, rz);
