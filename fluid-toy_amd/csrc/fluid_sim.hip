@@ -117,6 +117,7 @@ struct fluid_sim {
   // when both are clear; the captured step records whether it has one.
   bool cwork_zero = true, scan_reset = true;
   bool graph_has_clear = false;
+  bool fold_clear = true;  // separate work counters (grids below 2^20 cells)
   GpuRigidBodyAccum last_accum{};
   SweepCfg cfg{8, 8, 0};
   bool tuned = false;  // cfg chosen by fluid_set_sweep_lanes / fluid_autotune
@@ -232,7 +233,7 @@ int enqueue_range(fluid_sim *s, uint32_t first, uint32_t last) {
         break;
       case FLUID_STAGE_SCAN:
         HIP_TRY(launch_scan(u, b, s->total_cells, s->cfg, st));
-        s->cwork_zero = true;
+        s->cwork_zero = s->fold_clear;  // counted in place: left set
         s->scan_reset = false;
         break;
       case FLUID_STAGE_REORDER:
@@ -335,7 +336,8 @@ int replay(fluid_sim *s) {
   if (!s->graph_has_clear && !(s->cwork_zero && s->scan_reset))
     HIP_TRY(launch_clear(s->b, s->total_cells, s->ntiles, s->stream));
   HIP_TRY(hipGraphLaunch(s->graph, s->stream));
-  s->cwork_zero = s->scan_reset = true;
+  s->cwork_zero = s->fold_clear;
+  s->scan_reset = true;
   return FLUID_OK;
 }
 
@@ -580,6 +582,7 @@ int enqueue_tape_step(fluid_sim *s, const TapeSlot &T, float4 *scratch_v) {
   rb.svel = T.u;
   rb.srec = T.rec;
   rb.counts = T.counts;
+  if (!s->fold_clear) rb.cwork = T.counts;  // counted in place, as the live grid is
   rb.starts = T.starts;
   rb.gmap = T.gmap;
   HIP_TRY(launch_clear(rb, s->total_cells, s->ntiles, st));
@@ -769,7 +772,12 @@ static int create_handle(int device, std::vector<SceneState> scenes,
   ALLOC(scnt, uint32_t *, (size_t)kMaxLanes * N);  // one count per solve lane
   ALLOC(sid, uint32_t *, N);
   ALLOC(counts, uint32_t *, C);
-  ALLOC(cwork, uint32_t *, C);
+  if (C < kScanBigGrid) {
+    ALLOC(cwork, uint32_t *, C);
+  } else {
+    s->b.cwork = s->b.counts;  // counted in place, cleared every step
+    s->fold_clear = false;
+  }
   ALLOC(starts, uint32_t *, C);
   ALLOC(cell_of, uint32_t *, N);
   ALLOC(slot_of, uint32_t *, N);
@@ -1224,7 +1232,8 @@ static int read_u32(fluid_sim *s, const uint32_t *src, size_t count, uint32_t *o
 int fluid_read_cell_counts(fluid_sim *s, uint32_t *out, uint32_t cap) {
   if (!s) return fail(FLUID_ERR_INVALID_ARG, "null handle");
   // between a build and its scan the counts are still the build's counters
-  return read_u32(s, s->cwork_zero ? s->b.counts : s->b.cwork, s->total_cells, out, cap);
+  return read_u32(s, s->cwork_zero || !s->fold_clear ? s->b.counts : s->b.cwork, s->total_cells,
+                  out, cap);
 }
 
 int fluid_read_cell_starts(fluid_sim *s, uint32_t *out, uint32_t cap) {

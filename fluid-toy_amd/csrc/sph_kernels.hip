@@ -31,7 +31,9 @@ constexpr uint32_t kSpinCap = 1u << 24;
 // The build counts into `work`, which the scan zeroes again as it publishes
 // the counts, and the integrate pass resets the scan's tile states: a full
 // step that follows a full step finds all of it clear and launches no
-// clear (the step's range decides, fluid_sim.hip).
+// clear (the step's range decides, fluid_sim.hip).  Grids of 2^20 cells or
+// more (batches) count in place (work == counts) and clear every step:
+// there the clear's streaming writes cost less than the scan's publishing.
 __global__ __launch_bounds__(256) void k_clear(uint32_t *__restrict__ counts,
                                                uint32_t *__restrict__ work,
                                                uint32_t cells,
@@ -43,11 +45,11 @@ __global__ __launch_bounds__(256) void k_clear(uint32_t *__restrict__ counts,
   const uint32_t q = cells / 4;
   if (i < q) {
     reinterpret_cast<uint4 *>(counts)[i] = make_uint4(0, 0, 0, 0);
-    reinterpret_cast<uint4 *>(work)[i] = make_uint4(0, 0, 0, 0);
+    if (work != counts) reinterpret_cast<uint4 *>(work)[i] = make_uint4(0, 0, 0, 0);
   }
   if (i < cells - 4 * q) {
     counts[4 * q + i] = 0u;
-    work[4 * q + i] = 0u;
+    if (work != counts) work[4 * q + i] = 0u;
   }
   if (i < ntiles) status[i] = 0ull;
   if (i == 0) {
@@ -110,9 +112,10 @@ __global__ __launch_bounds__(256) void k_scan(Bufs b, uint32_t cells) {
   __syncthreads();
   const uint32_t tile = s_tile;
   const uint32_t base = tile * TILE + threadIdx.x * ITEMS;
-  // counts from b.cwork; with b.counts set they are published there and
-  // b.cwork is zeroed for the next build (after the look-back, with the
+  // counts from b.cwork; with a separate b.counts they are published there
+  // and b.cwork is zeroed for the next build (after the look-back, with the
   // starts: stores before it would delay the tile's aggregate)
+  const bool publish = b.counts && b.counts != b.cwork;
   uint32_t v[ITEMS];
   if (base + ITEMS <= cells) {
 #pragma unroll
@@ -182,7 +185,7 @@ __global__ __launch_bounds__(256) void k_scan(Bufs b, uint32_t cells) {
     for (uint32_t q = 0; q < ITEMS / 4; ++q)
       *reinterpret_cast<uint4 *>(b.starts + base + 4 * q) =
           make_uint4(off + v[4 * q], off + v[4 * q + 1], off + v[4 * q + 2], off + v[4 * q + 3]);
-    if (b.counts) {
+    if (publish) {
 #pragma unroll
       for (uint32_t q = 0; q < ITEMS / 4; ++q) {
         const uint32_t p = q ? v[4 * q - 1] : 0u;  // the counts from the running sums
@@ -197,7 +200,7 @@ __global__ __launch_bounds__(256) void k_scan(Bufs b, uint32_t cells) {
     for (uint32_t q = 0; q < ITEMS; ++q)
       if (base + q < cells) {
         b.starts[base + q] = off + v[q];
-        if (b.counts) {
+        if (publish) {
           b.counts[base + q] = v[q] - (q ? v[q - 1] : 0u);
           b.cwork[base + q] = 0u;
         }
