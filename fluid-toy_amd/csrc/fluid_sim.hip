@@ -1620,6 +1620,23 @@ int fluid_autotune(fluid_sim *s, uint32_t trials, uint32_t out[3]) {
     best.fuse_xf = fuse;
     best.wide_walk = wide;
   }
+  // 16 solve lanes against 8, as whole steps: the per-stage timing above
+  // often puts them within a few per cent, and only 8 lanes go on to the
+  // wider walks and first-iteration sweeps tuned below
+  if (best.solve_list && !best.tiled && best.solve_lanes == 16) {
+    float t16 = 0.f, t8 = 0.f, ms = 0.f;
+    for (int v = 0; v < 2 && e == hipSuccess && rc == FLUID_OK; ++v) {
+      s->cfg = best;
+      s->cfg.solve_lanes = v ? 8 : 16;
+      s->cfg.solve_first = v ? 1 : 0;
+      for (uint32_t t = 0; t < trials && rc == FLUID_OK; ++t) {
+        if ((e = copy(false)) != hipSuccess) break;
+        rc = fluid_time_steps(s, 0, 2, &ms);
+        (v ? t8 : t16) += ms;
+      }
+    }
+    if (t8 < t16) best.solve_lanes = 8;
+  }
   // the later PCISPH iterations' walkers per skin-list lane, as whole steps
   if (best.solve_list && !best.tiled && best.solve_lanes >= 2 && best.solve_lanes <= 8) {
     float best_f = 1e30f;
@@ -1663,6 +1680,22 @@ int fluid_autotune(fluid_sim *s, uint32_t trials, uint32_t out[3]) {
       }
     }
     best.solve_first = sf;
+  }
+  // the workgroup-to-XCD mapping last, as whole steps with everything else
+  // chosen: the per-stage sums that picked it differ by about as much as
+  // the mapping changes the step
+  if (!best.tiled && e == hipSuccess && rc == FLUID_OK) {
+    float tm[2] = {0.f, 0.f}, ms = 0.f;
+    for (int v = 0; v < 2 && e == hipSuccess && rc == FLUID_OK; ++v) {
+      s->cfg = best;
+      s->cfg.xcd_tiles = v;
+      for (uint32_t t = 0; t < trials && rc == FLUID_OK; ++t) {
+        if ((e = copy(false)) != hipSuccess) break;
+        rc = fluid_time_steps(s, 0, 2, &ms);
+        tm[v] += ms;
+      }
+    }
+    best.xcd_tiles = tm[1] < tm[0] ? 1 : 0;
   }
   if (e == hipSuccess) e = copy(false);
   if (e == hipSuccess)
