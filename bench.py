@@ -85,6 +85,7 @@ def run_gpu(pkg, scene, warmup: int, steps: int, stage_steps: int):
     cand, inr = sim.count_pairs()
     cells = sim.grid_params().total_cells
     iters = sim.pcisph_iterations
+    run_gpu.fusion = sim.fusion()
     sim.close()
     run_gpu.lanes = lanes
     return ms, stages, cand, inr, cells, iters
@@ -182,9 +183,11 @@ def run_batched(pkg, count: int, steps: int = 20):
     batch = pkg.make_batch(scenes)
     batch.step(5)
     cfg = batch.autotune(1)
+    fusion = batch.fusion()
     k = max(5, steps)
     t_batch = batch.time_steps(0, k) / k
     stages = batch.time_stages(3)
+    cand, inr = batch.count_pairs()
     batch.close()
     # the same steps (6 .. 5 + k of each scene) on the sequential side
     t_seq = 0.0
@@ -199,7 +202,9 @@ def run_batched(pkg, count: int, steps: int = 20):
             "sequential_ms_per_step": round(t_seq, 4),
             "speedup": round(t_seq / t_batch, 3),
             "batched_particle_updates_per_s": round(n / (t_batch / 1e3), 1),
-            "autotuned": list(cfg), "stage_ms": {a: round(b, 4) for a, b in stages.items()}}
+            "autotuned": list(cfg), "fusion_flags": fusion,
+            "pairs_per_particle": {"candidates": round(cand / n, 1), "in_range": round(inr / n, 1)},
+            "stage_ms": {a: round(b, 4) for a, b in stages.items()}}
 
 
 def run_spray(pkg, frames_warm: int = 60, frames: int = 20, substeps: int = 2):
@@ -569,7 +574,8 @@ def main() -> int:
         "config": {"workload": "dam_break_50k", "particles": scene.n, "h": scene.h,
                    "spacing": 0.025, "box": list(scene.box), "pcisph_iterations": iters,
                    "grid_cells": cells, "launch": "hipGraph per step",
-                   "lanes_per_particle": list(run_gpu.lanes)},
+                   "lanes_per_particle": list(run_gpu.lanes),
+                   "fusion_flags": run_gpu.fusion},
         "roofline": roof,
         "stages": per_stage,
         "pairs_per_particle": {"candidates": round(cand / scene.n, 1),
@@ -594,6 +600,7 @@ def main() -> int:
                       "value": round(big.n * k1 / (ms1 / 1e3), 1),
                       "ms_per_step": round(ms1 / k1, 4),
                       "lanes_per_particle": list(run_gpu.lanes),
+                      "fusion_flags": run_gpu.fusion,
                       "pairs_per_particle": {"candidates": round(c1 / big.n, 1),
                                              "in_range": round(i1 / big.n, 1)},
                       "stage_ms": {k: round(v, 4) for k, v in st1.items()},

@@ -315,6 +315,7 @@ FLUID_SOLVE_WALK_X2 = 16
 FLUID_SOLVE_WALK_X4 = 32
 FLUID_SOLVE_FIRST_X2 = 64
 FLUID_SOLVE_FIRST_X4 = 128
+FLUID_LIST_SKIP = 256
 
 # Gradient slots per scene written by fluid_backward_rollout.
 FLUID_GRAD_PARAMS = 5

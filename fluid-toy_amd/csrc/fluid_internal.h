@@ -167,6 +167,11 @@ constexpr uint32_t kScanBigGrid = 1u << 20;
 
 // In-range neighbour entries stored per particle (split over its lanes).
 constexpr uint32_t kListCapacity = 1024;
+// Candidate count above which a particle records no in-range list under
+// FLUID_LIST_SKIP (the density pass marks it overflowed; XSPH and force
+// sweep it), and the value that turns the skip off.
+constexpr uint32_t kListSkip = 2 * kListCapacity;
+constexpr uint32_t kListSkipOff = 0xFFFFFFFFu;
 // Solve candidates kept per particle for the later iterations (split over
 // its lanes; a lane that overflows sweeps its cells instead).
 constexpr uint32_t kSolveListCapacity = 1024;
@@ -203,6 +208,8 @@ struct SweepCfg {
   int solve_first;  // 0 / 1 / 2: the first PCISPH iteration sweeps with 1 / 2 / 4
                     // lanes per skin-list lane (skin lists on, global sweeps,
                     // solve_lanes <= 8)
+  uint32_t list_skip = kListSkipOff;  // density records no list for particles
+                                      // with more candidates than this
 };
 
 // Host-side launchers (sph_kernels.hip).  All enqueue on `st`.

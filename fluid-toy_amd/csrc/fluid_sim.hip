@@ -107,6 +107,7 @@ struct fluid_sim {
   uint32_t graph_tile_w = 0;
   int graph_fuse_xf = 0, graph_solve_list = 0, graph_wide = 0, graph_solve_walk = 0;
   int graph_solve_first = 0;
+  uint32_t graph_list_skip = kListSkipOff;
   uint32_t num_cu = 256;
   uint32_t *d_scene_first = nullptr;  // first tile id of each scene (+ total)
   int cursor = -1;  // last completed stage of a partial step, -1 = none
@@ -172,6 +173,7 @@ void update_cfg(fluid_sim *s) {
   if (s->tuned) return;
   const int wgs = s->cfg.tile_wgs, fuse = s->cfg.fuse_xf, sl = s->cfg.solve_list;
   const int wide = s->cfg.wide_walk, sw = s->cfg.solve_walk, sf = s->cfg.solve_first;
+  const uint32_t skip = s->cfg.list_skip;
   s->cfg = pick_cfg(s->n);
   s->cfg.tile_wgs = wgs;
   s->cfg.fuse_xf = fuse;
@@ -179,6 +181,7 @@ void update_cfg(fluid_sim *s) {
   s->cfg.wide_walk = wide;
   s->cfg.solve_walk = sw;
   s->cfg.solve_first = sf;
+  s->cfg.list_skip = skip;
 }
 
 int push_uniforms(fluid_sim *s) {
@@ -290,7 +293,7 @@ int ensure_graph(fluid_sim *s) {
       s->graph_wgs == s->cfg.tile_wgs && s->graph_tile_w == s->b.tiles.w &&
       s->graph_fuse_xf == s->cfg.fuse_xf && s->graph_solve_list == s->cfg.solve_list &&
       s->graph_wide == s->cfg.wide_walk && s->graph_solve_walk == s->cfg.solve_walk &&
-      s->graph_solve_first == s->cfg.solve_first)
+      s->graph_solve_first == s->cfg.solve_first && s->graph_list_skip == s->cfg.list_skip)
     return FLUID_OK;
   drop_graph(s);
   hipGraph_t g = nullptr;
@@ -326,6 +329,7 @@ int ensure_graph(fluid_sim *s) {
   s->graph_wide = s->cfg.wide_walk;
   s->graph_solve_walk = s->cfg.solve_walk;
   s->graph_solve_first = s->cfg.solve_first;
+  s->graph_list_skip = s->cfg.list_skip;
   return FLUID_OK;
 }
 
@@ -1454,7 +1458,7 @@ int fluid_set_fusion(fluid_sim *s, uint32_t flags) {
   if (!s) return fail(FLUID_ERR_INVALID_ARG, "null handle");
   if (flags & ~(FLUID_FUSE_XSPH_FORCE | FLUID_FUSE_SOLVE_LIST | FLUID_WALK_16 | FLUID_WALK_32 |
                 FLUID_SOLVE_WALK_X2 | FLUID_SOLVE_WALK_X4 | FLUID_SOLVE_FIRST_X2 |
-                FLUID_SOLVE_FIRST_X4))
+                FLUID_SOLVE_FIRST_X4 | FLUID_LIST_SKIP))
     return fail(FLUID_ERR_INVALID_ARG, "unknown fusion flag");
   if ((flags & FLUID_WALK_16) && (flags & FLUID_WALK_32))
     return fail(FLUID_ERR_INVALID_ARG, "FLUID_WALK_16 and FLUID_WALK_32 exclude each other");
@@ -1470,6 +1474,7 @@ int fluid_set_fusion(fluid_sim *s, uint32_t flags) {
   s->cfg.wide_walk = (flags & FLUID_WALK_16) ? 1 : (flags & FLUID_WALK_32) ? 2 : 0;
   s->cfg.solve_walk = (flags & FLUID_SOLVE_WALK_X2) ? 1 : (flags & FLUID_SOLVE_WALK_X4) ? 2 : 0;
   s->cfg.solve_first = (flags & FLUID_SOLVE_FIRST_X2) ? 1 : (flags & FLUID_SOLVE_FIRST_X4) ? 2 : 0;
+  s->cfg.list_skip = (flags & FLUID_LIST_SKIP) ? kListSkip : kListSkipOff;
   s->roll.valid = false;
   return FLUID_OK;
 }
@@ -1483,7 +1488,8 @@ uint32_t fluid_fusion(const fluid_sim *s) {
          (s->cfg.solve_walk == 1 ? FLUID_SOLVE_WALK_X2 : 0u) |
          (s->cfg.solve_walk == 2 ? FLUID_SOLVE_WALK_X4 : 0u) |
          (s->cfg.solve_first == 1 ? FLUID_SOLVE_FIRST_X2 : 0u) |
-         (s->cfg.solve_first == 2 ? FLUID_SOLVE_FIRST_X4 : 0u);
+         (s->cfg.solve_first == 2 ? FLUID_SOLVE_FIRST_X4 : 0u) |
+         (s->cfg.list_skip != kListSkipOff ? FLUID_LIST_SKIP : 0u);
 }
 
 int fluid_get_tile_config(const fluid_sim *s, uint32_t out[3]) {
@@ -1557,6 +1563,7 @@ int fluid_autotune(fluid_sim *s, uint32_t trials, uint32_t out[3]) {
     mc.wide_walk = 0;  // decided with the fusion below
     mc.solve_walk = 0;
     mc.solve_first = 0;
+    mc.list_skip = kListSkipOff;  // decided with the walks below
     for (int li = 0; li < 5 && e == hipSuccess && rc == FLUID_OK; ++li) {
       if (m.tiled && lanes[li] > 8) continue;  // tiled sweeps run at most 8 lanes
       s->cfg = mc;
@@ -1619,6 +1626,22 @@ int fluid_autotune(fluid_sim *s, uint32_t trials, uint32_t out[3]) {
     }
     best.fuse_xf = fuse;
     best.wide_walk = wide;
+  }
+  // lists for candidate-heavy particles or not, as whole steps: skipping
+  // them saves the density pass's writes and moves those particles' XSPH
+  // and force work to sweeps
+  {
+    float tm[2] = {0.f, 0.f}, ms = 0.f;
+    for (int v = 0; v < 2 && e == hipSuccess && rc == FLUID_OK; ++v) {
+      s->cfg = best;
+      s->cfg.list_skip = v ? kListSkip : kListSkipOff;
+      for (uint32_t t = 0; t < trials && rc == FLUID_OK; ++t) {
+        if ((e = copy(false)) != hipSuccess) break;
+        rc = fluid_time_steps(s, 0, 2, &ms);
+        tm[v] += ms;
+      }
+    }
+    best.list_skip = tm[1] < tm[0] ? kListSkip : kListSkipOff;
   }
   // 16 solve lanes against 8, as whole steps: the per-stage timing above
   // often puts them within a few per cent, and only 8 lanes go on to the

@@ -318,18 +318,21 @@ __device__ __forceinline__ void walk_rows(Row &&row, uint32_t total, const T *__
     }
   };
   settle();
-  auto trip = [&](uint32_t (&j)[U], uint32_t (&li)[U], bool (&v)[U]) {
+  auto trip = [&](uint32_t (&j)[U], T (&p)[U], bool (&v)[U]) {
     if (o + (uint32_t)(U - 1) * L < e.len) {
-      // the whole trip lies in the current row
+      // the whole trip lies in the current row: one address, the U loads
+      // at fixed offsets from it
+      const T *p0 = pts + (e.lb + o);
 #pragma unroll
       for (int q = 0; q < U; ++q) {
         v[q] = true;
         j[q] = e.gb + o + (uint32_t)q * L;
-        li[q] = e.lb + o + (uint32_t)q * L;
+        p[q] = p0[q * L];
       }
       o += (uint32_t)U * L;
       settle();
     } else {
+      uint32_t li[U];
 #pragma unroll
       for (int q = 0; q < U; ++q) {
         v[q] = r < 9u;
@@ -338,27 +341,25 @@ __device__ __forceinline__ void walk_rows(Row &&row, uint32_t total, const T *__
         o += L;
         settle();
       }
+#pragma unroll
+      for (int q = 0; q < U; ++q) p[q] = pts[li[q]];
     }
   };
   if (SYNC) {
     for (uint32_t base = 0; base < total; base += (uint32_t)U * L) {
-      uint32_t j[U], li[U];
+      uint32_t j[U];
       bool v[U];
-      trip(j, li, v);
       T p[U];
-#pragma unroll
-      for (int q = 0; q < U; ++q) p[q] = pts[li[q]];
+      trip(j, p, v);
 #pragma unroll
       for (int q = 0; q < U; ++q) f(j[q], p[q], v[q]);
     }
   } else {
     while (r < 9u) {
-      uint32_t j[U], li[U];
+      uint32_t j[U];
       bool v[U];
-      trip(j, li, v);
       T p[U];
-#pragma unroll
-      for (int q = 0; q < U; ++q) p[q] = pts[li[q]];
+      trip(j, p, v);
 #pragma unroll
       for (int q = 0; q < U; ++q)
         if (v[q]) f(j[q], p[q], true);

@@ -60,15 +60,30 @@ __global__ __launch_bounds__(256) void k_clear(uint32_t *__restrict__ counts,
 
 // ------------------------------------------------------------------------
 // Stage 1 — grid_build.wgsl:46-62.  cell = clamp(floor((p-o)/h)) linearised
-// x + y G + z G^2; the per-cell count is an atomic add.  Lanes holding the
-// same cell in a run (slots are spatially sorted from the previous step)
-// merge into one atomic; the returned base + rank in the run is the
-// particle's provisional in-cell slot, used only to scatter before ranking.
+// x + y G + z G^2; the per-cell count is an atomic add.  The particles of a
+// workgroup are first grouped by cell in an LDS hash table (open addressing,
+// twice as many slots as threads, so every probe sequence ends), which
+// gives each particle its rank within its cell's group; then one global
+// atomic per distinct cell of the workgroup adds the group's size, and the
+// returned base + rank is the particle's provisional in-cell slot, used
+// only to scatter before ranking.  Where the flow piles hundreds of
+// particles into one cell, the atomics on that cell's counter serialise in
+// the L2; grouping cuts them to one per workgroup that holds the cell.
+constexpr uint32_t kBuildHash = 512;
+constexpr uint32_t kBuildEmpty = 0xFFFFFFFFu;
+
 template <bool B>
 __global__ __launch_bounds__(256) void k_build(Uni un, Bufs b, uint32_t n) {
+  __shared__ uint32_t s_key[kBuildHash], s_cnt[kBuildHash], s_base[kBuildHash];
+  static_assert(kBuildHash >= 2 * 256, "the table must hold every thread's key");
+  for (uint32_t i = threadIdx.x; i < kBuildHash; i += blockDim.x) {
+    s_key[i] = kBuildEmpty;
+    s_cnt[i] = 0u;
+  }
+  __syncthreads();
   const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
   const bool valid = t < n;
-  uint32_t ci = 0xffffffffu;
+  uint32_t ci = 0, h = 0, rank = 0;
   if (valid) {
     const DevUniforms *__restrict__ U = uni_at<B>(un, t);
     const float4 p = b.pos[t];
@@ -77,22 +92,20 @@ __global__ __launch_bounds__(256) void k_build(Uni un, Bufs b, uint32_t n) {
     const int cz = min(max(cell_coord(p.z, U->oz, U->inv_cell), 0), (int)U->gz - 1);
     ci = U->cell_offset + (uint32_t)cx + (uint32_t)cy * U->gx + (uint32_t)cz * U->gxy;
     b.cell_of[t] = ci;
+    // at most 256 distinct keys in 512 slots: a free or matching slot is
+    // always found within the table's length
+    h = (ci * 2654435761u) >> 23;
+    for (uint32_t probe = 0; probe < kBuildHash; ++probe) {
+      const uint32_t k = atomicCAS(&s_key[h], kBuildEmpty, ci);
+      if (k == kBuildEmpty || k == ci) break;
+      h = (h + 1u) & (kBuildHash - 1u);
+    }
+    rank = atomicAdd(&s_cnt[h], 1u);
   }
-  const int lane = threadIdx.x & 63;
-  const uint32_t prev = __shfl_up(ci, 1);
-  const bool head = valid && (lane == 0 || prev != ci);
-  const unsigned long long heads = __ballot(head);
-  const unsigned long long ends = heads | __ballot(!valid);
-  uint32_t base = 0;
-  if (head) {
-    const unsigned long long above = ends & ~((2ull << lane) - 1ull);
-    const int next = above ? (__ffsll((long long)above) - 1) : 64;
-    base = atomicAdd(&b.cwork[ci], (uint32_t)(next - lane));
-  }
-  const unsigned long long below = heads & ((2ull << lane) - 1ull);
-  const int hl = below ? 63 - __clzll((long long)below) : lane;
-  const uint32_t hb = __shfl(base, hl);
-  if (valid) b.slot_of[t] = hb + (uint32_t)(lane - hl);
+  __syncthreads();
+  if (valid && rank == 0) s_base[h] = atomicAdd(&b.cwork[ci], s_cnt[h]);
+  __syncthreads();
+  if (valid) b.slot_of[t] = s_base[h] + rank;
 }
 
 // ------------------------------------------------------------------------
@@ -328,13 +341,22 @@ __global__ __launch_bounds__(256) void k_rank_gather(Bufs b, uint32_t n) {
 template <int L, class Sweep>
 __device__ __forceinline__ void density_particle(const DevUniforms *__restrict__ U,
                                                  const Bufs &b, uint32_t n, uint32_t s,
-                                                 uint32_t sub, float4 pi, Sweep &&sweep) {
+                                                 uint32_t sub, float4 pi, uint32_t total,
+                                                 uint32_t list_skip, Sweep &&sweep) {
   const float h = U->h, h2 = U->h2;
   const float mps = U->mass * U->poly6_scale, mns = U->mass * U->near_scale;
   const float mr = U->m_over_rho0;
   // the list is written for 8 walker lanes, whatever L
   uint32_t *__restrict__ nbr = b.nbr + lane_block<kListLanes>(s, kListCapacity);
   uint32_t run = 0;  // pairs recorded so far by the whole group
+  // A particle with more than list_skip candidates (the group's stencil
+  // total) records no list: it is marked overflowed, and XSPH and force
+  // sweep its candidates, as they would for a list that overflowed.  Such
+  // particles sit in dense clusters, where most candidates are pairs:
+  // their lists would mostly overflow, and writing the first kListCapacity
+  // entries of a list that is never read can cost more than the sweep
+  // saves for the few that would have fit.
+  const bool record = total <= list_skip;
   float dens = 0.f, near = 0.f, nx = 0.f, ny = 0.f, nz = 0.f;
   const PairTest in_pair(h2);
   // Branch-free body: a pair outside the support contributes exact zeros
@@ -353,9 +375,9 @@ __device__ __forceinline__ void density_particle(const DevUniforms *__restrict__
     nx += fn * rx;
     ny += fn * ry;
     nz += fn * rz;
-    group_list_append<L, kListLanes>(nbr, sub, run, pair, j);
+    if (record) group_list_append<L, kListLanes>(nbr, sub, run, pair, j);
   });
-  if (sub == 0) b.ncnt[s] = run;
+  if (sub == 0) b.ncnt[s] = record ? run : kListOverflow;
   dens = group_sum<L>(dens);
   near = group_sum<L>(near);
   nx = group_sum<L>(nx);
@@ -384,7 +406,8 @@ __device__ __forceinline__ void density_particle(const DevUniforms *__restrict__
 }
 
 template <int L, bool B>
-__global__ __launch_bounds__(256) void k_density(Uni un, Bufs b, uint32_t n, uint32_t xcd) {
+__global__ __launch_bounds__(256) void k_density(Uni un, Bufs b, uint32_t n, uint32_t xcd,
+                                                 uint32_t list_skip) {
   __shared__ RowEnt rows[256 / L][9];
   const uint32_t g = xcd_block(xcd) * blockDim.x + threadIdx.x;
   const uint32_t s = g / L, sub = g % L;
@@ -396,7 +419,7 @@ __global__ __launch_bounds__(256) void k_density(Uni un, Bufs b, uint32_t n, uin
   const int cz = cell_coord(pi.z, U->oz, U->inv_cell);
   RowEnt *tab = rows[threadIdx.x / L];
   const uint32_t total = rows27<L>(U, b.counts, b.starts, cx, cy, cz, sub, tab);
-  density_particle<L>(U, b, n, s, sub, pi, [&](auto &&f) {
+  density_particle<L>(U, b, n, s, sub, pi, total, list_skip, [&](auto &&f) {
     walk_rows<L, true>([&](uint32_t r) { return tab[r]; }, total, b.spos, sub, s, f);
   });
 }
@@ -407,7 +430,8 @@ __global__ __launch_bounds__(256) void k_density(Uni un, Bufs b, uint32_t n, uin
 // is not the tile's (outside the grid) and tiles too dense to stage sweep
 // from global memory.
 template <int L, bool B>
-__global__ __launch_bounds__(256) void k_density_t(Uni un, Bufs b, uint32_t n) {
+__global__ __launch_bounds__(256) void k_density_t(Uni un, Bufs b, uint32_t n,
+                                                   uint32_t list_skip) {
   extern __shared__ float4 stage[];
   __shared__ TileShared S;
   __shared__ RowEnt rows[256 / L][9];
@@ -430,13 +454,13 @@ __global__ __launch_bounds__(256) void k_density_t(Uni un, Bufs b, uint32_t n) {
       if (staged && tile_owns(S, U, cx, cy, cz)) {
         const TileRows tr(S, cx, true);
         const uint32_t total = tr.total();
-        density_particle<L>(U, b, n, s, sub, pi, [&](auto &&f) {
+        density_particle<L>(U, b, n, s, sub, pi, total, list_skip, [&](auto &&f) {
           walk_rows<L, true>(tr, total, stage, sub, tr.own(s), f);
         });
       } else {
         RowEnt *tab = rows[threadIdx.x / L];
         const uint32_t total = rows27<L>(U, b.counts, b.starts, cx, cy, cz, sub, tab);
-        density_particle<L>(U, b, n, s, sub, pi, [&](auto &&f) {
+        density_particle<L>(U, b, n, s, sub, pi, total, list_skip, [&](auto &&f) {
           walk_rows<L, true>([&](uint32_t r) { return tab[r]; }, total, b.spos, sub, s, f);
         });
       }
@@ -1288,11 +1312,11 @@ hipError_t launch_reorder(const Uni &, const Bufs &b, uint32_t n, hipStream_t st
 hipError_t launch_density(const Uni &u, const Bufs &b, uint32_t n, SweepCfg c, hipStream_t st) {
   if (n == 0) return hipSuccess;
   if (c.tiled)
-    FLUID_DISPATCH_TILED(k_density_t, c.lanes, u, b, n);
+    FLUID_DISPATCH_TILED(k_density_t, c.lanes, u, b, n, c.list_skip);
   else if (c.lanes == 16)
-    FLUID_WALK_CASE(k_density, 16, n, u, b, n, (uint32_t)c.xcd_tiles);
+    FLUID_WALK_CASE(k_density, 16, n, u, b, n, (uint32_t)c.xcd_tiles, c.list_skip);
   else
-    FLUID_DISPATCH_L(k_density, c.lanes, n, u, b, n, (uint32_t)c.xcd_tiles);
+    FLUID_DISPATCH_L(k_density, c.lanes, n, u, b, n, (uint32_t)c.xcd_tiles, c.list_skip);
   return hipGetLastError();
 }
 

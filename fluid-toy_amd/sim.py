@@ -277,7 +277,7 @@ class SphSimulation3DGrid:
         """Work-sharing flags (fluid_set_fusion): abi.FLUID_FUSE_XSPH_FORCE,
         abi.FLUID_FUSE_SOLVE_LIST, abi.FLUID_WALK_16 or abi.FLUID_WALK_32,
         abi.FLUID_SOLVE_WALK_X2 or abi.FLUID_SOLVE_WALK_X4,
-        abi.FLUID_SOLVE_FIRST_X2 or abi.FLUID_SOLVE_FIRST_X4."""
+        abi.FLUID_SOLVE_FIRST_X2 or abi.FLUID_SOLVE_FIRST_X4, abi.FLUID_LIST_SKIP."""
         self._chk(self._lib.fluid_set_fusion(self._h, int(flags)))
 
     def fusion(self) -> int:

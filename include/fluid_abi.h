@@ -449,7 +449,10 @@ int fluid_set_tile_config(fluid_sim *sim, const uint32_t cfg[3]);
  *   FLUID_SOLVE_FIRST_X2, FLUID_SOLVE_FIRST_X4 (at most one; with skin lists,
  *     global sweeps and at most 8 solve lanes): the first PCISPH iteration
  *     sweeps with two or four lanes per skin-list lane and ranks its entries
- *     into the lists by ballot. */
+ *     into the lists by ballot;
+ *   FLUID_LIST_SKIP: a particle with more than 2048 candidates (twice the
+ *     in-range list's capacity of 1024) records no list; XSPH and force
+ *     sweep its candidates, as for an overflowed list (dense clusters). */
 #define FLUID_FUSE_XSPH_FORCE 1u
 #define FLUID_FUSE_SOLVE_LIST 2u
 #define FLUID_WALK_16 4u
@@ -458,6 +461,7 @@ int fluid_set_tile_config(fluid_sim *sim, const uint32_t cfg[3]);
 #define FLUID_SOLVE_WALK_X4 32u
 #define FLUID_SOLVE_FIRST_X2 64u
 #define FLUID_SOLVE_FIRST_X4 128u
+#define FLUID_LIST_SKIP 256u
 int fluid_set_fusion(fluid_sim *sim, uint32_t flags);
 uint32_t fluid_fusion(const fluid_sim *sim);
 int fluid_get_tile_config(const fluid_sim *sim, uint32_t out[3]);

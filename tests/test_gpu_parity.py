@@ -92,7 +92,9 @@ def compare_grid(sim, o):
 
 
 def wide_walk(pkg, sim, flag):
-    """XSPH and force walk the 8-lane lists with 16 or 32 lanes per particle."""
+    """XSPH and force walk the 8-lane lists with 16 or 32 lanes per particle
+    (FLUID_WALK_16 = 4, FLUID_WALK_32 = 8), and/or candidate-heavy particles
+    record no list (FLUID_LIST_SKIP = 256)."""
     sim.set_sweep_config(8, 8, 0)
     sim.set_fusion(sim.fusion() | flag)
 
@@ -285,13 +287,15 @@ def test_pair_test_lower_bound(pkg, orc):
     sim.close()
 
 
-@pytest.mark.parametrize("wide", [0, 4, 8])
+@pytest.mark.parametrize("wide", [0, 4, 8, 256, 256 | 8])
 def test_list_overflow_cluster(pkg, orc, wide):
     """1,500 particles packed into a ball of radius 0.03 (< h / 2) inside the
     8k block, the collapsed clusters of the 1M and 4M blocks in small: each
     has more in-range neighbours than a list holds (1024), so XSPH and force
     sweep its candidates again (with the 8-lane walk and with 16 or 32
-    walkers); the step must still match the oracle."""
+    walkers); with FLUID_LIST_SKIP (256) the cluster's particles, each with
+    more than 2048 candidates, record no list at all.  The step must still
+    match the oracle."""
     scene = pkg.scenes.dam_break_8k()
     parts = scene.particles.copy()
     rng = np.random.default_rng(7)
