@@ -275,8 +275,10 @@ __global__ __launch_bounds__(256) void k_scatter(Bufs b, uint32_t n) {
 // (b) rank every entry by original index within its cell segment and copy
 // the particle (position, velocity, force) to its final sorted slot.  A
 // workgroup's 256 consecutive entries cover consecutive cell segments; their
-// original indices are staged in LDS when the covered range fits, so the
-// O(count) rank loop reads broadcast LDS words.
+// original indices are staged in LDS, kRankLds at a time, and the O(count)
+// rank loop reads broadcast LDS words.  A range longer than one stage (a
+// cluster cell of thousands of particles) is staged in turns, each thread
+// counting over the part of its cell in the stage.
 constexpr uint32_t kRankLds = 4096;
 
 __global__ __launch_bounds__(256) void k_rank_gather(Bufs b, uint32_t n) {
@@ -296,31 +298,29 @@ __global__ __launch_bounds__(256) void k_rank_gather(Bufs b, uint32_t n) {
     if (u == last) s_hi = end;
   }
   __syncthreads();
-  const uint32_t lo = s_lo, len = s_hi - s_lo;
-  const bool staged = len <= kRankLds;
-  if (staged)
-    for (uint32_t v = threadIdx.x; v < len; v += blockDim.x) s_ids[v] = b.tmp[lo + v].y;
-  __syncthreads();
-  if (u >= n) return;
+  const uint32_t lo = s_lo, hi = s_hi;
   // rank = entries of the cell with a smaller original index; eight
-  // independent loads per trip so the reads overlap (cells of several
-  // hundred particles form where the flow clusters)
+  // independent reads per trip so they overlap
   uint32_t r[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   const uint32_t key = e.y;
-  if (staged) {
-    uint32_t v = start - lo;
-    const uint32_t ve = end - lo;
-    for (; v + 8 <= ve; v += 8)
+  for (uint32_t c0 = lo; c0 < hi; c0 += kRankLds) {
+    const uint32_t len = min(hi - c0, kRankLds);
+    if (c0 != lo) __syncthreads();  // the previous stage has been read
+    for (uint32_t v = threadIdx.x; v < len; v += blockDim.x) s_ids[v] = b.tmp[c0 + v].y;
+    __syncthreads();
+    // this thread's cell within the stage: [vb, ve), empty when the cell
+    // lies wholly before or after it
+    const uint32_t vb = max(start, c0), vend = min(end, c0 + len);
+    if (u < n && vb < vend) {
+      uint32_t v = vb - c0;
+      const uint32_t ve = vend - c0;
+      for (; v + 8 <= ve; v += 8)
 #pragma unroll
-      for (int q = 0; q < 8; ++q) r[q] += (s_ids[v + q] < key) ? 1u : 0u;
-    for (; v < ve; ++v) r[0] += (s_ids[v] < key) ? 1u : 0u;
-  } else {
-    uint32_t v = start;
-    for (; v + 8 <= end; v += 8)
-#pragma unroll
-      for (int q = 0; q < 8; ++q) r[q] += (b.tmp[v + q].y < key) ? 1u : 0u;
-    for (; v < end; ++v) r[0] += (b.tmp[v].y < key) ? 1u : 0u;
+        for (int q = 0; q < 8; ++q) r[q] += (s_ids[v + q] < key) ? 1u : 0u;
+      for (; v < ve; ++v) r[0] += (s_ids[v] < key) ? 1u : 0u;
+    }
   }
+  if (u >= n) return;
   const uint32_t rank = ((r[0] + r[1]) + (r[2] + r[3])) + ((r[4] + r[5]) + (r[6] + r[7]));
   const uint32_t dest = start + rank;
   b.sid[dest] = e.y;

@@ -317,6 +317,39 @@ def test_list_overflow_cluster(pkg, orc, wide):
     sim.close()
 
 
+def test_reorder_big_cell(pkg, orc):
+    """6,000 of the 8k block's particles packed into one grid cell (a ball of
+    radius 0.012 about a cell centre): the cell's segment is longer than the
+    reorder's LDS stage (4,096 entries), so the rank pass stages it in turns.
+    Cell counts, starts and the permutation must be the oracle's exactly,
+    and the step must meet the bar."""
+    scene = pkg.scenes.dam_break_8k()
+    parts = scene.particles.copy()
+    rng = np.random.default_rng(11)
+    k = 6000
+    o_grid, cell = -2.08, 0.08
+    centre = parts[:, 0:3].mean(axis=0)
+    centre = o_grid + (np.floor((centre - o_grid) / cell) + 0.5) * cell
+    d = rng.normal(size=(k, 3))
+    d *= (0.012 * rng.uniform(size=(k, 1)) ** (1 / 3)) / np.linalg.norm(d, axis=1, keepdims=True)
+    parts[:k, 0:3] = (centre + d).astype(np.float32)
+    scene.particles = parts
+    sim, o = make_pair(pkg, orc, scene)
+    print("case: reorder with a 6,000-particle cell", flush=True)
+    S = pkg.abi.STAGE
+    sim.step_stages(0, S["reorder"]); o.stages(0, S["reorder"])
+    compare_grid(sim, o)
+    assert sim.read_cell_counts().max() >= 6000
+    np.testing.assert_array_equal(sim.read_sorted_to_orig(), o.sorted_to_orig())
+    gs, os_ = sim.read_sorted_particles(), o.sorted_particles()
+    np.testing.assert_array_equal(gs[:, 0:11], os_[:, 0:11])
+    sim.step_stages(S["density"], S["integrate"]); o.stages(S["density"], S["integrate"])
+    gp, op, o6 = sim.read_particles(), o.particles(), o.o64.particles()
+    check_fp("big cell position", gp[:, 0:3], op[:, 0:3], o6[:, 0:3])
+    check_fp("big cell velocity", gp[:, 4:7], op[:, 4:7], o6[:, 4:7])
+    sim.close()
+
+
 @pytest.mark.parametrize("mode", [0, 1, 2, 3, 4])
 def test_mouse_force_modes(pkg, orc, mode):
     scene = pkg.scenes.dam_break_8k()
