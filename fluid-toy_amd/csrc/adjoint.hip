@@ -260,6 +260,7 @@ __global__ __launch_bounds__(256) void k_adj_solve(Uni un, AdjBufs a, TapeStep t
                                                    TapeIter in, const float4 *__restrict__ lap_in,
                                                    float4 *__restrict__ lap_out, uint32_t k,
                                                    const float4 *__restrict__ xq0, uint32_t mode) {
+  __shared__ RowEnt rows[256 / L][9];
   const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
   const uint32_t s = g / L, sub = g % L;
   if (s >= n) return;
@@ -299,12 +300,22 @@ __global__ __launch_bounds__(256) void k_adj_solve(Uni un, AdjBufs a, TapeStep t
     lXy += cr * ry;
     lXz += cr * rz;
   };
+  // The sweeps walk the 27-cell stencil as one concatenated run of its nine
+  // x-rows; the row table is built (by the whole group) only where a lane
+  // of the group sweeps.
+  const uint32_t cntk = mode == 2 ? a.scnt[(size_t)s * L + sub] : kSolveListOverflow;
+  RowEnt *tab = rows[threadIdx.x / L];
+  const uint32_t total =
+      group_any<L>(cntk == kSolveListOverflow) ? rows27<L>(U, t.counts, t.starts, cx, cy, cz, sub, tab)
+                                               : 0u;
+  auto sweep = [&](auto &&f) {
+    walk_rows<L, false>([&](uint32_t r) { return tab[r]; }, total, in.xq, sub, s, f);
+  };
   if (mode == 1) {
     const float4 x0i = xq0[s];
     const float skin2 = U->solve_skin2;
     uint32_t cnt = 0;
-    sweep27<L, false>(U, t.counts, t.starts, in.xq, cx, cy, cz, sub,
-                      [&](uint32_t j, float4 xj, bool v) {
+    sweep([&](uint32_t j, float4 xj, bool v) {
       const float4 x0j = xq0[j];
       const float ex = x0i.x - x0j.x, ey = x0i.y - x0j.y, ez = x0i.z - x0j.z;
       if (dot3(ex, ey, ez, ex, ey, ez) < skin2) {
@@ -315,7 +326,7 @@ __global__ __launch_bounds__(256) void k_adj_solve(Uni un, AdjBufs a, TapeStep t
     });
     a.scnt[(size_t)s * L + sub] = cnt <= kCap ? cnt : kSolveListOverflow;
   } else {
-    const uint32_t cnt = mode == 2 ? a.scnt[(size_t)s * L + sub] : kSolveListOverflow;
+    const uint32_t cnt = cntk;
     if (cnt != kSolveListOverflow) {
       uint32_t e = 0;
       for (; e + 3 < cnt; e += 4) {
@@ -332,7 +343,7 @@ __global__ __launch_bounds__(256) void k_adj_solve(Uni un, AdjBufs a, TapeStep t
         pair(j, in.xq[j], true);
       }
     } else {
-      sweep27<L, false>(U, t.counts, t.starts, in.xq, cx, cy, cz, sub, pair);
+      sweep(pair);
     }
   }
   V3 lX = group_sum3<L>(v3(lXx, lXy, lXz));
@@ -514,6 +525,7 @@ __device__ __forceinline__ void for_each_pair(const DevUniforms *__restrict__ U,
 
 template <int L, bool B>
 __global__ __launch_bounds__(256) void k_adj_force(Uni un, AdjBufs a, TapeStep t, uint32_t n) {
+  __shared__ RowEnt rows[256 / L][9];
   const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
   const uint32_t s = g / L, sub = g % L;
   if (s >= n) return;
@@ -537,10 +549,14 @@ __global__ __launch_bounds__(256) void k_adj_force(Uni un, AdjBufs a, TapeStep t
   // pass 2 uses that, not pcnt (lane 0's store).  Pass 2 reads entries other
   // lanes of the wave stored in pass 1: the workgroup fence between the
   // passes waits for those stores and keeps the loads after them.
+  // The sweep walks the 27-cell stencil as one concatenated run of its nine
+  // x-rows (the forward density's walk).
   uint32_t run = 0;
   const PairTest in_pair(h2);
-  sweep27<L, true>(U, t.counts, t.starts, t.x, cx, cy, cz, sub,
-                   [&](uint32_t j, float4 xj, bool valid) {
+  RowEnt *tab = rows[threadIdx.x / L];
+  const uint32_t total = rows27<L>(U, t.counts, t.starts, cx, cy, cz, sub, tab);
+  walk_rows<L, true>([&](uint32_t r) { return tab[r]; }, total, t.x, sub, s,
+                     [&](uint32_t j, float4 xj, bool valid) {
     const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
     const float r2 = dot3(rx, ry, rz, rx, ry, rz);
     group_list_append<L, L>(a.plist + lane_block<L>(s, kListCapacity), sub, run,
