@@ -396,16 +396,45 @@ struct PairGrad {
   float visc, st;
 };
 
+// What the two terms of a pair (A's from B and B's from A) share: every
+// function of the distance alone.  r_vec flips sign between them, r^2 (a
+// sum of squares) keeps its bits.
+struct PairGeom {
+  float r2, ir, r, dr, gk, iden, dr2, r3, C, dCdr;
+  bool close, outer;
+};
+
+__device__ __forceinline__ PairGeom pair_geom(const DevUniforms *__restrict__ U, float rx,
+                                              float ry, float rz) {
+  PairGeom q;
+  q.r2 = dot3(rx, ry, rz, rx, ry, rz);
+  q.close = q.r2 < U->close_r2;
+  q.ir = __builtin_amdgcn_rsqf(q.r2);
+  q.r = q.r2 * q.ir;
+  q.dr = U->h - q.r;
+  q.gk = U->grad_scale * q.dr * q.dr;
+  q.iden = 1.0f / (q.r2 + U->visc_eps);
+  q.dr2 = q.dr * q.dr;
+  q.r3 = q.r2 * q.r;
+  q.outer = q.r >= U->half_h;
+  const float t3 = q.dr2 * q.dr * q.r3;  // 2 dr3 r3 = 2 t3 exactly
+  const float aks = U->akinci_scale;
+  q.C = aks * (q.outer ? t3 : __builtin_fmaf(2.0f, t3, -U->h6_64));
+  q.dCdr = aks * 3.0f * q.dr2 * q.r2 * (q.dr - q.r) * (q.outer ? 1.0f : 2.0f);
+  return q;
+}
+
+// The reverse of A's term from B; (rx, ry, rz) = x_A - x_B and q its
+// geometry.
 template <bool NEED_A, bool NEED_B>
 __device__ __forceinline__ void force_pair_vjp(const DevUniforms *__restrict__ U,
                                                const PairSide &A, const PairSide &Bn, V3 lam,
-                                               PairGrad &G) {
-  const float rx = A.x.x - Bn.x.x, ry = A.x.y - Bn.x.y, rz = A.x.z - Bn.x.z;
-  const float r2 = dot3(rx, ry, rz, rx, ry, rz);
+                                               PairGrad &G, const PairGeom &q, float rx, float ry,
+                                               float rz) {
   const float h = U->h, mass = U->mass, md = U->min_dist;
-  const bool close = r2 < U->close_r2;
-  const float ir = __builtin_amdgcn_rsqf(r2);
-  const float r = r2 * ir;
+  const bool close = q.close;
+  const float ir = q.ir;
+  const float r = q.r;
   V3 dir = v3(-rx * ir, -ry * ir, -rz * ir);
   if (close) {  // rare: the slot-difference direction
     const float3 c = close_dir(A.slot, Bn.slot);
@@ -414,26 +443,21 @@ __device__ __forceinline__ void force_pair_vjp(const DevUniforms *__restrict__ U
   const float da = max1(A.d), nda = max1(A.nd);
   const float db = max1(Bn.d), ndb = max1(Bn.nd);
   const float kn = U->near_k;
-  const float dr = h - r, gs = U->grad_scale;
-  const float gk = gs * dr * dr;
+  const float dr = q.dr, gs = U->grad_scale;
+  const float gk = q.gk;
   const float i2dn = 1.0f / (2.0f * da * ndb);
   const float S1 = -mass * kn * (nda + ndb) * i2dn;
   const float vx = A.v.x - Bn.v.x, vy = A.v.y - Bn.v.y, vz = A.v.z - Bn.v.z;
   const float vdx = dot3(vx, vy, vz, rx, ry, rz);
   const bool von = vdx < 0.f;
-  const float iden = 1.0f / (r2 + U->visc_eps);
+  const float iden = q.iden;
   const float mu = h * vdx * iden;
   const float idd = 1.0f / (da + db);
   const float visc = U->visc, st = U->st;
   const float cv = 2.0f * mass * idd;  // m / avg
   const float sv = von ? visc * mu * gk * cv : 0.f;
   const float K = 2.0f * U->rho0 * idd;
-  const float aks = U->akinci_scale;
-  const float dr2 = dr * dr, r3 = r2 * r;
-  const bool outer = r >= U->half_h;
-  const float dr3 = dr2 * dr;
-  const float t3 = dr3 * r3;  // 2 dr3 r3 = 2 t3 exactly
-  const float C = aks * (outer ? t3 : __builtin_fmaf(2.0f, t3, -U->h6_64));
+  const float C = q.C;
   const float smk = st * mass * K;
   const float coef = S1 * gk + (close ? -200.0f * (1.0f - r / md) : 0.f) + sv + smk * C;
   const float dnx = A.nrm.x - Bn.nrm.x, dny = A.nrm.y - Bn.nrm.y, dnz = A.nrm.z - Bn.nrm.z;
@@ -468,8 +492,7 @@ __device__ __forceinline__ void force_pair_vjp(const DevUniforms *__restrict__ U
     G.v_b = v3(-lvdx * rx, -lvdx * ry, -lvdx * rz);
   }
   // dependence on r: g, C, sep; on r_vec: dir and v.r_vec
-  const float dCdr = aks * 3.0f * dr2 * r2 * (dr - r) * (outer ? 1.0f : 2.0f);
-  lr += lA * smk * dCdr - lg * 2.0f * gs * dr;
+  lr += lA * smk * q.dCdr - lg * 2.0f * gs * dr;
   float gx = lvdx * vx, gy = lvdx * vy, gz = lvdx * vz;
   {
     // dir = -r_vec / r:  d dir / d r_vec = -(I - rh rh^T) / r; the close
@@ -568,9 +591,11 @@ __global__ __launch_bounds__(256) void k_adj_force(Uni un, AdjBufs a, TapeStep t
     const float4 *rj = t.rec + 3 * (size_t)j;
     const float4 rj0 = rj[0], rj1 = rj[1], rj2 = rj[2];
     PairGrad G;
+    const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
+    const PairGeom q = pair_geom(U, rx, ry, rz);
     // (i, j): the term of a_i
     const PairSide nb{xj, rj1, rj2, rj0.w, rj1.w, j};
-    force_pair_vjp<true, false>(U, own, nb, lai, G);
+    force_pair_vjp<true, false>(U, own, nb, lai, G, q, rx, ry, rz);
     gx = vadd(gx, G.x_a);
     gw = vadd(gw, G.v_a);
     gn = vadd(gn, G.n_a);
@@ -580,7 +605,7 @@ __global__ __launch_bounds__(256) void k_adj_force(Uni un, AdjBufs a, TapeStep t
     gst += G.st;
     // (j, i): the term of a_j that reads particle i
     const PairSide oj{xj, t.w[j], rj2, rj0.w, rj1.w, j};
-    force_pair_vjp<false, true>(U, oj, asn, v3(a.la[j]), G);
+    force_pair_vjp<false, true>(U, oj, asn, v3(a.la[j]), G, q, -rx, -ry, -rz);
     gx = vsub(gx, G.x_a);
     gu = vadd(gu, G.v_b);
     gn = vadd(gn, G.n_b);
