@@ -785,7 +785,7 @@ static int create_handle(int device, std::vector<SceneState> scenes,
   ALLOC(starts, uint32_t *, C);
   ALLOC(cell_of, uint32_t *, N);
   ALLOC(slot_of, uint32_t *, N);
-  ALLOC(tmp, uint2 *, N);
+  ALLOC(tmp, uint4 *, N);
   for (int k = 0; k < 2; ++k) {
     ALLOC(pred[k].xq, float4 *, N);
     ALLOC(pred[k].pr, float2 *, N);

@@ -103,6 +103,14 @@ struct PairTest {
   }
 };
 
+// Order key of a float: unsigned order of the keys is the float order of
+// the values (-0 before +0), a total order over every bit pattern (NaNs at
+// the two ends).
+__device__ __forceinline__ uint32_t x_key(float x) {
+  const uint32_t u = __float_as_uint(x);
+  return u ^ ((uint32_t)((int32_t)u >> 31) | 0x80000000u);
+}
+
 // fmaxf(x, 1.0f) for every non-NaN x in one integer max: non-negative floats
 // order as their bit patterns, and negative ones (sign bit set) are negative
 // as int32, below 1.0f's pattern.  fmaxf itself costs two instructions here

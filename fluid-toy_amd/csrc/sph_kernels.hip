@@ -262,32 +262,38 @@ __global__ __launch_bounds__(256) void k_tiles(Uni un, Bufs b) {
   for (uint32_t k = 0; k < items; ++k) tm.list[base + k] = make_uint4(scene, lt, k, 0u);
 }
 
-// Stage 3 — grid_reorder.wgsl:37-63 as a deterministic counting sort.
-// (a) scatter (old slot, original index) to cell start + provisional slot;
+// Stage 3 — grid_reorder.wgsl:37-63 as a deterministic counting sort.  The
+// reference's within-cell order comes from a racing atomicAdd
+// (grid_reorder.wgsl:51); here it is fixed to ascending position x (the
+// x_key of the position the build binned, ties by original index), so each
+// x-row of cells is one slot range sorted by x and the density pass can
+// narrow a row to the slots within reach of the particle.
+// (a) scatter (old slot, original index, x key) to cell start + provisional
+// slot;
 __global__ __launch_bounds__(256) void k_scatter(Bufs b, uint32_t n) {
   const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= n) return;
   const uint32_t ci = b.cell_of[t];
   const uint32_t start = b.starts[ci] - b.counts[ci];
-  b.tmp[start + b.slot_of[t]] = make_uint2(t, b.sid[t]);
+  b.tmp[start + b.slot_of[t]] = make_uint4(t, b.sid[t], x_key(b.pos[t].x), 0u);
 }
 
-// (b) rank every entry by original index within its cell segment and copy
-// the particle (position, velocity, force) to its final sorted slot.  A
-// workgroup's 256 consecutive entries cover consecutive cell segments; their
-// original indices are staged in LDS, kRankLds at a time, and the O(count)
-// rank loop reads broadcast LDS words.  A range longer than one stage (a
-// cluster cell of thousands of particles) is staged in turns, each thread
-// counting over the part of its cell in the stage.
+// (b) rank every entry by (x key, original index) within its cell segment
+// and copy the particle (position, velocity, force) to its final sorted
+// slot.  A workgroup's 256 consecutive entries cover consecutive cell
+// segments; their 64-bit order keys are staged in LDS, kRankLds at a time,
+// and the O(count) rank loop reads broadcast LDS words.  A range longer than
+// one stage (a cluster cell of thousands of particles) is staged in turns,
+// each thread counting over the part of its cell in the stage.
 constexpr uint32_t kRankLds = 4096;
 
 __global__ __launch_bounds__(256) void k_rank_gather(Bufs b, uint32_t n) {
-  __shared__ uint32_t s_ids[kRankLds];
+  __shared__ unsigned long long s_ids[kRankLds];
   __shared__ uint32_t s_lo, s_hi;
   const uint32_t u0 = blockIdx.x * blockDim.x;
   const uint32_t u = u0 + threadIdx.x;
   const uint32_t last = min(n, u0 + blockDim.x) - 1;
-  uint2 e = make_uint2(0, 0);
+  uint4 e = make_uint4(0, 0, 0, 0);
   uint32_t start = 0, end = 0;
   if (u < n) {
     e = b.tmp[u];
@@ -302,11 +308,14 @@ __global__ __launch_bounds__(256) void k_rank_gather(Bufs b, uint32_t n) {
   // rank = entries of the cell with a smaller original index; eight
   // independent reads per trip so they overlap
   uint32_t r[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-  const uint32_t key = e.y;
+  const unsigned long long key = (unsigned long long)e.z << 32 | e.y;
   for (uint32_t c0 = lo; c0 < hi; c0 += kRankLds) {
     const uint32_t len = min(hi - c0, kRankLds);
     if (c0 != lo) __syncthreads();  // the previous stage has been read
-    for (uint32_t v = threadIdx.x; v < len; v += blockDim.x) s_ids[v] = b.tmp[c0 + v].y;
+    for (uint32_t v = threadIdx.x; v < len; v += blockDim.x) {
+      const uint4 t = b.tmp[c0 + v];
+      s_ids[v] = (unsigned long long)t.z << 32 | t.y;
+    }
     __syncthreads();
     // this thread's cell within the stage: [vb, ve), empty when the cell
     // lies wholly before or after it

@@ -11,7 +11,10 @@
 //   pcisph_finalize.wgsl, sph_integrate_3d.wgsl.
 // One deliberate difference: the reorder's within-cell order, which the
 // reference takes from a racing atomicAdd (grid_reorder.wgsl:51), is fixed
-// to ascending particle index (a stable counting sort).
+// to ascending position x (compared as the float32 value's order key, ties
+// by particle index): any order is the reference's, and this one is the
+// HIP build's (its density pass narrows each x-row of cells to the slots
+// within reach).  Parity canonicalises the within-cell order anyway.
 //
 // It is built twice from one template: T=float (compile with
 // -ffp-contract=off so every WGSL operation rounds once, in source order) and
@@ -269,17 +272,32 @@ template <class T> struct Oracle {
     }
   }
 
-  // ---- stage 3: grid_reorder.wgsl:37-63, serial in index order ---------
+  // ---- stage 3: grid_reorder.wgsl:37-63 ---------------------------------
+  // Within a cell by (order key of x as float32, index): the key maps the
+  // float order to unsigned order over every bit pattern.
+  static uint32_t x_key(float x) {
+    uint32_t u;
+    std::memcpy(&u, &x, 4);
+    return u ^ ((u >> 31) ? 0xFFFFFFFFu : 0x80000000u);
+  }
   void reorder() {
+    std::vector<uint64_t> key(n);
+    std::vector<uint32_t> order(n);
     for (uint32_t i = 0; i < n; ++i) {
-      uint32_t ci = pci[i];
-      uint32_t start = starts[ci] - counts[ci];
-      uint32_t slot = offsets[ci]++;
-      uint32_t dest = start + slot;
-      sorted[dest] = particles[i];
-      pci[i] = dest;
-      s2o[dest] = i;
+      key[i] = (uint64_t)x_key((float)particles[i].pos[0]) << 32 | i;
+      order[i] = i;
     }
+    std::sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) {
+      return pci[a] != pci[b] ? pci[a] < pci[b] : key[a] < key[b];
+    });
+    for (uint32_t k = 0; k < n; ++k) {
+      const uint32_t i = order[k];
+      const uint32_t ci = pci[i];
+      offsets[ci] += 1u;
+      sorted[k] = particles[i];
+      s2o[k] = i;
+    }
+    for (uint32_t k = 0; k < n; ++k) pci[s2o[k]] = k;
   }
 
   // ---- stage 4: sph_density_3d_grid.wgsl:86-194 ------------------------

@@ -15,11 +15,16 @@ def small_batch(pkg, count=6, seed=11):
     return pkg.scenes.batch_scenes(count, seed=seed, n_min=1000, n_max=6000)
 
 
-def test_batch_matches_plain_and_oracle(pkg, orc):
+@pytest.mark.parametrize("flags", [0, 256])
+def test_batch_matches_plain_and_oracle(pkg, orc, flags):
+    """flags = FLUID_LIST_SKIP (256) on both sides."""
     scenes = small_batch(pkg)
-    print(f"case: batch of {len(scenes)} scenes, n={[s.n for s in scenes]}", flush=True)
+    print(f"case: batch of {len(scenes)} scenes, n={[s.n for s in scenes]}, flags={flags}",
+          flush=True)
     batch = pkg.make_batch(scenes)
     assert batch.num_scenes() == len(scenes)
+    if flags:
+        batch.set_fusion(batch.fusion() | flags)
     batch.step()
     counts, starts = batch.read_cell_counts(), batch.read_cell_starts()
     s2o = batch.read_sorted_to_orig()
@@ -29,6 +34,8 @@ def test_batch_matches_plain_and_oracle(pkg, orc):
         sl = slice(info["first"], info["first"] + info["count"])
         cl = slice(info["first_cell"], info["first_cell"] + info["cells"])
         plain = pkg.make_sim(sc)
+        if flags:
+            plain.set_fusion(plain.fusion() | flags)
         plain.step()
         np.testing.assert_array_equal(counts[cl], plain.read_cell_counts())
         np.testing.assert_array_equal(starts[cl] - info["first"], plain.read_cell_starts())

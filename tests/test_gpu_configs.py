@@ -4,7 +4,8 @@
     configuration the autotuner picks, and with each of the sweep variants it
     chooses between set explicitly (16-lane density, the 16- and 32-lane list
     walks, the wide first PCISPH iteration, the multi-walker later
-    iterations, the XCD-contiguous tile mapping);
+    iterations, the XCD-contiguous tile mapping, the list skip for
+    candidate-heavy particles);
   * a tilted container (tilt 0.3 about x, -0.2 about z) with the block
     against its walls, stage by stage and as graph steps, so the device-side
     frame changes of density's wall factor, integrate's wall layer and the
@@ -29,6 +30,7 @@ FUSE = 1 | 2  # FLUID_FUSE_XSPH_FORCE | FLUID_FUSE_SOLVE_LIST
 WALK_16, WALK_32 = 4, 8
 SOLVE_WALK_X2, SOLVE_WALK_X4 = 16, 32
 SOLVE_FIRST_X2, SOLVE_FIRST_X4 = 64, 128
+LIST_SKIP = 256
 
 
 def warmed_50k(pkg, orc, warm=60):
@@ -53,7 +55,7 @@ def teacher_forced(sim, o, steps, tag):
         o.sync64()
         one_step_compare(sim, o, f"{tag} step{k}")
         gs, os_, o6 = sim.read_sorted_particles(), o.sorted_particles(), \
-            o.o64.sorted_particles()
+            o.sorted64()
         check_fp(f"{tag} step{k} density", gs[:, 11], os_[:, 11], o6[:, 11])
         check_fp(f"{tag} step{k} near density", gs[:, 12], os_[:, 12], o6[:, 12])
 
@@ -78,6 +80,7 @@ CONFIGS = {
     "d8_s4_walkx2": (8, 4, 0, FUSE | SOLVE_WALK_X2),
     "d4_s2_walkx4_first4": (4, 2, 0, FUSE | SOLVE_WALK_X4 | SOLVE_FIRST_X4),
     "d16_unfused": (16, 8, 0, 0),
+    "d4_s4_skip": (4, 4, 0, FUSE | LIST_SKIP),
 }
 
 
@@ -111,7 +114,7 @@ def test_tilted_container_stagewise(pkg, orc):
     np.testing.assert_array_equal(sim.read_sorted_to_orig(), o.sorted_to_orig())
     sim.step_stages(S["density"], S["density"])
     o.stage(S["density"])
-    gs, os_, o6 = sim.read_sorted_particles(), o.sorted_particles(), o.o64.sorted_particles()
+    gs, os_, o6 = sim.read_sorted_particles(), o.sorted_particles(), o.sorted64()
     # the block sits in the wall layers of three faces: the wall factor is active
     assert np.sum(os_[:, 11] > 1.2 * np.median(os_[:, 11])) > 0
     check_fp("tilted density", gs[:, 11], os_[:, 11], o6[:, 11])

@@ -44,6 +44,20 @@ class Lockstep:
         """Restart the float64 oracle from the float32 oracle's state."""
         self.o64.set_particles(self.o32.particles())
 
+    def _to32(self, a64):
+        """Rows of a float64-oracle array in sorted-slot order, put in the
+        float32 oracle's slot order: the within-cell order follows position
+        x, which the two oracles can round apart."""
+        o2s = np.empty(self.o64.n, dtype=np.int64)
+        o2s[self.o64.sorted_to_orig()] = np.arange(self.o64.n)
+        return a64[o2s[self.o32.sorted_to_orig()]]
+
+    def sorted64(self):
+        return self._to32(self.o64.sorted_particles())
+
+    def predicted64(self, which):
+        return self._to32(self.o64.predicted(which))
+
     def __getattr__(self, name):
         return getattr(self.o32, name)
 
@@ -116,7 +130,7 @@ def test_stagewise_parity(pkg, orc, scene_name, wide):
     gs, os_ = sim.read_sorted_particles(), o.sorted_particles()
     np.testing.assert_array_equal(gs[:, 0:11], os_[:, 0:11])
     sim.step_stages(S["density"], S["density"]); o.stage(S["density"])
-    gs, os_, o64s = sim.read_sorted_particles(), o.sorted_particles(), o.o64.sorted_particles()
+    gs, os_, o64s = sim.read_sorted_particles(), o.sorted_particles(), o.sorted64()
     check_fp("density", gs[:, 11], os_[:, 11], o64s[:, 11])
     check_fp("near_density", gs[:, 12], os_[:, 12], o64s[:, 12])
     check_fp("normal", gs[:, 13:16], os_[:, 13:16], o64s[:, 13:16])
@@ -128,12 +142,12 @@ def test_stagewise_parity(pkg, orc, scene_name, wide):
              o.o64.particles()[:, 8:11])
     sim.step_stages(S["predict"], S["predict"]); o.stage(S["predict"])
     check_fp("predicted x*", sim.read_predicted(0)[:, 0:3], o.predicted(0)[:, 0:3],
-             o.o64.predicted(0)[:, 0:3])
+             o.predicted64(0)[:, 0:3])
     check_fp("predicted v*", sim.read_predicted(0)[:, 4:7], o.predicted(0)[:, 4:7],
-             o.o64.predicted(0)[:, 4:7])
+             o.predicted64(0)[:, 4:7])
     sim.step_stages(S["solve"], S["solve"]); o.stage(S["solve"])
     for w in (0, 1):
-        gp, op, o6 = sim.read_predicted(w), o.predicted(w), o.o64.predicted(w)
+        gp, op, o6 = sim.read_predicted(w), o.predicted(w), o.predicted64(w)
         check_fp(f"solve[{w}] x*", gp[:, 0:3], op[:, 0:3], o6[:, 0:3])
         check_fp(f"solve[{w}] pressure", gp[:, 3], op[:, 3], o6[:, 3])
         check_fp(f"solve[{w}] v*", gp[:, 4:7], op[:, 4:7], o6[:, 4:7])
@@ -169,7 +183,7 @@ def test_graph_step_teacher_forced(pkg, orc, wide):
         check_fp(f"step{k} position", gp[:, 0:3], op[:, 0:3], o6[:, 0:3])
         check_fp(f"step{k} velocity", gp[:, 4:7], op[:, 4:7], o6[:, 4:7])
         check_fp(f"step{k} force", gp[:, 8:11], op[:, 8:11], o6[:, 8:11])
-        gs, os_, o6s = sim.read_sorted_particles(), o.sorted_particles(), o.o64.sorted_particles()
+        gs, os_, o6s = sim.read_sorted_particles(), o.sorted_particles(), o.sorted64()
         check_fp(f"step{k} sorted density", gs[:, 11], os_[:, 11], o6s[:, 11])
     sim.close()
 
@@ -470,7 +484,7 @@ def test_block_1m_full_size(pkg, orc):
     check_fp("1m position", gp[:, 0:3], op[:, 0:3], o6[:, 0:3])
     check_fp("1m velocity", gp[:, 4:7], op[:, 4:7], o6[:, 4:7])
     check_fp("1m force", gp[:, 8:11], op[:, 8:11], o6[:, 8:11])
-    gs, os_, o6s = sim.read_sorted_particles(), o.sorted_particles(), o.o64.sorted_particles()
+    gs, os_, o6s = sim.read_sorted_particles(), o.sorted_particles(), o.sorted64()
     check_fp("1m density", gs[:, 11], os_[:, 11], o6s[:, 11])
     sim.close()
 
@@ -499,6 +513,6 @@ def test_block_4m_full_size(pkg, orc):
     check_fp("4m position", gp[:, 0:3], op[:, 0:3], o6[:, 0:3])
     check_fp("4m velocity", gp[:, 4:7], op[:, 4:7], o6[:, 4:7])
     check_fp("4m force", gp[:, 8:11], op[:, 8:11], o6[:, 8:11])
-    gs, os_, o6s = sim.read_sorted_particles(), o.sorted_particles(), o.o64.sorted_particles()
+    gs, os_, o6s = sim.read_sorted_particles(), o.sorted_particles(), o.sorted64()
     check_fp("4m density", gs[:, 11], os_[:, 11], o6s[:, 11])
     sim.close()

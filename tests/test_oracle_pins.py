@@ -110,8 +110,14 @@ def test_grid_build_scan_reorder_exact(pkg, orc):
     counts = np.bincount(cells, minlength=grid.total_cells).astype(np.uint32)
     np.testing.assert_array_equal(o.cell_counts(), counts)
     np.testing.assert_array_equal(o.cell_starts(), np.cumsum(counts).astype(np.uint32))
-    s2o = np.lexsort((np.arange(sc.n), cells))  # by cell, then original index
+    # by cell, then position x (float32 order key), then original index
+    xb = sc.particles[:, 0].astype(np.float32).view(np.uint32)
+    xkey = np.where(xb >> 31, ~xb, xb | np.uint32(0x80000000))
+    s2o = np.lexsort((np.arange(sc.n), xkey, cells))
     np.testing.assert_array_equal(o.sorted_to_orig(), s2o)
+    srt_x = sc.particles[s2o, 0]
+    same_cell = cells[s2o][1:] == cells[s2o][:-1]
+    assert (srt_x[1:][same_cell] >= srt_x[:-1][same_cell]).all()  # x-sorted rows
     o2s = np.empty_like(s2o)
     o2s[s2o] = np.arange(sc.n)
     np.testing.assert_array_equal(o.particle_cell_indices(), o2s)

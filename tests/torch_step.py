@@ -56,13 +56,16 @@ class Uniforms:
 
 def sort_order(x: np.ndarray, U: Uniforms):
     """(sorted -> original permutation, clamped cell coordinates per sorted
-    slot) with the oracle's f32 cell decision and stable order."""
+    slot) with the oracle's f32 cell decision and within-cell order (by the
+    float32 x's order key, then by index)."""
     xf = x.astype(np.float32)
     c = np.floor((xf - np.array(U.origin, np.float32)) * U.inv_cell).astype(np.int64)
     g = np.array(U.gsize)
     cc = np.clip(c, 0, g - 1)
     cell = cc[:, 0] + cc[:, 1] * g[0] + cc[:, 2] * g[0] * g[1]
-    perm = np.argsort(cell, kind="stable")
+    xb = np.ascontiguousarray(xf[:, 0]).view(np.uint32)
+    xkey = np.where(xb >> 31, ~xb, xb | np.uint32(0x80000000))
+    perm = np.lexsort((np.arange(len(x)), xkey, cell))
     return perm, c[perm], cc[perm]
 
 
