@@ -29,7 +29,8 @@
 //   Grid: counts / starts (inclusive scan) over G^3 cells; cell_of / slot_of
 //   per old slot; tmp = (old slot, original index, x key) per new slot.
 //   Within a cell, slots are ordered by position x (its order key, ties by
-//   original index), so every x-row of cells is one x-sorted slot range.
+//   original index), so every x-row of cells is one x-sorted slot range
+//   and neighbouring slots are close in space.
 #pragma once
 
 #include <hip/hip_runtime.h>

@@ -265,9 +265,9 @@ __global__ __launch_bounds__(256) void k_tiles(Uni un, Bufs b) {
 // Stage 3 — grid_reorder.wgsl:37-63 as a deterministic counting sort.  The
 // reference's within-cell order comes from a racing atomicAdd
 // (grid_reorder.wgsl:51); here it is fixed to ascending position x (the
-// x_key of the position the build binned, ties by original index), so each
-// x-row of cells is one slot range sorted by x and the density pass can
-// narrow a row to the slots within reach of the particle.
+// x_key of the position the build binned, ties by original index): each
+// x-row of cells is one slot range sorted by x, so neighbouring slots are
+// close in space and the list walks' record gathers share cache lines.
 // (a) scatter (old slot, original index, x key) to cell start + provisional
 // slot;
 __global__ __launch_bounds__(256) void k_scatter(Bufs b, uint32_t n) {

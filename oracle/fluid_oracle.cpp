@@ -13,8 +13,7 @@
 // reference takes from a racing atomicAdd (grid_reorder.wgsl:51), is fixed
 // to ascending position x (compared as the float32 value's order key, ties
 // by particle index): any order is the reference's, and this one is the
-// HIP build's (its density pass narrows each x-row of cells to the slots
-// within reach).  Parity canonicalises the within-cell order anyway.
+// HIP build's (neighbouring slots close in space for its gathers).
 //
 // It is built twice from one template: T=float (compile with
 // -ffp-contract=off so every WGSL operation rounds once, in source order) and
