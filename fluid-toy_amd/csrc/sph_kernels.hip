@@ -305,8 +305,8 @@ __global__ __launch_bounds__(256) void k_rank_gather(Bufs b, uint32_t n) {
   }
   __syncthreads();
   const uint32_t lo = s_lo, hi = s_hi;
-  // rank = entries of the cell with a smaller original index; eight
-  // independent reads per trip so they overlap
+  // rank = entries of the cell with a smaller order key (x key, then
+  // original index); eight independent reads per trip so they overlap
   uint32_t r[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   const unsigned long long key = (unsigned long long)e.z << 32 | e.y;
   for (uint32_t c0 = lo; c0 < hi; c0 += kRankLds) {
