@@ -78,7 +78,8 @@ def run_gpu(pkg, scene, warmup: int, steps: int, stage_steps: int):
     sim = pkg.make_sim(scene)
     sim.step(warmup)
     run_gpu.start_state = sim.read_particles()  # the timed region's starting state
-    lanes = sim.autotune(2)  # does not advance the state
+    # does not advance the state; more trials where they are cheap
+    lanes = sim.autotune(4 if scene.n <= 100_000 else 2)
     ms = sim.time_steps(0, steps)
     stages = sim.time_stages(stage_steps)
     run_gpu.kernels = step_kernels(pkg, sim, lanes, 200 if scene.n <= 100_000 else 10)
