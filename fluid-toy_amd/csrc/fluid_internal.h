@@ -27,7 +27,7 @@
 //                       xq = (x*, q) with q = p / max(rho*,1)^2,
 //                       pr = (p, rho*), v = (v*, 0)
 //   Grid: counts / starts (inclusive scan) over G^3 cells; cell_of / slot_of
-//   per old slot; tmp = (old slot, original index, x key) per new slot.
+//   per old slot; tmp = (old slot, original index, x key, cell) per new slot.
 //   Within a cell, slots are ordered by position x (its order key, ties by
 //   original index), so every x-row of cells is one x-sorted slot range
 //   and neighbouring slots are close in space.
@@ -149,7 +149,7 @@ struct Bufs {
   uint32_t *counts, *starts;  // the step's cell counts (as the scan publishes them), inclusive starts
   uint32_t *cwork;            // the build's per-cell counters, zeroed again by the scan
   uint32_t *cell_of, *slot_of;
-  uint4 *tmp;  // reorder entries: (old slot, original index, x order key, 0)
+  uint4 *tmp;  // reorder entries: (old slot, original index, x order key, cell)
   PredBuf pred[2];
   unsigned long long *tile_status;
   uint32_t *tile_counter;

@@ -268,14 +268,14 @@ __global__ __launch_bounds__(256) void k_tiles(Uni un, Bufs b) {
 // x_key of the position the build binned, ties by original index): each
 // x-row of cells is one slot range sorted by x, so neighbouring slots are
 // close in space and the list walks' record gathers share cache lines.
-// (a) scatter (old slot, original index, x key) to cell start + provisional
-// slot;
+// (a) scatter (old slot, original index, x key, cell) to cell start +
+// provisional slot;
 __global__ __launch_bounds__(256) void k_scatter(Bufs b, uint32_t n) {
   const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= n) return;
   const uint32_t ci = b.cell_of[t];
   const uint32_t start = b.starts[ci] - b.counts[ci];
-  b.tmp[start + b.slot_of[t]] = make_uint4(t, b.sid[t], x_key(b.pos[t].x), 0u);
+  b.tmp[start + b.slot_of[t]] = make_uint4(t, b.sid[t], x_key(b.pos[t].x), ci);
 }
 
 // (b) rank every entry by (x key, original index) within its cell segment
@@ -297,7 +297,7 @@ __global__ __launch_bounds__(256) void k_rank_gather(Bufs b, uint32_t n) {
   uint32_t start = 0, end = 0;
   if (u < n) {
     e = b.tmp[u];
-    const uint32_t ci = b.cell_of[e.x];
+    const uint32_t ci = e.w;  // the entry's cell, carried by the scatter
     end = b.starts[ci];
     start = end - b.counts[ci];
     if (u == u0) s_lo = start;
