@@ -103,6 +103,13 @@ struct PairTest {
   }
 };
 
+// First slot of cell c: the inclusive scan's end of the cell before it (the
+// scan runs over the whole concatenated cell array, so across scenes too).
+// One load, where starts[c] - counts[c] reads two arrays.
+__device__ __forceinline__ uint32_t cell_begin(const uint32_t *__restrict__ starts, uint32_t c) {
+  return c ? starts[c - 1u] : 0u;
+}
+
 // Order key of a float: unsigned order of the keys is the float order of
 // the values (-0 before +0), a total order over every bit pattern (NaNs at
 // the two ends).
@@ -190,7 +197,7 @@ __device__ __forceinline__ void sweep27(const DevUniforms *__restrict__ U,
     for (int y = y0; y <= y1; ++y) {
       const uint32_t row = U->cell_offset + (uint32_t)y * U->gx + (uint32_t)z * U->gxy;
       const uint32_t c0 = row + (uint32_t)x0, c1 = row + (uint32_t)x1;
-      const uint32_t begin = starts[c0] - counts[c0];
+      const uint32_t begin = cell_begin(starts, c0);
       const uint32_t end = starts[c1];
       // Full trips address through one running pointer (the four loads use
       // immediate offsets); only a SYNC tail trip clamps its indices.
@@ -289,7 +296,7 @@ __device__ __forceinline__ uint32_t rows27(const DevUniforms *__restrict__ U,
     if (x0 <= x1 && z >= 0 && z < gz && y >= 0 && y < gy) {
       const uint32_t row = U->cell_offset + (uint32_t)y * U->gx + (uint32_t)z * U->gxy;
       const uint32_t c0 = row + (uint32_t)x0, c1 = row + (uint32_t)x1;
-      gb = starts[c0] - counts[c0];
+      gb = cell_begin(starts, c0);
       len = starts[c1] - gb;
     }
     tab[q] = RowEnt{gb, len, gb};

@@ -274,8 +274,7 @@ __global__ __launch_bounds__(256) void k_scatter(Bufs b, uint32_t n) {
   const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= n) return;
   const uint32_t ci = b.cell_of[t];
-  const uint32_t start = b.starts[ci] - b.counts[ci];
-  b.tmp[start + b.slot_of[t]] = make_uint4(t, b.sid[t], x_key(b.pos[t].x), ci);
+  b.tmp[cell_begin(b.starts, ci) + b.slot_of[t]] = make_uint4(t, b.sid[t], x_key(b.pos[t].x), ci);
 }
 
 // (b) rank every entry by (x key, original index) within its cell segment
