@@ -1686,7 +1686,9 @@ int fluid_autotune(fluid_sim *s, uint32_t trials, uint32_t out[3]) {
   }
   // the first PCISPH iteration's sweep lanes per skin-list lane, as whole
   // steps; the wider sweeps regroup the sums, so they must win by 1% (their
-  // measured gains are 0.7-1.5%)
+  // measured gains are 0.7-1.5%), four lanes by 3%: their gain at the tuned
+  // state has not held over the 50k dam break's later flow (1-2% slower over
+  // 1000 steps)
   if (best.solve_list && !best.tiled && best.solve_lanes <= 8) {
     float best_f = 1e30f;
     int sf = 0;
@@ -1699,7 +1701,7 @@ int fluid_autotune(fluid_sim *s, uint32_t trials, uint32_t out[3]) {
         rc = fluid_time_steps(s, 1, 2, &ms);
         tot += ms;
       }
-      if (v == 0 || tot < 0.99f * best_f) {
+      if (v == 0 || tot < (v == 2 ? 0.97f : 0.99f) * best_f) {
         best_f = tot;
         sf = v;
       }
